@@ -1,0 +1,117 @@
+/*
+ * nvq — MI355X-native (gfx950) read-only query path for NanoVDB grids.
+ *
+ * C ABI of libnvq.so.  Plain pointers and sizes only.  Unless a name says
+ * "host_", every pointer argument is a DEVICE pointer.  A `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).  Every call is
+ * stream-ordered and asynchronous.  Every call returns NVQ_OK (0) or a negative
+ * NVQ_ERR_* status.  Nothing throws across the ABI.  The library holds no
+ * global mutable state: calls on distinct streams are thread-safe.  The caller
+ * owns every buffer and keeps the grid alive until work queued on it has
+ * finished.
+ *
+ * The grid is an unmodified NanoVDB buffer (magic "NanoVDB1", ABI major 32),
+ * as written by NanoVDB's own tools (createLevelSetSphere, createNanoGrid, …)
+ * or by this library's host builder (nvq_build.h).  The kernels read it by
+ * byte offset; NanoVDB.h is never compiled for the device.
+ *
+ * Reference citations are relative to the reference tree (AcademySoftwareFoundation/openvdb).
+ */
+#ifndef NVQ_H
+#define NVQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NVQ_ABI_VERSION 1
+
+enum nvq_status {
+    NVQ_OK = 0,
+    NVQ_ERR_INVALID_ARG = -1,     /* NULL pointer, n too large, bad mode */
+    NVQ_ERR_BAD_MAGIC = -2,       /* GridData::isValid magic check, NanoVDB.h:2002-2013 */
+    NVQ_ERR_BAD_VERSION = -3,     /* major version != 32, NanoVDB.h:729-769 */
+    NVQ_ERR_UNSUPPORTED_TYPE = -4, /* grid type not Float/Fp4/Fp8/Fp16/FpN, or op needs Float */
+    NVQ_ERR_MISALIGNED = -5,      /* buffer not 32-byte aligned, NanoVDB.h:69-74 */
+    NVQ_ERR_TRUNCATED = -6,       /* buffer shorter than GridData::mGridSize / node offsets out of range */
+    NVQ_ERR_HIP = -7,             /* a HIP runtime call failed */
+    NVQ_ERR_NO_DEVICE = -8        /* no usable gfx950 device */
+};
+
+/* NanoVDB GridType values the kernels decode (NanoVDB.h:219-247) */
+enum nvq_grid_type { NVQ_FLOAT = 1, NVQ_FP4 = 13, NVQ_FP8 = 14, NVQ_FP16 = 15, NVQ_FPN = 16 };
+
+typedef struct nvq_grid nvq_grid; /* opaque device-resident grid */
+
+typedef struct nvq_grid_info {
+    uint32_t grid_type;   /* enum nvq_grid_type */
+    uint32_t grid_class;  /* NanoVDB GridClass: 1 LevelSet, 2 FogVolume, ... */
+    uint64_t bytes;       /* buffer size */
+    uint32_t node_count[3]; /* leaf, lower, upper */
+    uint32_t root_tiles;
+    int32_t index_bbox[6];  /* RootData::mBBox (min xyz, max xyz) */
+    double voxel_size[3];
+    float background;
+    uint32_t pad;
+    const void* device_buffer; /* the flat buffer in device memory */
+} nvq_grid_info;
+
+/* Human-readable text for a status code (static storage). */
+const char* nvq_status_string(int status);
+
+/* Validate a host NanoVDB buffer and upload it with one hipMalloc + one
+ * hipMemcpyAsync on `stream`.  Replaces GridHandle<BufferT>::deviceUpload +
+ * deviceGrid<float>() (GridHandle.h:159-194, cuda/DeviceBuffer.h:422-437); where
+ * deviceGrid<T>() returns nullptr on a type mismatch, this returns
+ * NVQ_ERR_UNSUPPORTED_TYPE / NVQ_ERR_BAD_MAGIC / ... and leaves *out NULL. */
+int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid** out);
+/* Free the device copy (waits for the device). */
+int nvq_grid_free(nvq_grid* grid);
+int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info);
+
+/* Batched ReadAccessor::getValue / isActive / getDim (NanoVDB.h:5370-5377, 5431-5447)
+ * for n integer voxels ijk[n][3].  Any of val / active / dim may be NULL.
+ * dim follows ReadAccessor::getDim: 1 inside a leaf, 8 / 128 / 4096 for tiles and background. */
+int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* val, uint8_t* active,
+                  uint32_t* dim, void* stream);
+
+/* Batched point queries at xyz[n][3] (index space if world == 0, else world space mapped with
+ * Grid::worldToIndex in FP64 as the reference CUDA op does, python/cuda/PySampleFromVoxels.cu:23,38).
+ *   nn   : SampleFromVoxels<Acc,0>   (getValue at Round(xyz)),  math/SampleFromVoxels.h:149-156
+ *   val  : SampleFromVoxels<Acc,1>   (trilinear),               math/SampleFromVoxels.h:423-464
+ *   grad : grad_mode 1 = FD6 of trilinear * 0.5/voxelSize        python/cuda/PySampleFromVoxels.cu:42-47
+ *          grad_mode 2 = GradStencil::gradient at RoundDown(xyz)  math/Stencils.h:502-507
+ *          grad_mode 3 = TrilinearSampler::gradient (index space) math/SampleFromVoxels.h:232-271
+ * nn / val / grad may be NULL.  Replaces the nanobind op sampleFromVoxels(points, d_grid, values[, gradients]). */
+int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
+               int grad_mode, void* stream);
+
+/* GradStencil at integer voxels (math/Stencils.h:455-576): gradient, normSqGrad, laplacian,
+ * zeroCrossing.  Float grids only.  Outputs may be NULL. */
+int nvq_grad_stencil(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* grad, float* norm_sq, float* lap,
+                     uint8_t* zero_crossing, void* stream);
+
+/* World-space rays eye[n][3], dir[n][3] -> Ray::worldToIndexF -> math::zeroCrossing
+ * (math/Ray.h:258-268, math/HDDA.h:189-214).  Outputs per ray: hit (0/1), ijk[3], t (index
+ * space), v (value at ijk), normal[3] (normalised world-space trilinear gradient at ray(t)).
+ * Misses write zeros.  Any output may be NULL.  If image_width > 0 the rays are a row-major
+ * image of that width (n a multiple of it) and are scheduled in 8x8 pixel tiles per wavefront;
+ * results are written at the input index either way. */
+int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                      uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream);
+
+/* Emission-absorption march of world-space rays through a Float fog volume.  HDDA over the
+ * tree at leaf-node granularity skips inactive tiles and background (math/HDDA.h:28-179,
+ * 450-536); inside leaves and active tiles the ray is sampled trilinearly at globally aligned
+ * steps t = k*dt; per step a = exp(-sigma_t*rho*dt), L += T*(1-a)*rho, T *= a (the EA integrator
+ * of openvdb/openvdb/tools/RayTracer.h:993-1064 with density emission).  radiance / transmittance
+ * may be NULL.  image_width as for nvq_zero_crossing. */
+int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                  float dt, float sigma_t, float* radiance, float* transmittance, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NVQ_H */

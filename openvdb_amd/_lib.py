@@ -1,0 +1,75 @@
+"""ctypes binding of libnvq.so (the C ABI declared in include/nvq.h).
+
+The shared library is built in-tree by ``openvdb_amd.build.build()``.  There
+is no fallback path: if the library is missing or fails to load, every entry
+point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnvq.so")
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_u32 = ctypes.c_uint32
+_i = ctypes.c_int
+_f = ctypes.c_float
+
+
+class GridInfo(ctypes.Structure):
+    _fields_ = [
+        ("grid_type", ctypes.c_uint32),
+        ("grid_class", ctypes.c_uint32),
+        ("bytes", ctypes.c_uint64),
+        ("node_count", ctypes.c_uint32 * 3),
+        ("root_tiles", ctypes.c_uint32),
+        ("index_bbox", ctypes.c_int32 * 6),
+        ("voxel_size", ctypes.c_double * 3),
+        ("background", ctypes.c_float),
+        ("pad", ctypes.c_uint32),
+        ("device_buffer", ctypes.c_void_p),
+    ]
+
+
+# (name, argtypes) of every entry point the header declares
+SIGNATURES = {
+    "nvq_status_string": ([_i], ctypes.c_char_p),
+    "nvq_grid_upload": ([_vp, _u64, _vp, ctypes.POINTER(_vp)], _i),
+    "nvq_grid_free": ([_vp], _i),
+    "nvq_grid_get_info": ([_vp, ctypes.POINTER(GridInfo)], _i),
+    "nvq_get_value": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp], _i),
+    "nvq_sample": ([_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _vp], _i),
+    "nvq_grad_stencil": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp], _i),
+    "nvq_zero_crossing": ([_vp, _vp, _vp, _u64, _u32, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "nvq_fog_march": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp], _i),
+}
+
+_lib = None
+
+
+class NvqError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().nvq_status_string(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else f"{msg} (status {status})")
+
+
+def lib():
+    """Load libnvq.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run openvdb_amd.build.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(status, what=""):
+    if status != 0:
+        raise NvqError(status, what)

@@ -1,0 +1,431 @@
+// Device-side read path over a flat NanoVDB buffer for gfx950.
+//
+// The buffer is read by byte offset (NanoVDB.h is never compiled for the
+// device).  Layout: nanovdb/nanovdb/NanoVDB.h GridData 1944-2135, Map
+// 1418-1587, TreeData 2393-2423, RootData 2621-2840, InternalData 3181-3250,
+// LeafData 3671-3973.  Every float expression keeps the reference's operation
+// order and the file is compiled with -ffp-contract=off; fmaf is written out
+// exactly where the reference writes fmaf, so hit voxels, active states and
+// lookups are bit-identical to the reference host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nvq {
+
+// ------------------------------------------------------------------ layout
+constexpr uint32_t kGridMap = 296, kGridVoxelSize = 608, kGridClass = 632, kGridType = 636, kGridSize = 672;
+constexpr uint32_t kMapInvMatF = 36, kMapVecF = 72, kMapInvMatD = 160, kMapVecD = 232;
+constexpr uint32_t kRootBBox = 0, kRootTableSize = 24, kRootBackground = 28, kRootSize = 64, kTileSize = 32;
+constexpr uint32_t kUpFlags = 24, kUpVMask = 32, kUpCMask = 4128, kUpTable = 8256, kUpSize = 270400;
+constexpr uint32_t kLoFlags = 24, kLoVMask = 32, kLoCMask = 544, kLoTable = 1088, kLoSize = 33856;
+constexpr uint32_t kLeafFlags = 15, kLeafVMask = 16, kLeafMin = 80, kLeafQuantum = 84, kLeafData = 96;
+enum GridTypeId : uint32_t { GT_FLOAT = 1, GT_FP4 = 13, GT_FP8 = 14, GT_FP16 = 15, GT_FPN = 16 };
+
+// Host FLT_MAX: the reference's Maximum<float> on the host (math/Math.h:118-124);
+// the reference's own HIP branch would use 1e38 (:112-115), which would change
+// which rays are clipped, so the host value is used.
+constexpr float kFltMax = 3.40282346638528859812e+38F;
+
+// Kernel-argument view of an uploaded grid (all pointers are device pointers).
+struct GridView {
+    const uint8_t* root;  // RootData
+    const uint8_t* leaf0; // first leaf node
+    uint32_t tableSize;
+    uint32_t type;
+    float background;
+    int32_t bbox[6];      // RootData::mBBox
+    float invMatF[9], vecF[3];
+    double invMatD[9], vecD[3];
+    double voxelSize[3];
+    uint32_t leafCount;
+    uint32_t leafSize;    // 0 for FpN (variable)
+};
+
+__device__ __forceinline__ uint64_t ld_u64(const uint8_t* p) { return *reinterpret_cast<const uint64_t*>(p); }
+__device__ __forceinline__ int64_t ld_i64(const uint8_t* p) { return *reinterpret_cast<const int64_t*>(p); }
+__device__ __forceinline__ uint32_t ld_u32(const uint8_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
+__device__ __forceinline__ float ld_f32(const uint8_t* p) { return *reinterpret_cast<const float*>(p); }
+__device__ __forceinline__ uint32_t ld_u16(const uint8_t* p) { return *reinterpret_cast<const uint16_t*>(p); }
+__device__ __forceinline__ uint32_t ld_u8(const uint8_t* p) { return *p; }
+
+__device__ __forceinline__ bool maskOn(const uint8_t* mask, uint32_t n)
+{
+    return (ld_u32(mask + 4 * (n >> 5)) >> (n & 31)) & 1u;
+}
+
+// ------------------------------------------------------------------ leaf codecs
+// float NanoVDB.h:3703; Fp4 :3839-3847; Fp8 :3875-3878; Fp16 :3906-3909; FpN :3937-3951.
+// Quantised values are float(code)*quantum + minimum with no fused multiply-add.
+template<uint32_t GT> struct Codec;
+template<> struct Codec<GT_FLOAT> {
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n) { return ld_f32(leaf + kLeafData + 4 * n); }
+};
+__device__ __forceinline__ float dequant(const uint8_t* leaf, uint32_t code)
+{
+    return __fadd_rn(__fmul_rn(float(code), ld_f32(leaf + kLeafQuantum)), ld_f32(leaf + kLeafMin));
+}
+template<> struct Codec<GT_FP4> {
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n)
+    {
+        return dequant(leaf, (ld_u8(leaf + kLeafData + (n >> 1)) >> ((n & 1) << 2)) & 15u);
+    }
+};
+template<> struct Codec<GT_FP8> {
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n) { return dequant(leaf, ld_u8(leaf + kLeafData + n)); }
+};
+template<> struct Codec<GT_FP16> {
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n) { return dequant(leaf, ld_u16(leaf + kLeafData + 2 * n)); }
+};
+template<> struct Codec<GT_FPN> {
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n)
+    {
+        const uint32_t b = ld_u8(leaf + kLeafFlags) >> 5; // log2 of the bit width
+        uint32_t code = ld_u32(leaf + kLeafData + 4 * (n >> (5 - b)));
+        code >>= (n & ((32u >> b) - 1u)) << b;
+        code &= (1u << (1u << b)) - 1u;
+        return dequant(leaf, code);
+    }
+};
+
+// ------------------------------------------------------------------ accessor
+// Returned values and getDim results do not depend on the cache state
+// (NanoVDB.h:5379-5403), so a single-key three-level cache (the
+// NANOVDB_USE_SINGLE_ACCESSOR_KEY form, NanoVDB.h:5329-5348) is used: one
+// Coord key plus three node pointers per lane.
+struct Probe {
+    float value;
+    uint32_t dim;       // ReadAccessor::getDim semantics
+    const uint8_t* leaf; // leaf holding the voxel or nullptr
+    uint32_t n;         // voxel offset in the leaf
+    bool active;
+};
+
+template<uint32_t GT>
+struct Accessor {
+    const uint8_t* root;
+    uint32_t tableSize;
+    float background;
+    int32_t kx, ky, kz;
+    const uint8_t* upper;
+    const uint8_t* lower;
+    const uint8_t* leaf;
+
+    __device__ __forceinline__ explicit Accessor(const GridView& grid)
+        : root(grid.root), tableSize(grid.tableSize), background(grid.background), kx(0x7fffffff), ky(0x7fffffff),
+          kz(0x7fffffff), upper(nullptr), lower(nullptr), leaf(nullptr) {}
+
+    __device__ __forceinline__ static uint32_t leafOffset(int32_t x, int32_t y, int32_t z)
+    {
+        return ((uint32_t(x) & 7u) << 6) | ((uint32_t(y) & 7u) << 3) | (uint32_t(z) & 7u);
+    }
+
+    __device__ __forceinline__ void fromLeaf(Probe& r, int32_t x, int32_t y, int32_t z) const
+    {
+        const uint32_t n = leafOffset(x, y, z);
+        r.leaf = leaf;
+        r.n = n;
+        r.value = Codec<GT>::get(leaf, n);
+        r.active = maskOn(leaf + kLeafVMask, n);
+        r.dim = (ld_u8(leaf + kLeafFlags) & 1u) ? 8u : 1u;
+    }
+
+    // lower node: InternalNode<Leaf,4>::getAndCache / getDimAndCache, NanoVDB.h:3619-3647
+    __device__ __forceinline__ void fromLower(Probe& r, int32_t x, int32_t y, int32_t z)
+    {
+        const uint32_t n = (((uint32_t(x) & 127u) >> 3) << 8) | (((uint32_t(y) & 127u) >> 3) << 4) | ((uint32_t(z) & 127u) >> 3);
+        const uint64_t entry = ld_u64(lower + kLoTable + 8 * n);
+        if (maskOn(lower + kLoCMask, n)) {
+            leaf = lower + int64_t(entry);
+            kx = x; ky = y; kz = z;
+            fromLeaf(r, x, y, z);
+            if (ld_u64(lower + kLoFlags) & 1u) {
+                // render-skip flag: getDimAndCache stops at this node (NanoVDB.h:3622); keep the
+                // leaf out of the cache so later getDim calls see the flag again
+                r.dim = 128u;
+                leaf = nullptr;
+            }
+        } else {
+            leaf = nullptr;
+            r.leaf = nullptr;
+            r.n = 0;
+            r.value = __uint_as_float(uint32_t(entry));
+            r.active = maskOn(lower + kLoVMask, n);
+            r.dim = (ld_u64(lower + kLoFlags) & 1u) ? 128u : 8u;
+        }
+    }
+
+    // upper node: InternalNode<Lower,5>, NanoVDB.h:3619-3647
+    __device__ __forceinline__ void fromUpper(Probe& r, int32_t x, int32_t y, int32_t z)
+    {
+        const uint32_t n = (((uint32_t(x) & 4095u) >> 7) << 10) | (((uint32_t(y) & 4095u) >> 7) << 5) | ((uint32_t(z) & 4095u) >> 7);
+        const uint64_t entry = ld_u64(upper + kUpTable + 8 * n);
+        if (maskOn(upper + kUpCMask, n)) {
+            lower = upper + int64_t(entry);
+            kx = x; ky = y; kz = z;
+            fromLower(r, x, y, z);
+        } else {
+            lower = nullptr;
+            leaf = nullptr;
+            r.leaf = nullptr;
+            r.n = 0;
+            r.value = __uint_as_float(uint32_t(entry));
+            r.active = maskOn(upper + kUpVMask, n);
+            r.dim = 128u;
+        }
+        if (ld_u64(upper + kUpFlags) & 1u) {
+            r.dim = 4096u;
+            lower = nullptr;
+            leaf = nullptr;
+        }
+    }
+
+    // root: CoordToKey (single 64-bit key, NanoVDB.h:2633-2640) + linear tile probe (:2785-2799)
+    __device__ __forceinline__ void fromRoot(Probe& r, int32_t x, int32_t y, int32_t z)
+    {
+        const uint64_t key = uint64_t(uint32_t(z) >> 12) | (uint64_t(uint32_t(y) >> 12) << 21) |
+                             (uint64_t(uint32_t(x) >> 12) << 42);
+        const uint8_t* tile = nullptr;
+        for (uint32_t i = 0; i < tableSize; ++i) {
+            const uint8_t* t = root + kRootSize + kTileSize * i;
+            if (ld_u64(t) == key) { tile = t; break; }
+        }
+        upper = nullptr;
+        lower = nullptr;
+        leaf = nullptr;
+        r.leaf = nullptr;
+        r.n = 0;
+        r.dim = 4096u;
+        if (!tile) {
+            r.value = background;
+            r.active = false;
+            return;
+        }
+        const int64_t child = ld_i64(tile + 8);
+        if (child == 0) {
+            r.value = ld_f32(tile + 20);
+            r.active = ld_u32(tile + 16) > 0u;
+            return;
+        }
+        upper = root + child;
+        kx = x; ky = y; kz = z;
+        fromUpper(r, x, y, z);
+    }
+
+    __device__ __forceinline__ Probe probe(int32_t x, int32_t y, int32_t z)
+    {
+        Probe r;
+        const uint32_t dirty = uint32_t((x ^ kx) | (y ^ ky) | (z ^ kz));
+        if (leaf && !(dirty & ~7u)) fromLeaf(r, x, y, z);
+        else if (lower && !(dirty & ~127u)) fromLower(r, x, y, z);
+        else if (upper && !(dirty & ~4095u)) fromUpper(r, x, y, z);
+        else fromRoot(r, x, y, z);
+        return r;
+    }
+
+    __device__ __forceinline__ float getValue(int32_t x, int32_t y, int32_t z)
+    {
+        const uint32_t dirty = uint32_t((x ^ kx) | (y ^ ky) | (z ^ kz));
+        if (leaf && !(dirty & ~7u)) return Codec<GT>::get(leaf, leafOffset(x, y, z));
+        return probe(x, y, z).value;
+    }
+};
+
+// ------------------------------------------------------------------ samplers
+// lerp a + w*(b-a), math/SampleFromVoxels.h:423-464
+__device__ __forceinline__ float lerpf(float a, float b, float w) { return a + w * (b - a); }
+
+struct Cell { float v[2][2][2]; };
+
+template<class AccT>
+__device__ __forceinline__ void stencil8(AccT& acc, int32_t i, int32_t j, int32_t k, Cell& c)
+{
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) c.v[a][b][e] = acc.getValue(i + a, j + b, k + e);
+}
+
+// c.v[a][b][e] for a,b,e in {0,1} without dynamic register indexing
+__device__ __forceinline__ float pick(const Cell& c, int32_t a, int32_t b, int32_t e)
+{
+    const float x0 = b ? (e ? c.v[0][1][1] : c.v[0][1][0]) : (e ? c.v[0][0][1] : c.v[0][0][0]);
+    const float x1 = b ? (e ? c.v[1][1][1] : c.v[1][1][0]) : (e ? c.v[1][0][1] : c.v[1][0][0]);
+    return a ? x1 : x0;
+}
+
+__device__ __forceinline__ float trilerp(const Cell& s, float u, float v, float w)
+{
+    return lerpf(lerpf(lerpf(s.v[0][0][0], s.v[0][0][1], w), lerpf(s.v[0][1][0], s.v[0][1][1], w), v),
+                 lerpf(lerpf(s.v[1][0][0], s.v[1][0][1], w), lerpf(s.v[1][1][0], s.v[1][1][1], w), v), u);
+}
+
+// Floor<Coord>(Vec3f&) + fused trilinear, math/SampleFromVoxels.h:61-69, 423-464
+template<class AccT>
+__device__ __forceinline__ float trilinear(AccT& acc, float x, float y, float z)
+{
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    Cell c;
+    stencil8(acc, int32_t(fx), int32_t(fy), int32_t(fz), c);
+    return trilerp(c, x - fx, y - fy, z - fz);
+}
+
+// TrilinearSampler::gradient, math/SampleFromVoxels.h:232-271 (index space)
+__device__ __forceinline__ void trilinearGradient(const Cell& s, float u, float v, float w, float out[3])
+{
+    float D[4] = {s.v[0][0][1] - s.v[0][0][0], s.v[0][1][1] - s.v[0][1][0], s.v[1][0][1] - s.v[1][0][0],
+                  s.v[1][1][1] - s.v[1][1][0]};
+    out[2] = lerpf(lerpf(D[0], D[1], v), lerpf(D[2], D[3], v), u);
+    D[0] = s.v[0][0][0] + D[0] * w;
+    D[1] = s.v[0][1][0] + D[1] * w;
+    D[2] = s.v[1][0][0] + D[2] * w;
+    D[3] = s.v[1][1][0] + D[3] * w;
+    out[0] = lerpf(D[2], D[3], v) - lerpf(D[0], D[1], v);
+    out[1] = lerpf(D[1] - D[0], D[3] - D[2], u);
+}
+
+// ------------------------------------------------------------------ Ray / HDDA
+// math/Ray.h:64-74, 154-163, 258-268, 398-424; math/HDDA.h:28-179
+struct Ray {
+    float eye[3], dir[3], inv[3], t0, t1;
+};
+
+__device__ __forceinline__ void rayAt(const Ray& r, float t, float p[3])
+{
+    p[0] = __builtin_fmaf(t, r.dir[0], r.eye[0]);
+    p[1] = __builtin_fmaf(t, r.dir[1], r.eye[1]);
+    p[2] = __builtin_fmaf(t, r.dir[2], r.eye[2]);
+}
+
+__device__ __forceinline__ Ray makeRay(const float e[3], const float d[3], float t0, float t1)
+{
+    Ray r;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r.eye[a] = e[a];
+        r.dir[a] = d[a];
+        r.inv[a] = 1.0f / d[a];
+    }
+    r.t0 = t0;
+    r.t1 = t1;
+    return r;
+}
+
+// Ray::worldToIndexF: Map::applyInverseMapF / applyInverseJacobianF (fmaf form, math/Math.h:1704-1709)
+__device__ __forceinline__ Ray worldToIndexRay(const GridView& g, const float e[3], const float d[3])
+{
+    const float t[3] = {e[0] - g.vecF[0], e[1] - g.vecF[1], e[2] - g.vecF[2]};
+    float eye[3], dir[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        eye[r] = __builtin_fmaf(t[0], g.invMatF[3 * r], __builtin_fmaf(t[1], g.invMatF[3 * r + 1], t[2] * g.invMatF[3 * r + 2]));
+        dir[r] = __builtin_fmaf(d[0], g.invMatF[3 * r], __builtin_fmaf(d[1], g.invMatF[3 * r + 1], d[2] * g.invMatF[3 * r + 2]));
+    }
+    const float length = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const float invLength = 1.0f / length;
+    const float nd[3] = {invLength * dir[0], invLength * dir[1], invLength * dir[2]};
+    return makeRay(eye, nd, length * 1.0e-5f, kFltMax);
+}
+
+// Ray::clip(CoordBBox) with the +1 max padding
+__device__ __forceinline__ bool rayClip(Ray& r, const int32_t* bbox)
+{
+    float t0 = r.t0, t1 = r.t1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float a = float(bbox[i]), b = float(bbox[3 + i] + 1);
+        if (a >= b) return false;
+        a = (a - r.eye[i]) * r.inv[i];
+        b = (b - r.eye[i]) * r.inv[i];
+        if (a > b) { const float tmp = a; a = b; b = tmp; }
+        if (a > t0) t0 = a;
+        if (b < t1) t1 = b;
+        if (t0 > t1) return false;
+    }
+    r.t0 = t0;
+    r.t1 = t1;
+    return true;
+}
+
+__device__ __forceinline__ int32_t ifloor(float x) { return int32_t(floorf(x)); }
+
+struct HDDA {
+    int32_t dim;
+    float t0, t1;
+    int32_t voxel[3], step[3];
+    float delta[3], next[3];
+
+    __device__ __forceinline__ void init(const Ray& r, float startTime, float maxTime, int32_t d)
+    {
+        dim = d;
+        t0 = startTime;
+        t1 = maxTime;
+        float pos[3];
+        rayAt(r, t0, pos);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            voxel[a] = ifloor(pos[a]) & ~(d - 1);
+            delta[a] = 0.0f;
+            if (r.dir[a] == 0.0f) {
+                next[a] = kFltMax;
+                step[a] = 0;
+            } else if (r.inv[a] > 0.0f) {
+                step[a] = 1;
+                next[a] = t0 + (float(voxel[a] + d) - pos[a]) * r.inv[a];
+                delta[a] = r.inv[a];
+            } else {
+                step[a] = -1;
+                next[a] = t0 + (float(voxel[a]) - pos[a]) * r.inv[a];
+                delta[a] = -r.inv[a];
+            }
+        }
+    }
+
+    __device__ __forceinline__ bool update(const Ray& r, int32_t d)
+    {
+        if (dim == d) return false;
+        int32_t vmax[3], vmin[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vmax[a] = (voxel[a] + (dim - 1)) & ~(d - 1);
+            vmin[a] = voxel[a] & ~(d - 1);
+        }
+        dim = d;
+        float pos[3];
+        rayAt(r, t0, pos);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int32_t v = ifloor(pos[a]) & ~(d - 1);
+            v = v < vmax[a] ? v : vmax[a];
+            v = v > vmin[a] ? v : vmin[a];
+            voxel[a] = v;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (step[a] == 0) continue;
+            next[a] = t0 + (float(voxel[a]) - pos[a]) * r.inv[a];
+            if (step[a] > 0) next[a] += float(d) * r.inv[a];
+        }
+        return true;
+    }
+
+    __device__ __forceinline__ bool stepOnce()
+    {
+        // MinIndex, math/Math.h:294-309
+        const int a = (next[0] < next[1] && next[0] < next[2]) ? 0 : (next[1] < next[2] ? 1 : 2);
+        float na = a == 0 ? next[0] : (a == 1 ? next[1] : next[2]);
+        const float da = a == 0 ? delta[0] : (a == 1 ? delta[1] : delta[2]);
+        const int32_t sa = a == 0 ? step[0] : (a == 1 ? step[1] : step[2]);
+        if (na <= t0) na += t0 - 0.999999f * na + 1.0e-6f; // ENFORCE_FORWARD_STEPPING, math/HDDA.h:163-169
+        t0 = na;
+        na += float(dim) * da;
+        if (a == 0) { next[0] = na; voxel[0] += dim * sa; }
+        else if (a == 1) { next[1] = na; voxel[1] += dim * sa; }
+        else { next[2] = na; voxel[2] += dim * sa; }
+        return t0 <= t1;
+    }
+};
+
+} // namespace nvq

@@ -1,0 +1,669 @@
+// gfx950 kernels and the C ABI of libnvq.so (declared in include/nvq.h).
+//
+// One lane per query / ray.  Every kernel templates on the leaf codec so the
+// decode is branch-free in the inner loops.  Ray kernels schedule 8x8 pixel
+// tiles per 64-wide wavefront and remap workgroups so that each of the eight
+// XCDs (hardware dispatch is round-robin over XCDs) walks a contiguous band of
+// tiles and keeps its own L2 warm with the nodes that band touches.
+#include "nvq_device.h"
+#include "../../include/nvq.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstring>
+#include <new>
+
+using namespace nvq;
+
+struct nvq_grid {
+    void* dbuf;
+    uint64_t bytes;
+    int device;
+    GridView view;
+    nvq_grid_info info;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------ scheduling
+struct Sched {
+    uint64_t n;
+    uint32_t width, height, tilesX;
+    uint64_t tiles;
+    uint32_t blocks;  // logical blocks
+    uint32_t perXcd;  // logical blocks per XCD
+};
+
+Sched makeSched(uint64_t n, uint32_t width)
+{
+    Sched s{};
+    s.n = n;
+    s.width = width;
+    if (width) {
+        s.height = uint32_t(n / width);
+        s.tilesX = (width + 7) / 8;
+        s.tiles = uint64_t(s.tilesX) * ((s.height + 7) / 8);
+        s.blocks = uint32_t((s.tiles + (kBlock / 64) - 1) / (kBlock / 64));
+    } else {
+        s.blocks = uint32_t((n + kBlock - 1) / kBlock);
+    }
+    s.perXcd = (s.blocks + 7) / 8;
+    return s;
+}
+
+__device__ __forceinline__ bool schedule(const Sched& s, uint64_t& idx)
+{
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = (b & 7u) * s.perXcd + (b >> 3);
+    if (lb >= s.blocks) return false;
+    if (s.width == 0) {
+        idx = uint64_t(lb) * kBlock + threadIdx.x;
+        return idx < s.n;
+    }
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t tile = uint64_t(lb) * (kBlock / 64) + wave;
+    if (tile >= s.tiles) return false;
+    const uint32_t tx = uint32_t(tile % s.tilesX), ty = uint32_t(tile / s.tilesX);
+    const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
+    if (px >= s.width || py >= s.height) return false;
+    idx = uint64_t(py) * s.width + px;
+    return true;
+}
+
+inline uint32_t launchBlocks(const Sched& s) { return s.perXcd * 8u; }
+
+// ------------------------------------------------------------------ getValue
+template<uint32_t GT>
+__global__ __launch_bounds__(kBlock) void k_get_value(GridView g, const int32_t* __restrict__ ijk, uint64_t n,
+                                                      float* __restrict__ val, uint8_t* __restrict__ active,
+                                                      uint32_t* __restrict__ dim)
+{
+    Accessor<GT> acc(g);
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        const Probe p = acc.probe(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+        if (val) val[i] = p.value;
+        if (active) active[i] = p.active ? 1 : 0;
+        if (dim) dim[i] = p.dim;
+    }
+}
+
+// ------------------------------------------------------------------ point sampling
+// Grid::worldToIndex for a Vec3f (FP64 Map, NanoVDB.h:1507-1510, math/Math.h:1718-1723)
+__device__ __forceinline__ void worldToIndexF64(const GridView& g, float p[3])
+{
+    float t[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) t[a] = float(double(p[a]) - g.vecD[a]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        p[r] = float(__builtin_fma(double(t[0]), g.invMatD[3 * r],
+                                   __builtin_fma(double(t[1]), g.invMatD[3 * r + 1], double(t[2]) * g.invMatD[3 * r + 2])));
+}
+
+// One trilinear sample of the FD6 stencil at q, served from the 32-voxel plus-shaped block
+// around (i0,j0,k0) when its cell is the expected one, else from the accessor.
+template<uint32_t GT>
+__device__ __forceinline__ float fd6Sample(Accessor<GT>& acc, float qx, float qy, float qz, int32_t ei, int32_t ej,
+                                           int32_t ek, const Cell& blk)
+{
+    const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+    const int32_t bi = int32_t(fx), bj = int32_t(fy), bk = int32_t(fz);
+    if (bi == ei && bj == ej && bk == ek) return trilerp(blk, qx - fx, qy - fy, qz - fz);
+    Cell c;
+    stencil8(acc, bi, bj, bk, c);
+    return trilerp(c, qx - fx, qy - fy, qz - fz);
+}
+
+template<uint32_t GT>
+__global__ __launch_bounds__(kBlock) void k_sample(GridView g, const float* __restrict__ xyz, uint64_t n, int world,
+                                                   float* __restrict__ nn, float* __restrict__ val,
+                                                   float* __restrict__ grad, int mode, float fdx, float fdy,
+                                                   float fdz, float gsInv2dx)
+{
+    Accessor<GT> acc(g);
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        float p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        if (world) worldToIndexF64(g, p);
+        const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+        const int32_t i0 = int32_t(fx), j0 = int32_t(fy), k0 = int32_t(fz);
+        const float u = p[0] - fx, v = p[1] - fy, w = p[2] - fz;
+        Cell C;
+        stencil8(acc, i0, j0, k0, C);
+        if (nn) {
+            // SampleFromVoxels<Acc,0>: getValue(Round(xyz)); Round uses rintf, which lands in this cell
+            const int32_t ri = int32_t(rintf(p[0])) - i0, rj = int32_t(rintf(p[1])) - j0, rk = int32_t(rintf(p[2])) - k0;
+            nn[i] = ((ri | rj | rk) & ~1) ? acc.getValue(i0 + ri, j0 + rj, k0 + rk) : pick(C, ri, rj, rk);
+        }
+        if (val) val[i] = trilerp(C, u, v, w);
+        if (!grad) continue;
+        float o[3] = {0.0f, 0.0f, 0.0f};
+        if (mode == 1) {
+            // FD6 of trilinear samples, python/cuda/PySampleFromVoxels.cu:42-47
+            Cell B;
+            // +x / -x
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    B.v[0][b][e] = C.v[1][b][e];
+                    B.v[1][b][e] = acc.getValue(i0 + 2, j0 + b, k0 + e);
+                }
+            const float xp = fd6Sample(acc, p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f, i0 + 1, j0, k0, B);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    B.v[1][b][e] = C.v[0][b][e];
+                    B.v[0][b][e] = acc.getValue(i0 - 1, j0 + b, k0 + e);
+                }
+            const float xm = fd6Sample(acc, p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f, i0 - 1, j0, k0, B);
+            // +y / -y
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    B.v[a][0][e] = C.v[a][1][e];
+                    B.v[a][1][e] = acc.getValue(i0 + a, j0 + 2, k0 + e);
+                }
+            const float yp = fd6Sample(acc, p[0] + 0.0f, p[1] + 1.0f, p[2] + 0.0f, i0, j0 + 1, k0, B);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    B.v[a][1][e] = C.v[a][0][e];
+                    B.v[a][0][e] = acc.getValue(i0 + a, j0 - 1, k0 + e);
+                }
+            const float ym = fd6Sample(acc, p[0] - 0.0f, p[1] - 1.0f, p[2] - 0.0f, i0, j0 - 1, k0, B);
+            // +z / -z
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    B.v[a][b][0] = C.v[a][b][1];
+                    B.v[a][b][1] = acc.getValue(i0 + a, j0 + b, k0 + 2);
+                }
+            const float zp = fd6Sample(acc, p[0] + 0.0f, p[1] + 0.0f, p[2] + 1.0f, i0, j0, k0 + 1, B);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    B.v[a][b][1] = C.v[a][b][0];
+                    B.v[a][b][0] = acc.getValue(i0 + a, j0 + b, k0 - 1);
+                }
+            const float zm = fd6Sample(acc, p[0] - 0.0f, p[1] - 0.0f, p[2] - 1.0f, i0, j0, k0 - 1, B);
+            o[0] = (xp - xm) * fdx;
+            o[1] = (yp - ym) * fdy;
+            o[2] = (zp - zm) * fdz;
+        } else if (mode == 2) {
+            // GradStencil::moveTo(RoundDown(xyz)).gradient(), math/Stencils.h:148-152, 502-507
+            o[0] = gsInv2dx * (acc.getValue(i0 + 1, j0, k0) - acc.getValue(i0 - 1, j0, k0));
+            o[1] = gsInv2dx * (acc.getValue(i0, j0 + 1, k0) - acc.getValue(i0, j0 - 1, k0));
+            o[2] = gsInv2dx * (acc.getValue(i0, j0, k0 + 1) - acc.getValue(i0, j0, k0 - 1));
+        } else if (mode == 3) {
+            trilinearGradient(C, u, v, w, o);
+        }
+        grad[3 * i] = o[0];
+        grad[3 * i + 1] = o[1];
+        grad[3 * i + 2] = o[2];
+    }
+}
+
+// ------------------------------------------------------------------ GradStencil
+__device__ __forceinline__ float sqf(float x) { return x * x; }
+
+template<uint32_t GT>
+__global__ __launch_bounds__(kBlock) void k_grad_stencil(GridView g, const int32_t* __restrict__ ijk, uint64_t n,
+                                                         float* __restrict__ grad, float* __restrict__ normsq,
+                                                         float* __restrict__ lap, uint8_t* __restrict__ zc,
+                                                         float inv2dx, float invdx2)
+{
+    Accessor<GT> acc(g);
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        const int32_t a = ijk[3 * i], b = ijk[3 * i + 1], c = ijk[3 * i + 2];
+        float v[7];
+        v[0] = acc.getValue(a, b, c);
+        v[1] = acc.getValue(a - 1, b, c);
+        v[2] = acc.getValue(a + 1, b, c);
+        v[3] = acc.getValue(a, b - 1, c);
+        v[4] = acc.getValue(a, b + 1, c);
+        v[5] = acc.getValue(a, b, c - 1);
+        v[6] = acc.getValue(a, b, c + 1);
+        if (grad) {
+            grad[3 * i] = inv2dx * (v[2] - v[1]);
+            grad[3 * i + 1] = inv2dx * (v[4] - v[3]);
+            grad[3 * i + 2] = inv2dx * (v[6] - v[5]);
+        }
+        if (normsq) {
+            // GodunovsNormSqrd, math/Stencils.h:62-80
+            const float xm = v[0] - v[1], xp = v[2] - v[0], ym = v[0] - v[3], yp = v[4] - v[0], zm = v[0] - v[5],
+                        zp = v[6] - v[0];
+            float d;
+            if (v[0] > 0.0f) {
+                d = fmaxf(sqf(fmaxf(xm, 0.0f)), sqf(fminf(xp, 0.0f)));
+                d += fmaxf(sqf(fmaxf(ym, 0.0f)), sqf(fminf(yp, 0.0f)));
+                d += fmaxf(sqf(fmaxf(zm, 0.0f)), sqf(fminf(zp, 0.0f)));
+            } else {
+                d = fmaxf(sqf(fminf(xm, 0.0f)), sqf(fmaxf(xp, 0.0f)));
+                d += fmaxf(sqf(fminf(ym, 0.0f)), sqf(fmaxf(yp, 0.0f)));
+                d += fmaxf(sqf(fminf(zm, 0.0f)), sqf(fmaxf(zp, 0.0f)));
+            }
+            normsq[i] = invdx2 * d;
+        }
+        if (lap) lap[i] = invdx2 * (v[1] + v[2] + v[3] + v[4] + v[5] + v[6] - 6.0f * v[0]);
+        if (zc)
+            zc[i] = (v[0] > 0.0f ? (v[1] < 0 || v[2] < 0 || v[3] < 0 || v[4] < 0 || v[5] < 0 || v[6] < 0)
+                                 : (v[1] > 0 || v[2] > 0 || v[3] > 0 || v[4] > 0 || v[5] > 0 || v[6] > 0))
+                        ? 1
+                        : 0;
+    }
+}
+
+// ------------------------------------------------------------------ zeroCrossing
+// math/HDDA.h:189-214 over the device accessor; normal from the trilinear gradient at ray(t),
+// Grid::indexToWorldGradF (Map::applyIJTF, math/Math.h:1758-1763), normalised.
+template<uint32_t GT>
+__global__ __launch_bounds__(kBlock) void k_zero_crossing(GridView g, const float* __restrict__ eye,
+                                                          const float* __restrict__ dir, Sched s,
+                                                          uint8_t* __restrict__ hit, int32_t* __restrict__ ijk,
+                                                          float* __restrict__ tout, float* __restrict__ vout,
+                                                          float* __restrict__ normal)
+{
+    uint64_t i;
+    if (!schedule(s, i)) return;
+    Accessor<GT> acc(g);
+    const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
+    const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    Ray ray = worldToIndexRay(g, e, d);
+    bool found = false;
+    int32_t c[3] = {0, 0, 0};
+    float v = 0.0f, t = 0.0f;
+    if (rayClip(ray, g.bbox) && !(double(ray.t1) > 1e20)) {
+        const float Delta = 1.0001f;
+        float p[3];
+        rayAt(ray, ray.t0, p);
+        c[0] = ifloor(p[0]);
+        c[1] = ifloor(p[1]);
+        c[2] = ifloor(p[2]);
+        const Probe p0 = acc.probe(c[0], c[1], c[2]);
+        HDDA h;
+        h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
+        const float v0 = p0.value;
+        while (!found && h.stepOnce()) {
+            rayAt(ray, h.t0 + Delta, p);
+            c[0] = ifloor(p[0]);
+            c[1] = ifloor(p[1]);
+            c[2] = ifloor(p[2]);
+            const Probe q = acc.probe(c[0], c[1], c[2]);
+            h.update(ray, int32_t(q.dim));
+            if (h.dim > 1 || !q.active) continue;
+            while (h.stepOnce()) { // in the narrow band
+                const Probe r = acc.probe(h.voxel[0], h.voxel[1], h.voxel[2]);
+                if (!r.active) break;
+                v = r.value;
+                if (v * v0 < 0) {
+                    c[0] = h.voxel[0];
+                    c[1] = h.voxel[1];
+                    c[2] = h.voxel[2];
+                    t = h.t0;
+                    found = true;
+                    break;
+                }
+            }
+        }
+    }
+    if (hit) hit[i] = found ? 1 : 0;
+    if (ijk) {
+        ijk[3 * i] = found ? c[0] : 0;
+        ijk[3 * i + 1] = found ? c[1] : 0;
+        ijk[3 * i + 2] = found ? c[2] : 0;
+    }
+    if (tout) tout[i] = found ? t : 0.0f;
+    if (vout) vout[i] = found ? v : 0.0f;
+    if (normal) {
+        float nr[3] = {0.0f, 0.0f, 0.0f};
+        if (found) {
+            float p[3], gi[3], gw[3];
+            rayAt(ray, t, p);
+            const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+            Cell C;
+            stencil8(acc, int32_t(fx), int32_t(fy), int32_t(fz), C);
+            trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, gi);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                gw[a] = __builtin_fmaf(gi[0], g.invMatF[a], __builtin_fmaf(gi[1], g.invMatF[3 + a], gi[2] * g.invMatF[6 + a]));
+            const float len = sqrtf(gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2]);
+            if (len > 0.0f) {
+                const float sc = 1.0f / len;
+                nr[0] = sc * gw[0];
+                nr[1] = sc * gw[1];
+                nr[2] = sc * gw[2];
+            }
+        }
+        normal[3 * i] = nr[0];
+        normal[3 * i + 1] = nr[1];
+        normal[3 * i + 2] = nr[2];
+    }
+}
+
+// ------------------------------------------------------------------ fog march
+// HDDA over cells of size >= 8 (leaf level, as TreeMarcher, math/HDDA.h:450-536, Eps 1e-6,
+// Delta 0.01); leaves and active tiles are marched at globally aligned steps t = k*dt with
+// ta <= t < tb; a = exp(-sigma*rho*dt), L += T*(1-a)*rho, T *= a.
+template<uint32_t GT>
+__global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* __restrict__ eye,
+                                                      const float* __restrict__ dir, Sched s, float dt, float sigma,
+                                                      float* __restrict__ Lout, float* __restrict__ Tout)
+{
+    uint64_t i;
+    if (!schedule(s, i)) return;
+    Accessor<GT> acc(g);
+    const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
+    const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    Ray R = worldToIndexRay(g, e, d);
+    float L = 0.0f, T = 1.0f;
+    if (rayClip(R, g.bbox) && !(double(R.t1) > 1e20)) {
+        const float Eps = 0.000001f, Delta = 0.01f;
+        const float t0 = R.t0 + Eps, t1 = R.t1 - Eps;
+        if (t0 <= t1) {
+            float p[3];
+            rayAt(R, t0, p);
+            Probe l = acc.probe(ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+            HDDA h;
+            h.init(R, t0, t1, int32_t(l.dim > 8u ? l.dim : 8u));
+            bool on = (l.dim == 1u) || l.active;
+            for (;;) {
+                const float ta = h.t0;
+                const bool valid = h.stepOnce();
+                const float tb = fminf(h.t0, t1);
+                if (on && ta < tb) {
+                    float kf = ceilf(ta / dt);
+                    for (float t = kf * dt; t < tb; kf += 1.0f, t = kf * dt) {
+                        float x[3];
+                        rayAt(R, t, x);
+                        const float rho = trilinear(acc, x[0], x[1], x[2]);
+                        const float a = expf(-sigma * rho * dt);
+                        L += T * (1.0f - a) * rho;
+                        T *= a;
+                    }
+                }
+                if (!valid) break;
+                rayAt(R, h.t0 + Delta, p);
+                l = acc.probe(ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+                h.update(R, int32_t(l.dim > 8u ? l.dim : 8u));
+                on = (l.dim == 1u) || l.active;
+            }
+        }
+    }
+    if (Lout) Lout[i] = L;
+    if (Tout) Tout[i] = T;
+}
+
+// ------------------------------------------------------------------ host helpers
+template<template<uint32_t> class K, typename... Args>
+int launchTyped(uint32_t type, Args&&... args)
+{
+    switch (type) {
+    case GT_FLOAT: K<GT_FLOAT>::run(args...); break;
+    case GT_FP4: K<GT_FP4>::run(args...); break;
+    case GT_FP8: K<GT_FP8>::run(args...); break;
+    case GT_FP16: K<GT_FP16>::run(args...); break;
+    case GT_FPN: K<GT_FPN>::run(args...); break;
+    default: return NVQ_ERR_UNSUPPORTED_TYPE;
+    }
+    return hipGetLastError() == hipSuccess ? NVQ_OK : NVQ_ERR_HIP;
+}
+
+uint32_t gridStrideBlocks(uint64_t n)
+{
+    const uint64_t b = (n + kBlock - 1) / kBlock;
+    const uint64_t cap = 1u << 20;
+    return uint32_t(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+template<uint32_t GT> struct RunGetValue {
+    static void run(const GridView& g, const int32_t* ijk, uint64_t n, float* val, uint8_t* act, uint32_t* dim, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_get_value<GT>, dim3(gridStrideBlocks(n)), dim3(kBlock), 0, s, g, ijk, n, val, act, dim);
+    }
+};
+template<uint32_t GT> struct RunSample {
+    static void run(const GridView& g, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
+                    int mode, hipStream_t s)
+    {
+        const float fdx = float(0.5 / g.voxelSize[0]), fdy = float(0.5 / g.voxelSize[1]), fdz = float(0.5 / g.voxelSize[2]);
+        const float gs = float(0.5 / g.voxelSize[0]);
+        hipLaunchKernelGGL(k_sample<GT>, dim3(gridStrideBlocks(n)), dim3(kBlock), 0, s, g, xyz, n, world, nn, val, grad,
+                           mode, fdx, fdy, fdz, gs);
+    }
+};
+template<uint32_t GT> struct RunGradStencil {
+    static void run(const GridView& g, const int32_t* ijk, uint64_t n, float* grad, float* ns, float* lap, uint8_t* zc,
+                    hipStream_t s)
+    {
+        const float inv2dx = float(0.5 / g.voxelSize[0]);
+        const float invdx2 = float(4.0 * double(inv2dx) * double(inv2dx));
+        hipLaunchKernelGGL(k_grad_stencil<GT>, dim3(gridStrideBlocks(n)), dim3(kBlock), 0, s, g, ijk, n, grad, ns, lap, zc,
+                           inv2dx, invdx2);
+    }
+};
+template<uint32_t GT> struct RunZeroCrossing {
+    static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, uint8_t* hit, int32_t* ijk,
+                    float* t, float* v, float* nrm, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_zero_crossing<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, hit, ijk, t,
+                           v, nrm);
+    }
+};
+template<uint32_t GT> struct RunFogMarch {
+    static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, float dt, float sigma,
+                    float* L, float* T, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_fog_march<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T);
+    }
+};
+
+template<typename T> T rd(const uint8_t* p)
+{
+    T v;
+    std::memcpy(&v, p, sizeof(T));
+    return v;
+}
+
+uint32_t leafSizeOf(uint32_t type)
+{
+    switch (type) {
+    case GT_FLOAT: return 2144u;
+    case GT_FP4: return 352u;
+    case GT_FP8: return 608u;
+    case GT_FP16: return 1120u;
+    default: return 0u;
+    }
+}
+
+// Host-side validation of a NanoVDB buffer before upload (GridData::isValid, NanoVDB.h:2002-2013,
+// plus bounds checks of every node offset the kernels follow).
+int validate(const uint8_t* b, uint64_t bytes)
+{
+    if (!b || bytes < kGridSize + 64 + kRootSize) return b ? NVQ_ERR_TRUNCATED : NVQ_ERR_INVALID_ARG;
+    if (reinterpret_cast<uintptr_t>(b) % 32u) return NVQ_ERR_MISALIGNED;
+    const uint64_t magic = rd<uint64_t>(b) & 0x00FFFFFFFFFFFFFFull;
+    if (magic != (0x314244566f6e614eull & 0x00FFFFFFFFFFFFFFull)) return NVQ_ERR_BAD_MAGIC;
+    if ((rd<uint32_t>(b + 16) >> 21) != 32u) return NVQ_ERR_BAD_VERSION;
+    const uint32_t type = rd<uint32_t>(b + kGridType);
+    if (type != GT_FLOAT && type != GT_FP4 && type != GT_FP8 && type != GT_FP16 && type != GT_FPN)
+        return NVQ_ERR_UNSUPPORTED_TYPE;
+    const uint64_t gridSize = rd<uint64_t>(b + 32);
+    if (gridSize > bytes) return NVQ_ERR_TRUNCATED;
+    const uint8_t* tree = b + kGridSize;
+    const int64_t offLeaf = rd<int64_t>(tree), offLower = rd<int64_t>(tree + 8), offUpper = rd<int64_t>(tree + 16),
+                  offRoot = rd<int64_t>(tree + 24);
+    const uint32_t nLeaf = rd<uint32_t>(tree + 32), nLower = rd<uint32_t>(tree + 36), nUpper = rd<uint32_t>(tree + 40);
+    const uint64_t base = kGridSize;
+    auto inside = [&](int64_t off, uint64_t len) {
+        return off >= 0 && base + uint64_t(off) + len <= bytes;
+    };
+    if (!inside(offRoot, kRootSize)) return NVQ_ERR_TRUNCATED;
+    const uint32_t tableSize = rd<uint32_t>(tree + offRoot + kRootTableSize);
+    if (!inside(offRoot, kRootSize + uint64_t(kTileSize) * tableSize)) return NVQ_ERR_TRUNCATED;
+    if (nUpper && !inside(offUpper, uint64_t(kUpSize) * nUpper)) return NVQ_ERR_TRUNCATED;
+    if (nLower && !inside(offLower, uint64_t(kLoSize) * nLower)) return NVQ_ERR_TRUNCATED;
+    const uint32_t ls = leafSizeOf(type);
+    if (nLeaf && ls && !inside(offLeaf, uint64_t(ls) * nLeaf)) return NVQ_ERR_TRUNCATED;
+    if (nLeaf && !ls && !inside(offLeaf, 96u)) return NVQ_ERR_TRUNCATED;
+    return NVQ_OK;
+}
+
+} // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+const char* nvq_status_string(int status)
+{
+    switch (status) {
+    case NVQ_OK: return "ok";
+    case NVQ_ERR_INVALID_ARG: return "invalid argument";
+    case NVQ_ERR_BAD_MAGIC: return "not a NanoVDB grid buffer (bad magic number)";
+    case NVQ_ERR_BAD_VERSION: return "unsupported NanoVDB major version (expected 32)";
+    case NVQ_ERR_UNSUPPORTED_TYPE: return "unsupported grid type for this operation";
+    case NVQ_ERR_MISALIGNED: return "grid buffer is not 32-byte aligned";
+    case NVQ_ERR_TRUNCATED: return "grid buffer is truncated or has out-of-range node offsets";
+    case NVQ_ERR_HIP: return "HIP runtime error";
+    case NVQ_ERR_NO_DEVICE: return "no usable GPU device";
+    default: return "unknown status";
+    }
+}
+
+int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    const uint8_t* b = static_cast<const uint8_t*>(host_buf);
+    const int st = validate(b, bytes);
+    if (st != NVQ_OK) return st;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return NVQ_ERR_HIP;
+    if (hipMemcpyAsync(d, host_buf, bytes, hipMemcpyHostToDevice, S(stream)) != hipSuccess) {
+        (void)hipFree(d);
+        return NVQ_ERR_HIP;
+    }
+    nvq_grid* g = new (std::nothrow) nvq_grid;
+    if (!g) {
+        (void)hipFree(d);
+        return NVQ_ERR_HIP;
+    }
+    g->dbuf = d;
+    g->bytes = bytes;
+    g->device = device;
+    const uint8_t* tree = b + kGridSize;
+    const int64_t offLeaf = rd<int64_t>(tree), offRoot = rd<int64_t>(tree + 24);
+    const uint8_t* root = tree + offRoot;
+    const uint8_t* dbase = static_cast<const uint8_t*>(d);
+    GridView& v = g->view;
+    std::memset(&v, 0, sizeof(v));
+    v.root = dbase + kGridSize + offRoot;
+    v.leaf0 = dbase + kGridSize + offLeaf;
+    v.tableSize = rd<uint32_t>(root + kRootTableSize);
+    v.type = rd<uint32_t>(b + kGridType);
+    v.background = rd<float>(root + kRootBackground);
+    for (int i = 0; i < 6; ++i) v.bbox[i] = rd<int32_t>(root + kRootBBox + 4 * i);
+    for (int i = 0; i < 9; ++i) {
+        v.invMatF[i] = rd<float>(b + kGridMap + kMapInvMatF + 4 * i);
+        v.invMatD[i] = rd<double>(b + kGridMap + kMapInvMatD + 8 * i);
+    }
+    for (int i = 0; i < 3; ++i) {
+        v.vecF[i] = rd<float>(b + kGridMap + kMapVecF + 4 * i);
+        v.vecD[i] = rd<double>(b + kGridMap + kMapVecD + 8 * i);
+        v.voxelSize[i] = rd<double>(b + kGridVoxelSize + 8 * i);
+    }
+    v.leafCount = rd<uint32_t>(tree + 32);
+    v.leafSize = leafSizeOf(v.type);
+    nvq_grid_info& in = g->info;
+    std::memset(&in, 0, sizeof(in));
+    in.grid_type = v.type;
+    in.grid_class = rd<uint32_t>(b + kGridClass);
+    in.bytes = bytes;
+    for (int i = 0; i < 3; ++i) in.node_count[i] = rd<uint32_t>(tree + 32 + 4 * i);
+    in.root_tiles = v.tableSize;
+    for (int i = 0; i < 6; ++i) in.index_bbox[i] = v.bbox[i];
+    for (int i = 0; i < 3; ++i) in.voxel_size[i] = v.voxelSize[i];
+    in.background = v.background;
+    in.device_buffer = d;
+    *out = g;
+    return NVQ_OK;
+}
+
+int nvq_grid_free(nvq_grid* grid)
+{
+    if (!grid) return NVQ_ERR_INVALID_ARG;
+    const hipError_t e1 = hipDeviceSynchronize();
+    const hipError_t e2 = hipFree(grid->dbuf);
+    delete grid;
+    return (e1 == hipSuccess && e2 == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
+}
+
+int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info)
+{
+    if (!grid || !info) return NVQ_ERR_INVALID_ARG;
+    *info = grid->info;
+    return NVQ_OK;
+}
+
+int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* val, uint8_t* active, uint32_t* dim,
+                  void* stream)
+{
+    if (!grid || (n && !ijk)) return NVQ_ERR_INVALID_ARG;
+    if (n == 0) return NVQ_OK;
+    return launchTyped<RunGetValue>(grid->view.type, grid->view, ijk, n, val, active, dim, S(stream));
+}
+
+int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
+               int grad_mode, void* stream)
+{
+    if (!grid || (n && !xyz) || (grad && (grad_mode < 1 || grad_mode > 3))) return NVQ_ERR_INVALID_ARG;
+    if (n == 0) return NVQ_OK;
+    return launchTyped<RunSample>(grid->view.type, grid->view, xyz, n, world, nn, val, grad, grad_mode, S(stream));
+}
+
+int nvq_grad_stencil(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* grad, float* norm_sq, float* lap,
+                     uint8_t* zero_crossing, void* stream)
+{
+    if (!grid || (n && !ijk)) return NVQ_ERR_INVALID_ARG;
+    if (grid->view.type != GT_FLOAT) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (n == 0) return NVQ_OK;
+    return launchTyped<RunGradStencil>(grid->view.type, grid->view, ijk, n, grad, norm_sq, lap, zero_crossing, S(stream));
+}
+
+int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                      uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream)
+{
+    if (!grid || (n && (!eye || !dir))) return NVQ_ERR_INVALID_ARG;
+    if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
+    if (n == 0) return NVQ_OK;
+    if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
+    const Sched sc = makeSched(n, image_width);
+    return launchTyped<RunZeroCrossing>(grid->view.type, grid->view, eye, dir, sc, hit, ijk, t, v, normal, S(stream));
+}
+
+int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width, float dt,
+                  float sigma_t, float* radiance, float* transmittance, void* stream)
+{
+    if (!grid || (n && (!eye || !dir)) || !(dt > 0.0f)) return NVQ_ERR_INVALID_ARG;
+    if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
+    if (n == 0) return NVQ_OK;
+    if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
+    const Sched sc = makeSched(n, image_width);
+    return launchTyped<RunFogMarch>(grid->view.type, grid->view, eye, dir, sc, dt, sigma_t, radiance, transmittance,
+                                    S(stream));
+}
+
+} // extern "C"

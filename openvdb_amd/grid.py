@@ -1,0 +1,90 @@
+"""Device-resident NanoVDB grids (the GridHandle::deviceUpload / deviceGrid<T>() role).
+
+A ``DeviceGrid`` owns one device copy of an unmodified NanoVDB buffer, made by
+``nvq_grid_upload`` (one hipMalloc + one hipMemcpyAsync; GridHandle.h:159-194,
+cuda/DeviceBuffer.h:422-437).  Errors surface as ``NvqError`` with the C-ABI
+status, where the reference's ``deviceGrid<T>()`` would return nullptr.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import GridInfo, check, lib
+
+GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN"}
+GRID_CLASSES = {0: "Unknown", 1: "LevelSet", 2: "FogVolume"}
+
+
+def aligned_buffer(data):
+    """Copy bytes into a 32-byte aligned uint8 array (NanoVDB.h:69-74)."""
+    src = np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data.reshape(-1).view(np.uint8)
+    raw = np.empty(src.nbytes + 32, np.uint8)
+    off = (-raw.ctypes.data) % 32
+    out = raw[off:off + src.nbytes]
+    out[:] = src
+    return out
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class DeviceGrid:
+    """An uploaded grid.  ``host_buffer`` is a uint8 numpy array holding the NanoVDB buffer."""
+
+    def __init__(self, host_buffer, stream=None):
+        buf = np.ascontiguousarray(host_buffer).reshape(-1).view(np.uint8)
+        if buf.ctypes.data % 32:
+            buf = aligned_buffer(buf)
+        self._host = buf
+        handle = ctypes.c_void_p()
+        s = _stream_ptr(stream)
+        check(lib().nvq_grid_upload(buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes, s, ctypes.byref(handle)),
+              "nvq_grid_upload")
+        self.handle = handle
+        # the copy is asynchronous; keep the host bytes alive until it is done
+        import torch
+        torch.cuda.synchronize()
+        info = GridInfo()
+        check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
+        self.info = info
+
+    @property
+    def grid_type(self):
+        return GRID_TYPES.get(self.info.grid_type, str(self.info.grid_type))
+
+    @property
+    def grid_class(self):
+        return GRID_CLASSES.get(self.info.grid_class, str(self.info.grid_class))
+
+    @property
+    def leaf_count(self):
+        return int(self.info.node_count[0])
+
+    @property
+    def nbytes(self):
+        return int(self.info.bytes)
+
+    @property
+    def index_bbox(self):
+        return tuple(self.info.index_bbox)
+
+    @property
+    def voxel_size(self):
+        return tuple(self.info.voxel_size)
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            check(lib().nvq_grid_free(self.handle), "nvq_grid_free")
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

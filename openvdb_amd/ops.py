@@ -1,0 +1,114 @@
+"""Torch-facing batched queries over a ``DeviceGrid`` (thin wrappers of the C ABI).
+
+Each function mirrors one reference call shape, applied to a batch:
+
+* ``get_value``        ReadAccessor::getValue / isActive / getDim   (NanoVDB.h:5370-5447)
+* ``sample_from_voxels`` nanovdb.tools.cuda.sampleFromVoxels(points, d_grid, values[, gradients])
+                       (python/cuda/PySampleFromVoxels.cu:58-89)
+* ``sample``           SampleFromVoxels<Acc,0|1> + gradient modes   (math/SampleFromVoxels.h)
+* ``grad_stencil``     GradStencil                                  (math/Stencils.h:455-576)
+* ``zero_crossing``    Ray::worldToIndexF + math::zeroCrossing      (math/HDDA.h:189-214)
+* ``fog_march``        emission-absorption march with HDDA node skipping
+
+All tensors must be CUDA tensors on the grid's device; work is queued on the
+current torch stream.
+"""
+import ctypes
+
+import torch
+
+from ._lib import check, lib
+
+GRAD_FD6, GRAD_STENCIL, GRAD_TRILINEAR = 1, 2, 3
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _points(x, dtype, name):
+    if not x.is_cuda:
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if x.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}")
+    if x.dim() != 2 or x.shape[1] != 3:
+        raise ValueError(f"{name} must have shape [N, 3]")
+    return x.contiguous()
+
+
+def get_value(grid, ijk, want_dim=False):
+    ijk = _points(ijk, torch.int32, "ijk")
+    n = ijk.shape[0]
+    val = torch.empty(n, dtype=torch.float32, device=ijk.device)
+    act = torch.empty(n, dtype=torch.uint8, device=ijk.device)
+    dim = torch.empty(n, dtype=torch.int32, device=ijk.device) if want_dim else None
+    check(lib().nvq_get_value(grid.handle, _ptr(ijk), n, _ptr(val), _ptr(act), _ptr(dim), _stream()), "nvq_get_value")
+    return (val, act, dim) if want_dim else (val, act)
+
+
+def sample(grid, xyz, world=False, nn=True, val=True, grad_mode=GRAD_FD6):
+    xyz = _points(xyz, torch.float32, "xyz")
+    n = xyz.shape[0]
+    dev = xyz.device
+    o_nn = torch.empty(n, dtype=torch.float32, device=dev) if nn else None
+    o_val = torch.empty(n, dtype=torch.float32, device=dev) if val else None
+    o_grad = torch.empty((n, 3), dtype=torch.float32, device=dev) if grad_mode else None
+    check(lib().nvq_sample(grid.handle, _ptr(xyz), n, int(world), _ptr(o_nn), _ptr(o_val), _ptr(o_grad),
+                           int(grad_mode or 0), _stream()), "nvq_sample")
+    return o_nn, o_val, o_grad
+
+
+def sample_from_voxels(points, grid, values, gradients=None):
+    """Same arguments and semantics as the reference op: world-space ``points[N,3]``,
+    trilinear ``values[N]`` and optional FD6 ``gradients[N,3]`` written in place."""
+    points = _points(points, torch.float32, "points")
+    n = points.shape[0]
+    if values.shape[0] != n or not values.is_contiguous():
+        raise ValueError("values must be a contiguous [N] tensor")
+    if gradients is not None and (tuple(gradients.shape) != (n, 3) or not gradients.is_contiguous()):
+        raise ValueError("gradients must be a contiguous [N, 3] tensor")
+    check(lib().nvq_sample(grid.handle, _ptr(points), n, 1, None, _ptr(values), _ptr(gradients), GRAD_FD6,
+                           _stream()), "nvq_sample")
+
+
+def grad_stencil(grid, ijk):
+    ijk = _points(ijk, torch.int32, "ijk")
+    n = ijk.shape[0]
+    dev = ijk.device
+    g = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    ns = torch.empty(n, dtype=torch.float32, device=dev)
+    lap = torch.empty(n, dtype=torch.float32, device=dev)
+    zc = torch.empty(n, dtype=torch.uint8, device=dev)
+    check(lib().nvq_grad_stencil(grid.handle, _ptr(ijk), n, _ptr(g), _ptr(ns), _ptr(lap), _ptr(zc), _stream()),
+          "nvq_grad_stencil")
+    return g, ns, lap, zc
+
+
+def zero_crossing(grid, eye, dir, image_width=0):
+    eye = _points(eye, torch.float32, "eye")
+    dir = _points(dir, torch.float32, "dir")
+    n = eye.shape[0]
+    dev = eye.device
+    hit = torch.empty(n, dtype=torch.uint8, device=dev)
+    ijk = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    v = torch.empty(n, dtype=torch.float32, device=dev)
+    nrm = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    check(lib().nvq_zero_crossing(grid.handle, _ptr(eye), _ptr(dir), n, image_width, _ptr(hit), _ptr(ijk), _ptr(t),
+                                  _ptr(v), _ptr(nrm), _stream()), "nvq_zero_crossing")
+    return hit, ijk, t, v, nrm
+
+
+def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):
+    eye = _points(eye, torch.float32, "eye")
+    dir = _points(dir, torch.float32, "dir")
+    n = eye.shape[0]
+    L = torch.empty(n, dtype=torch.float32, device=eye.device)
+    T = torch.empty(n, dtype=torch.float32, device=eye.device)
+    check(lib().nvq_fog_march(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t, _ptr(L), _ptr(T),
+                              _stream()), "nvq_fog_march")
+    return L, T

@@ -1,0 +1,775 @@
+/* Oracle: CPU restatement of NanoVDB's read-only query path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py as the checker; never linked into, loaded by or
+ * called from the product library (openvdb_amd/).
+ *
+ * The flat buffer is read by byte offsets.  Layout and algorithms follow the
+ * reference files cited at each function (paths relative to the reference
+ * root).  Every FP32 routine keeps the reference's operation order; the file is
+ * compiled with -ffp-contract=off and fmaf appears only where the reference
+ * writes fmaf/fma.
+ *
+ * Pinning (what checks each part; the tests under tests/ hold them):
+ *   lookup/getValue/isActive/getDim, codecs  : compiled reference (oracle/_ref) bit-exact, closed-form sphere,
+ *                                              RandomReadAccessorGet known answers
+ *   trilinear / FD6 / GradStencil / gradient : compiled reference bit-exact, linear-field closed forms,
+ *                                              sampleFromVoxels golden values of the reference Python tests
+ *   Ray clip / HDDA / zeroCrossing           : reference unit-test known answers, compiled reference bit-exact
+ *   fog march (FP64)                         : compiled-reference march over the reference HDDA/accessor/sampler
+ *                                              (same contract), constant-density closed form
+ *   fog-march adjoint (FP64)                 : central finite differences of or_fog_march; no reference
+ *                                              implementation exists (parity vs the reference is unpinned by
+ *                                              construction, the adjoint is pinned by calculus only)
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ layout
+ * nanovdb/nanovdb/NanoVDB.h: GridData 1944-2135 (672 B), Map 1418-1587,
+ * TreeData 2393-2423, RootData 2621-2840 (+32 B tiles), InternalData 3181-3250,
+ * LeafData 3671-3973.  Byte offsets as pinned by unittest/TestNanoVDB.cc:4938+. */
+enum {
+    GRID_MAGIC = 0, GRID_VERSION = 16, GRID_MAP = 296, GRID_VOXELSIZE = 608, GRID_CLASS = 632, GRID_TYPE = 636,
+    GRID_SIZE = 672,
+    MAP_INVMATF = 36, MAP_VECF = 72, MAP_INVMATD = 160, MAP_VECD = 232,
+    TREE_NODEOFF = 0, TREE_NODECNT = 32,
+    ROOT_BBOX = 0, ROOT_TABLESIZE = 24, ROOT_BACKGROUND = 28, ROOT_SIZE = 64, TILE_SIZE = 32,
+    UP_FLAGS = 24, UP_VMASK = 32, UP_CMASK = 4128, UP_TABLE = 8256,
+    LO_FLAGS = 24, LO_VMASK = 32, LO_CMASK = 544, LO_TABLE = 1088,
+    LEAF_FLAGS = 15, LEAF_VMASK = 16, LEAF_MIN = 80, LEAF_QUANTUM = 84, LEAF_DATA = 96
+};
+enum { GT_FLOAT = 1, GT_FP4 = 13, GT_FP8 = 14, GT_FP16 = 15, GT_FPN = 16 };
+static const uint64_t MAGIC_GRID = 0x314244566f6e614eULL; /* "NanoVDB1", NanoVDB.h:140 */
+
+static inline uint64_t rd_u64(const uint8_t* p) { uint64_t v; memcpy(&v, p, 8); return v; }
+static inline int64_t rd_i64(const uint8_t* p) { int64_t v; memcpy(&v, p, 8); return v; }
+static inline uint32_t rd_u32(const uint8_t* p) { uint32_t v; memcpy(&v, p, 4); return v; }
+static inline int32_t rd_i32(const uint8_t* p) { int32_t v; memcpy(&v, p, 4); return v; }
+static inline uint16_t rd_u16(const uint8_t* p) { uint16_t v; memcpy(&v, p, 2); return v; }
+static inline float rd_f32(const uint8_t* p) { float v; memcpy(&v, p, 4); return v; }
+static inline double rd_f64(const uint8_t* p) { double v; memcpy(&v, p, 8); return v; }
+
+typedef struct {
+    const uint8_t* buf;
+    const uint8_t* tree;
+    const uint8_t* root;
+    const uint8_t* leaf0;
+    uint32_t type, leafCount, tableSize, leafSize;
+    float background;
+    int32_t bbox[6];
+} G;
+
+static uint32_t leaf_size_of(uint32_t type)
+{
+    switch (type) {
+    case GT_FLOAT: return 2144u;
+    case GT_FP4: return 352u;
+    case GT_FP8: return 608u;
+    case GT_FP16: return 1120u;
+    default: return 0u; /* FpN: variable */
+    }
+}
+
+static G open_grid(const void* p)
+{
+    G g;
+    g.buf = (const uint8_t*)p;
+    g.tree = g.buf + GRID_SIZE;
+    g.type = rd_u32(g.buf + GRID_TYPE);
+    g.root = g.tree + rd_i64(g.tree + TREE_NODEOFF + 8 * 3);
+    g.leaf0 = g.tree + rd_i64(g.tree + TREE_NODEOFF);
+    g.leafCount = rd_u32(g.tree + TREE_NODECNT);
+    g.tableSize = rd_u32(g.root + ROOT_TABLESIZE);
+    g.background = rd_f32(g.root + ROOT_BACKGROUND);
+    g.leafSize = leaf_size_of(g.type);
+    for (int i = 0; i < 6; ++i) g.bbox[i] = rd_i32(g.root + ROOT_BBOX + 4 * i);
+    return g;
+}
+
+int or_validate(const void* buf, uint64_t bytes)
+{
+    if (!buf || bytes < GRID_SIZE + 64) return -1;
+    const uint8_t* b = (const uint8_t*)buf;
+    if ((rd_u64(b + GRID_MAGIC) & 0x00FFFFFFFFFFFFFFULL) != (MAGIC_GRID & 0x00FFFFFFFFFFFFFFULL)) return -2;
+    if ((rd_u32(b + GRID_VERSION) >> 21) != 32u) return -3; /* Version: major<<21 | minor<<10 | patch */
+    const uint32_t t = rd_u32(b + GRID_TYPE);
+    if (t != GT_FLOAT && t != GT_FP4 && t != GT_FP8 && t != GT_FP16 && t != GT_FPN) return -4;
+    return 0;
+}
+uint32_t or_grid_type(const void* buf) { return rd_u32((const uint8_t*)buf + GRID_TYPE); }
+uint32_t or_leaf_count(const void* buf) { return open_grid(buf).leafCount; }
+
+/* ------------------------------------------------------------------ leaf decode
+ * float NanoVDB.h:3703; Fp4 :3839-3847; Fp8 :3875-3878; Fp16 :3906-3909;
+ * FpN branchless :3937-3951.  value = float(code)*quantum + minimum (no fma). */
+static uint32_t leaf_code(const G* g, const uint8_t* leaf, uint32_t n)
+{
+    const uint8_t* d = leaf + LEAF_DATA;
+    switch (g->type) {
+    case GT_FP4: return (d[n >> 1] >> ((n & 1) << 2)) & 15u;
+    case GT_FP8: return d[n];
+    case GT_FP16: return rd_u16(d + 2 * n);
+    default: {
+        const int b = leaf[LEAF_FLAGS] >> 5;
+        uint32_t code = rd_u32(d + 4 * (n >> (5 - b)));
+        code >>= (n & ((32u >> b) - 1u)) << b;
+        return code & ((1u << (1u << b)) - 1u);
+    }
+    }
+}
+
+static float leaf_value(const G* g, const uint8_t* leaf, uint32_t n)
+{
+    if (g->type == GT_FLOAT) return rd_f32(leaf + LEAF_DATA + 4 * n);
+    return (float)leaf_code(g, leaf, n) * rd_f32(leaf + LEAF_QUANTUM) + rd_f32(leaf + LEAF_MIN);
+}
+
+/* same code, FP64 reconstruction code*quantum + minimum */
+static double leaf_value_f64(const G* g, const uint8_t* leaf, uint32_t n)
+{
+    if (g->type == GT_FLOAT) return (double)rd_f32(leaf + LEAF_DATA + 4 * n);
+    return (double)leaf_code(g, leaf, n) * (double)rd_f32(leaf + LEAF_QUANTUM) + (double)rd_f32(leaf + LEAF_MIN);
+}
+
+static inline int mask_on(const uint8_t* mask, uint32_t n) { return (int)((rd_u64(mask + 8 * (n >> 6)) >> (n & 63)) & 1u); }
+
+/* ------------------------------------------------------------------ traversal
+ * RootData::CoordToKey (NANOVDB_USE_SINGLE_ROOT_KEY) NanoVDB.h:2633-2640; linear probe :2785-2799;
+ * InternalNode::CoordToOffset :3560-3565; LeafNode::CoordToOffset :4516-4519;
+ * getDimAndCache root :3126-3137, internal :3619-3633, leaf :4565-4573;
+ * GetValue/GetState :6196-6236. */
+typedef struct {
+    float value;
+    int active;
+    uint32_t dim;
+    const uint8_t* leaf; /* leaf node holding the voxel, or NULL */
+    uint32_t n;          /* voxel offset in that leaf */
+} Look;
+
+static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
+{
+    Look r;
+    r.leaf = NULL;
+    r.n = 0;
+    const uint64_t key = ((uint64_t)((uint32_t)z >> 12)) | ((uint64_t)((uint32_t)y >> 12) << 21) |
+                         ((uint64_t)((uint32_t)x >> 12) << 42);
+    const uint8_t* tile = NULL;
+    for (uint32_t i = 0; i < g->tableSize; ++i) {
+        const uint8_t* t = g->root + ROOT_SIZE + (uint64_t)TILE_SIZE * i;
+        if (rd_u64(t) == key) { tile = t; break; }
+    }
+    if (!tile) { r.value = g->background; r.active = 0; r.dim = 4096u; return r; }
+    const int64_t child = rd_i64(tile + 8);
+    if (child == 0) { r.value = rd_f32(tile + 20); r.active = rd_u32(tile + 16) > 0u; r.dim = 4096u; return r; }
+    const uint8_t* up = g->root + child;
+    const int upSkip = (int)(rd_u64(up + UP_FLAGS) & 1u);
+    const uint32_t nu = ((((uint32_t)x & 4095u) >> 7) << 10) | ((((uint32_t)y & 4095u) >> 7) << 5) | (((uint32_t)z & 4095u) >> 7);
+    if (!mask_on(up + UP_CMASK, nu)) {
+        r.value = rd_f32(up + UP_TABLE + 8 * nu);
+        r.active = mask_on(up + UP_VMASK, nu);
+        r.dim = upSkip ? 4096u : 128u;
+        return r;
+    }
+    const uint8_t* lo = up + rd_i64(up + UP_TABLE + 8 * nu);
+    const int loSkip = (int)(rd_u64(lo + LO_FLAGS) & 1u);
+    const uint32_t nl = ((((uint32_t)x & 127u) >> 3) << 8) | ((((uint32_t)y & 127u) >> 3) << 4) | (((uint32_t)z & 127u) >> 3);
+    if (!mask_on(lo + LO_CMASK, nl)) {
+        r.value = rd_f32(lo + LO_TABLE + 8 * nl);
+        r.active = mask_on(lo + LO_VMASK, nl);
+        r.dim = upSkip ? 4096u : (loSkip ? 128u : 8u);
+        return r;
+    }
+    const uint8_t* lf = lo + rd_i64(lo + LO_TABLE + 8 * nl);
+    const uint32_t n = (((uint32_t)x & 7u) << 6) | (((uint32_t)y & 7u) << 3) | ((uint32_t)z & 7u);
+    r.leaf = lf;
+    r.n = n;
+    r.value = leaf_value(g, lf, n);
+    r.active = mask_on(lf + LEAF_VMASK, n);
+    r.dim = upSkip ? 4096u : (loSkip ? 128u : ((lf[LEAF_FLAGS] & 1u) ? 8u : 1u));
+    return r;
+}
+
+static inline float get_value(const G* g, int32_t x, int32_t y, int32_t z) { return lookup(g, x, y, z).value; }
+
+void or_get_value(const void* buf, const int32_t* ijk, uint64_t n, float* val, uint8_t* active, uint32_t* dim)
+{
+    const G g = open_grid(buf);
+    for (uint64_t i = 0; i < n; ++i) {
+        const Look l = lookup(&g, ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+        if (val) val[i] = l.value;
+        if (active) active[i] = (uint8_t)l.active;
+        if (dim) dim[i] = l.dim;
+    }
+}
+
+/* ------------------------------------------------------------------ samplers
+ * Floor<Coord>(Vec3f&) math/SampleFromVoxels.h:61-69; Round (rintf) math/Math.h:255-261;
+ * fused non-cached trilinear :423-464 (lerp a + w*(b-a)); TrilinearSampler::gradient :232-271. */
+static inline float lerpf(float a, float b, float w) { return a + w * (b - a); }
+
+static void stencil8(const G* g, int32_t i, int32_t j, int32_t k, float v[2][2][2])
+{
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) v[a][b][c] = get_value(g, i + a, j + b, k + c);
+}
+
+static float trilinear(const G* g, float x, float y, float z)
+{
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const float u = x - fx, v = y - fy, w = z - fz;
+    float s[2][2][2];
+    stencil8(g, (int32_t)fx, (int32_t)fy, (int32_t)fz, s);
+    return lerpf(lerpf(lerpf(s[0][0][0], s[0][0][1], w), lerpf(s[0][1][0], s[0][1][1], w), v),
+                 lerpf(lerpf(s[1][0][0], s[1][0][1], w), lerpf(s[1][1][0], s[1][1][1], w), v), u);
+}
+
+static void trilinear_grad(const G* g, float x, float y, float z, float out[3])
+{
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const float u = x - fx, v = y - fy, w = z - fz;
+    float s[2][2][2];
+    stencil8(g, (int32_t)fx, (int32_t)fy, (int32_t)fz, s);
+    float D[4] = {s[0][0][1] - s[0][0][0], s[0][1][1] - s[0][1][0], s[1][0][1] - s[1][0][0], s[1][1][1] - s[1][1][0]};
+    out[2] = lerpf(lerpf(D[0], D[1], v), lerpf(D[2], D[3], v), u);
+    D[0] = s[0][0][0] + D[0] * w;
+    D[1] = s[0][1][0] + D[1] * w;
+    D[2] = s[1][0][0] + D[2] * w;
+    D[3] = s[1][1][0] + D[3] * w;
+    out[0] = lerpf(D[2], D[3], v) - lerpf(D[0], D[1], v);
+    out[1] = lerpf(D[1] - D[0], D[3] - D[2], u);
+}
+
+/* Grid::worldToIndex -> Map::applyInverseMap with Vec3f: translation subtracted in FP64 and
+ * rounded to float, matMult(const double*, Vec3f) with fma, result rounded to float
+ * (NanoVDB.h:1507-1510, math/Math.h:1718-1723). */
+static void world_to_index_f64map(const G* g, const float in[3], float out[3])
+{
+    const uint8_t* m = g->buf + GRID_MAP;
+    float t[3];
+    for (int a = 0; a < 3; ++a) t[a] = (float)((double)in[a] - rd_f64(m + MAP_VECD + 8 * a));
+    for (int r = 0; r < 3; ++r) {
+        const double m0 = rd_f64(m + MAP_INVMATD + 8 * (3 * r)), m1 = rd_f64(m + MAP_INVMATD + 8 * (3 * r + 1)),
+                     m2 = rd_f64(m + MAP_INVMATD + 8 * (3 * r + 2));
+        out[r] = (float)fma((double)t[0], m0, fma((double)t[1], m1, (double)t[2] * m2));
+    }
+}
+
+typedef struct { float inv2dx, invdx2; } GradC;
+static GradC grad_consts(const G* g)
+{
+    GradC c; /* GradStencil ctor, math/Stencils.h:477-481 */
+    c.inv2dx = (float)(0.5 / rd_f64(g->buf + GRID_VOXELSIZE));
+    c.invdx2 = (float)(4.0 * (double)c.inv2dx * (double)c.inv2dx);
+    return c;
+}
+
+void or_sample(const void* buf, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
+               int grad_mode)
+{
+    const G g = open_grid(buf);
+    /* FD6 scale (BuildT).5/voxelSize() computed in FP64 then rounded, PySampleFromVoxels.cu:42 */
+    float inv2dx[3];
+    for (int a = 0; a < 3; ++a) inv2dx[a] = (float)(0.5 / rd_f64(g.buf + GRID_VOXELSIZE + 8 * a));
+    const GradC gc = grad_consts(&g);
+    for (uint64_t i = 0; i < n; ++i) {
+        float p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        if (world) {
+            float q[3];
+            world_to_index_f64map(&g, p, q);
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+        }
+        if (nn) nn[i] = get_value(&g, (int32_t)rintf(p[0]), (int32_t)rintf(p[1]), (int32_t)rintf(p[2]));
+        if (val) val[i] = trilinear(&g, p[0], p[1], p[2]);
+        if (grad) {
+            float o[3] = {0.0f, 0.0f, 0.0f};
+            if (grad_mode == 1) {
+                const float dx = trilinear(&g, p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f) -
+                                 trilinear(&g, p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f);
+                const float dy = trilinear(&g, p[0] + 0.0f, p[1] + 1.0f, p[2] + 0.0f) -
+                                 trilinear(&g, p[0] - 0.0f, p[1] - 1.0f, p[2] - 0.0f);
+                const float dz = trilinear(&g, p[0] + 0.0f, p[1] + 0.0f, p[2] + 1.0f) -
+                                 trilinear(&g, p[0] - 0.0f, p[1] - 0.0f, p[2] - 1.0f);
+                o[0] = dx * inv2dx[0];
+                o[1] = dy * inv2dx[1];
+                o[2] = dz * inv2dx[2];
+            } else if (grad_mode == 2) {
+                /* GradStencil::moveTo(RoundDown(xyz)) + gradient(), math/Stencils.h:148-152, 502-507 */
+                const int32_t a = (int32_t)floorf(p[0]), b = (int32_t)floorf(p[1]), c = (int32_t)floorf(p[2]);
+                o[0] = gc.inv2dx * (get_value(&g, a + 1, b, c) - get_value(&g, a - 1, b, c));
+                o[1] = gc.inv2dx * (get_value(&g, a, b + 1, c) - get_value(&g, a, b - 1, c));
+                o[2] = gc.inv2dx * (get_value(&g, a, b, c + 1) - get_value(&g, a, b, c - 1));
+            } else if (grad_mode == 3) {
+                trilinear_grad(&g, p[0], p[1], p[2], o);
+            }
+            grad[3 * i] = o[0];
+            grad[3 * i + 1] = o[1];
+            grad[3 * i + 2] = o[2];
+        }
+    }
+}
+
+static double trilinear_f64(const G* g, float x, float y, float z)
+{
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const double u = (double)(x - fx), v = (double)(y - fy), w = (double)(z - fz);
+    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
+    double s[2][2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                const Look l = lookup(g, i + a, j + b, k + c);
+                s[a][b][c] = l.leaf ? leaf_value_f64(g, l.leaf, l.n) : (double)l.value;
+            }
+#define LERPD(a, b, t) ((a) + (t) * ((b) - (a)))
+    return LERPD(LERPD(LERPD(s[0][0][0], s[0][0][1], w), LERPD(s[0][1][0], s[0][1][1], w), v),
+                 LERPD(LERPD(s[1][0][0], s[1][0][1], w), LERPD(s[1][1][0], s[1][1][1], w), v), u);
+}
+
+void or_sample_f64(const void* buf, const float* xyz, uint64_t n, double* val, double* grad_fd6)
+{
+    const G g = open_grid(buf);
+    double inv2dx[3];
+    for (int a = 0; a < 3; ++a) inv2dx[a] = 0.5 / rd_f64(g.buf + GRID_VOXELSIZE + 8 * a);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float* p = xyz + 3 * i;
+        if (val) val[i] = trilinear_f64(&g, p[0], p[1], p[2]);
+        if (grad_fd6) {
+            grad_fd6[3 * i] = (trilinear_f64(&g, p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f) -
+                               trilinear_f64(&g, p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f)) * inv2dx[0];
+            grad_fd6[3 * i + 1] = (trilinear_f64(&g, p[0] + 0.0f, p[1] + 1.0f, p[2] + 0.0f) -
+                                   trilinear_f64(&g, p[0] - 0.0f, p[1] - 1.0f, p[2] - 0.0f)) * inv2dx[1];
+            grad_fd6[3 * i + 2] = (trilinear_f64(&g, p[0] + 0.0f, p[1] + 0.0f, p[2] + 1.0f) -
+                                   trilinear_f64(&g, p[0] - 0.0f, p[1] - 0.0f, p[2] - 1.0f)) * inv2dx[2];
+        }
+    }
+}
+
+/* GradStencil, math/Stencils.h:455-576; GodunovsNormSqrd :62-80 */
+static inline float sqf(float x) { return x * x; }
+void or_grad_stencil(const void* buf, const int32_t* ijk, uint64_t n, float* grad, float* normsq, float* lap,
+                     uint8_t* zc)
+{
+    const G g = open_grid(buf);
+    const GradC gc = grad_consts(&g);
+    for (uint64_t i = 0; i < n; ++i) {
+        const int32_t a = ijk[3 * i], b = ijk[3 * i + 1], c = ijk[3 * i + 2];
+        float v[7];
+        v[0] = get_value(&g, a, b, c);
+        v[1] = get_value(&g, a - 1, b, c);
+        v[2] = get_value(&g, a + 1, b, c);
+        v[3] = get_value(&g, a, b - 1, c);
+        v[4] = get_value(&g, a, b + 1, c);
+        v[5] = get_value(&g, a, b, c - 1);
+        v[6] = get_value(&g, a, b, c + 1);
+        if (grad) {
+            grad[3 * i] = gc.inv2dx * (v[2] - v[1]);
+            grad[3 * i + 1] = gc.inv2dx * (v[4] - v[3]);
+            grad[3 * i + 2] = gc.inv2dx * (v[6] - v[5]);
+        }
+        if (normsq) {
+            const float xm = v[0] - v[1], xp = v[2] - v[0], ym = v[0] - v[3], yp = v[4] - v[0], zm = v[0] - v[5],
+                        zp = v[6] - v[0];
+            float d;
+            if (v[0] > 0.0f) {
+                d = fmaxf(sqf(fmaxf(xm, 0.0f)), sqf(fminf(xp, 0.0f)));
+                d += fmaxf(sqf(fmaxf(ym, 0.0f)), sqf(fminf(yp, 0.0f)));
+                d += fmaxf(sqf(fmaxf(zm, 0.0f)), sqf(fminf(zp, 0.0f)));
+            } else {
+                d = fmaxf(sqf(fminf(xm, 0.0f)), sqf(fmaxf(xp, 0.0f)));
+                d += fmaxf(sqf(fminf(ym, 0.0f)), sqf(fmaxf(yp, 0.0f)));
+                d += fmaxf(sqf(fminf(zm, 0.0f)), sqf(fmaxf(zp, 0.0f)));
+            }
+            normsq[i] = gc.invdx2 * d;
+        }
+        if (lap) lap[i] = gc.invdx2 * (v[1] + v[2] + v[3] + v[4] + v[5] + v[6] - 6.0f * v[0]);
+        if (zc)
+            zc[i] = (uint8_t)(v[0] > 0.0f ? (v[1] < 0 || v[2] < 0 || v[3] < 0 || v[4] < 0 || v[5] < 0 || v[6] < 0)
+                                          : (v[1] > 0 || v[2] > 0 || v[3] > 0 || v[4] > 0 || v[5] > 0 || v[6] > 0));
+    }
+}
+
+/* ------------------------------------------------------------------ Ray
+ * math/Ray.h: ctor :64-74 (invDir = 1/dir), operator() via fmaf :154-163,
+ * worldToIndexF :258-268, intersects(CoordBBox) :398-424 (max padded by +1), clip :506-515.
+ * Maximum<float> is FLT_MAX on the host (math/Math.h:118-124). */
+typedef struct { float eye[3], dir[3], inv[3], t0, t1; } Ray;
+
+static Ray make_ray(const float e[3], const float d[3], float t0, float t1)
+{
+    Ray r;
+    for (int a = 0; a < 3; ++a) {
+        r.eye[a] = e[a];
+        r.dir[a] = d[a];
+        r.inv[a] = 1.0f / d[a];
+    }
+    r.t0 = t0;
+    r.t1 = t1;
+    return r;
+}
+
+static inline void ray_at(const Ray* r, float t, float p[3])
+{
+    p[0] = fmaf(t, r->dir[0], r->eye[0]);
+    p[1] = fmaf(t, r->dir[1], r->eye[1]);
+    p[2] = fmaf(t, r->dir[2], r->eye[2]);
+}
+
+static Ray world_to_index_ray(const G* g, const float e[3], const float d[3])
+{
+    /* Map::applyInverseMapF / applyInverseJacobianF: matMult(const float*, Vec3f) with fmaf,
+     * NanoVDB.h:1518-1539, math/Math.h:1704-1709 */
+    const uint8_t* m = g->buf + GRID_MAP;
+    float M[9], V[3];
+    for (int a = 0; a < 9; ++a) M[a] = rd_f32(m + MAP_INVMATF + 4 * a);
+    for (int a = 0; a < 3; ++a) V[a] = rd_f32(m + MAP_VECF + 4 * a);
+    float t[3] = {e[0] - V[0], e[1] - V[1], e[2] - V[2]}, eye[3], dir[3];
+    for (int r = 0; r < 3; ++r) {
+        eye[r] = fmaf(t[0], M[3 * r], fmaf(t[1], M[3 * r + 1], t[2] * M[3 * r + 2]));
+        dir[r] = fmaf(d[0], M[3 * r], fmaf(d[1], M[3 * r + 1], d[2] * M[3 * r + 2]));
+    }
+    const float length = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const float invLength = 1.0f / length;
+    const float nd[3] = {invLength * dir[0], invLength * dir[1], invLength * dir[2]};
+    const float t0 = 1.0e-5f, t1 = FLT_MAX; /* Delta<float>, Maximum<float> (host) */
+    return make_ray(eye, nd, length * t0, t1);
+}
+
+static int ray_clip(Ray* r, const int32_t* bbox)
+{
+    float t0 = r->t0, t1 = r->t1;
+    for (int i = 0; i < 3; ++i) {
+        float a = (float)bbox[i], b = (float)(bbox[3 + i] + 1);
+        if (a >= b) return 0;
+        a = (a - r->eye[i]) * r->inv[i];
+        b = (b - r->eye[i]) * r->inv[i];
+        if (a > b) { const float tmp = a; a = b; b = tmp; }
+        if (a > t0) t0 = a;
+        if (b < t1) t1 = b;
+        if (t0 > t1) return 0;
+    }
+    r->t0 = t0;
+    r->t1 = t1;
+    return 1;
+}
+
+int or_ray_clip(const float* eye, const float* dir, float t0, float t1, const int32_t* bbox, float* out)
+{
+    Ray r = make_ray(eye, dir, t0, t1);
+    const int hit = ray_clip(&r, bbox);
+    out[0] = r.t0;
+    out[1] = r.t1;
+    return hit;
+}
+
+/* ------------------------------------------------------------------ HDDA, math/HDDA.h:28-179 */
+typedef struct {
+    int32_t dim;
+    float t0, t1;
+    int32_t voxel[3], step[3];
+    float delta[3], next[3];
+} HDDA;
+
+static inline int32_t ifloor(float x) { return (int32_t)floorf(x); }
+
+static void hdda_init(HDDA* h, const Ray* r, float startTime, float maxTime, int dim)
+{
+    h->dim = dim;
+    h->t0 = startTime;
+    h->t1 = maxTime;
+    float pos[3];
+    ray_at(r, h->t0, pos);
+    for (int a = 0; a < 3; ++a) h->voxel[a] = ifloor(pos[a]) & ~(dim - 1);
+    for (int a = 0; a < 3; ++a) {
+        h->delta[a] = 0.0f;
+        if (r->dir[a] == 0.0f) {
+            h->next[a] = FLT_MAX;
+            h->step[a] = 0;
+        } else if (r->inv[a] > 0.0f) {
+            h->step[a] = 1;
+            h->next[a] = h->t0 + ((float)(h->voxel[a] + dim) - pos[a]) * r->inv[a];
+            h->delta[a] = r->inv[a];
+        } else {
+            h->step[a] = -1;
+            h->next[a] = h->t0 + ((float)h->voxel[a] - pos[a]) * r->inv[a];
+            h->delta[a] = -r->inv[a];
+        }
+    }
+}
+
+static int hdda_update(HDDA* h, const Ray* r, int dim)
+{
+    if (h->dim == dim) return 0;
+    int32_t vmax[3], vmin[3];
+    for (int a = 0; a < 3; ++a) {
+        vmax[a] = (h->voxel[a] + (h->dim - 1)) & ~(dim - 1);
+        vmin[a] = h->voxel[a] & ~(dim - 1);
+    }
+    h->dim = dim;
+    float pos[3];
+    ray_at(r, h->t0, pos);
+    for (int a = 0; a < 3; ++a) {
+        int32_t v = ifloor(pos[a]) & ~(dim - 1);
+        v = v < vmax[a] ? v : vmax[a];
+        v = v > vmin[a] ? v : vmin[a];
+        h->voxel[a] = v;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (h->step[a] == 0) continue;
+        h->next[a] = h->t0 + ((float)h->voxel[a] - pos[a]) * r->inv[a];
+        if (h->step[a] > 0) h->next[a] += (float)dim * r->inv[a];
+    }
+    return 1;
+}
+
+static inline int min_index(const float v[3])
+{
+    if (v[0] < v[1] && v[0] < v[2]) return 0;
+    return v[1] < v[2] ? 1 : 2;
+}
+
+static int hdda_step(HDDA* h)
+{
+    const int a = min_index(h->next);
+    if (h->next[a] <= h->t0) h->next[a] += h->t0 - 0.999999f * h->next[a] + 1.0e-6f; /* ENFORCE_FORWARD_STEPPING */
+    h->t0 = h->next[a];
+    h->next[a] += (float)h->dim * h->delta[a];
+    h->voxel[a] += h->dim * h->step[a];
+    return h->t0 <= h->t1;
+}
+
+static inline float hdda_next(const HDDA* h) { return fminf(h->t1, fminf(h->next[0], fminf(h->next[1], h->next[2]))); }
+
+void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,
+                   float* nexts, int32_t* voxels, uint8_t* ok)
+{
+    const Ray r = make_ray(eye, dir, t0, t1);
+    HDDA h;
+    hdda_init(&h, &r, r.t0, r.t1, dim);
+    for (int s = 0; s <= nsteps; ++s) {
+        ok[s] = s > 0 ? (uint8_t)hdda_step(&h) : 1;
+        times[s] = h.t0;
+        nexts[s] = hdda_next(&h);
+        for (int a = 0; a < 3; ++a) voxels[3 * s + a] = h.voxel[a];
+    }
+}
+
+/* ------------------------------------------------------------------ zeroCrossing, math/HDDA.h:189-214 */
+static int zero_crossing(const G* g, Ray* ray, int32_t ijk[3], float* v, float* t)
+{
+    if (!ray_clip(ray, g->bbox) || (double)ray->t1 > 1e20) return 0;
+    const float Delta = 1.0001f;
+    float p[3];
+    ray_at(ray, ray->t0, p);
+    int32_t c[3] = {ifloor(p[0]), ifloor(p[1]), ifloor(p[2])};
+    HDDA h;
+    hdda_init(&h, ray, ray->t0, ray->t1, (int)lookup(g, c[0], c[1], c[2]).dim);
+    const float v0 = get_value(g, c[0], c[1], c[2]);
+    while (hdda_step(&h)) {
+        ray_at(ray, h.t0 + Delta, p);
+        c[0] = ifloor(p[0]);
+        c[1] = ifloor(p[1]);
+        c[2] = ifloor(p[2]);
+        const Look l = lookup(g, c[0], c[1], c[2]);
+        hdda_update(&h, ray, (int)l.dim);
+        if (h.dim > 1 || !l.active) continue;
+        while (hdda_step(&h)) {
+            const Look lv = lookup(g, h.voxel[0], h.voxel[1], h.voxel[2]);
+            if (!lv.active) break;
+            *v = lv.value;
+            if (*v * v0 < 0) {
+                ijk[0] = h.voxel[0];
+                ijk[1] = h.voxel[1];
+                ijk[2] = h.voxel[2];
+                *t = h.t0;
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
+                      float* t, float* v, float* normal)
+{
+    const G g = open_grid(buf);
+    const uint8_t* m = g.buf + GRID_MAP;
+    float IM[9];
+    for (int a = 0; a < 9; ++a) IM[a] = rd_f32(m + MAP_INVMATF + 4 * a);
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r = world_to_index_ray(&g, eye + 3 * i, dir + 3 * i);
+        int32_t c[3] = {0, 0, 0};
+        float vv = 0.0f, tt = 0.0f;
+        const int h = zero_crossing(&g, &r, c, &vv, &tt);
+        if (hit) hit[i] = (uint8_t)h;
+        if (ijk) for (int a = 0; a < 3; ++a) ijk[3 * i + a] = h ? c[a] : 0;
+        if (t) t[i] = h ? tt : 0.0f;
+        if (v) v[i] = h ? vv : 0.0f;
+        if (normal) {
+            float nr[3] = {0.0f, 0.0f, 0.0f};
+            if (h) {
+                float p[3], gi[3], gw[3];
+                ray_at(&r, tt, p);
+                trilinear_grad(&g, p[0], p[1], p[2], gi);
+                /* Grid::indexToWorldGradF -> Map::applyIJTF -> matMultT(const float*, Vec3f), math/Math.h:1758-1763 */
+                for (int a = 0; a < 3; ++a) gw[a] = fmaf(gi[0], IM[a], fmaf(gi[1], IM[3 + a], gi[2] * IM[6 + a]));
+                const float len = sqrtf(gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2]);
+                if (len > 0.0f) {
+                    const float s = 1.0f / len;
+                    nr[0] = s * gw[0];
+                    nr[1] = s * gw[1];
+                    nr[2] = s * gw[2];
+                }
+            }
+            for (int a = 0; a < 3; ++a) normal[3 * i + a] = nr[a];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ fog march
+ * Emission-absorption march over HDDA cells of size >= 8 (the leaf level, as TreeMarcher,
+ * math/HDDA.h:450-536, with Eps = 1e-6 and Delta = 0.01).  A cell is marched when it lies in a
+ * leaf (getDim == 1) or an active tile.  Steps are globally aligned at t = k*dt, ta <= t < tb.
+ * Per step: rho = trilinear(x(t)), a = exp(-sigma*rho*dt), L += T*(1-a)*rho, T *= a
+ * (emission-absorption as openvdb/openvdb/tools/RayTracer.h:993-1064 with density emission).
+ * The step structure is FP32 (identical to the GPU path); the arithmetic here is FP64. */
+typedef struct {
+    float u, v, w;     /* fractional position */
+    int32_t c[3];      /* base voxel */
+} Step;
+
+typedef void (*step_fn)(const G* g, const Ray* r, float t, void* ctx);
+
+static void march_structure(const G* g, const float* eye, const float* dir, float dt, step_fn fn, void* ctx)
+{
+    Ray R = world_to_index_ray(g, eye, dir);
+    if (!ray_clip(&R, g->bbox) || (double)R.t1 > 1e20) return;
+    const float Eps = 0.000001f, Delta = 0.01f;
+    const float t0 = R.t0 + Eps, t1 = R.t1 - Eps;
+    if (t0 > t1) return;
+    float p[3];
+    ray_at(&R, t0, p);
+    Look l = lookup(g, ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+    HDDA h;
+    hdda_init(&h, &R, t0, t1, (int)(l.dim > 8u ? l.dim : 8u));
+    int on = (l.dim == 1u) || l.active;
+    for (;;) {
+        const float ta = h.t0;
+        const int valid = hdda_step(&h);
+        const float tb = fminf(h.t0, t1);
+        if (on && ta < tb) {
+            float kf = ceilf(ta / dt);
+            for (float t = kf * dt; t < tb; kf += 1.0f, t = kf * dt) fn(g, &R, t, ctx);
+        }
+        if (!valid) break;
+        ray_at(&R, h.t0 + Delta, p);
+        l = lookup(g, ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+        hdda_update(&h, &R, (int)(l.dim > 8u ? l.dim : 8u));
+        on = (l.dim == 1u) || l.active;
+    }
+}
+
+typedef struct {
+    double sigma_dt, L, T;
+    uint32_t steps;
+    /* adjoint recording */
+    int record;
+    size_t cap, count;
+    double* rho;
+    double* a;
+    double* Tk;
+    Step* st;
+} MarchCtx;
+
+static void fwd_step(const G* g, const Ray* r, float t, void* vctx)
+{
+    MarchCtx* m = (MarchCtx*)vctx;
+    float p[3];
+    ray_at(r, t, p);
+    const double rho = trilinear_f64(g, p[0], p[1], p[2]);
+    const double a = exp(-m->sigma_dt * rho);
+    if (m->record) {
+        if (m->count == m->cap) {
+            m->cap = m->cap ? 2 * m->cap : 1024;
+            m->rho = (double*)realloc(m->rho, m->cap * sizeof(double));
+            m->a = (double*)realloc(m->a, m->cap * sizeof(double));
+            m->Tk = (double*)realloc(m->Tk, m->cap * sizeof(double));
+            m->st = (Step*)realloc(m->st, m->cap * sizeof(Step));
+        }
+        const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+        Step s;
+        s.u = p[0] - fx; s.v = p[1] - fy; s.w = p[2] - fz;
+        s.c[0] = (int32_t)fx; s.c[1] = (int32_t)fy; s.c[2] = (int32_t)fz;
+        m->rho[m->count] = rho;
+        m->a[m->count] = a;
+        m->Tk[m->count] = m->T;
+        m->st[m->count] = s;
+        ++m->count;
+    }
+    m->L += m->T * (1.0 - a) * rho;
+    m->T *= a;
+    ++m->steps;
+}
+
+void or_fog_march(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma, double* L,
+                  double* T, uint32_t* steps)
+{
+    const G g = open_grid(buf);
+    for (uint64_t i = 0; i < n; ++i) {
+        MarchCtx m;
+        memset(&m, 0, sizeof(m));
+        m.sigma_dt = (double)sigma * (double)dt;
+        m.T = 1.0;
+        march_structure(&g, eye + 3 * i, dir + 3 * i, dt, fwd_step, &m);
+        if (L) L[i] = m.L;
+        if (T) T[i] = m.T;
+        if (steps) steps[i] = m.steps;
+    }
+}
+
+/* dLoss/drho_k = dL*(T_k[(1-a_k) + rho_k*s*a_k] - s*R_{>k}) - dT*s*T_N, with s = sigma*dt and
+ * R_{>k} = sum_{j>k} T_j (1-a_j) rho_j; scattered with the trilinear weights of step k onto
+ * the corner voxels that live in leaf nodes (grad index = leafIndex*512 + n). */
+void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma,
+                          const double* dL, const double* dT, double* grad)
+{
+    const G g = open_grid(buf);
+    if (g.leafSize == 0) return;
+    MarchCtx m;
+    memset(&m, 0, sizeof(m));
+    m.record = 1;
+    for (uint64_t i = 0; i < n; ++i) {
+        m.count = 0;
+        m.L = 0.0;
+        m.T = 1.0;
+        m.steps = 0;
+        m.sigma_dt = (double)sigma * (double)dt;
+        march_structure(&g, eye + 3 * i, dir + 3 * i, dt, fwd_step, &m);
+        const double s = m.sigma_dt, TN = m.T, gl = dL ? dL[i] : 0.0, gt = dT ? dT[i] : 0.0;
+        double R = 0.0; /* suffix radiance after step k */
+        for (size_t k = m.count; k-- > 0;) {
+            const double rho = m.rho[k], a = m.a[k], Tk = m.Tk[k];
+            const double gk = gl * (Tk * ((1.0 - a) + rho * s * a) - s * R) - gt * s * TN;
+            R += Tk * (1.0 - a) * rho;
+            const Step st = m.st[k];
+            const double wx[2] = {1.0 - st.u, st.u}, wy[2] = {1.0 - st.v, st.v}, wz[2] = {1.0 - st.w, st.w};
+            for (int a0 = 0; a0 < 2; ++a0)
+                for (int b0 = 0; b0 < 2; ++b0)
+                    for (int c0 = 0; c0 < 2; ++c0) {
+                        const Look l = lookup(&g, st.c[0] + a0, st.c[1] + b0, st.c[2] + c0);
+                        if (!l.leaf) continue;
+                        const uint64_t li = (uint64_t)(l.leaf - g.leaf0) / g.leafSize;
+                        grad[li * 512u + l.n] += gk * wx[a0] * wy[b0] * wz[c0];
+                    }
+        }
+    }
+    free(m.rho);
+    free(m.a);
+    free(m.Tk);
+    free(m.st);
+}

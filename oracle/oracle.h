@@ -1,0 +1,61 @@
+/* Oracle: CPU restatement of NanoVDB's read-only query path, used ONLY as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg.
+ * It is never linked into, loaded by or called from the product library.
+ *
+ * Everything reads the flat NanoVDB buffer by byte offsets (the layout pinned
+ * by nanovdb/nanovdb/NanoVDB.h and its C99 port nanovdb/nanovdb/PNanoVDB.h).
+ * FP32 functions reproduce the reference's float operation order exactly
+ * (built with -ffp-contract=off, fmaf only where the reference writes fmaf);
+ * the *_f64 functions keep the FP32 structural decisions (cells, steps,
+ * stencils) and evaluate the arithmetic in FP64.
+ */
+#ifndef NVQ_ORACLE_H
+#define NVQ_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 0 ok, <0 invalid (magic / version / unsupported type / truncated) */
+int or_validate(const void* buf, uint64_t bytes);
+uint32_t or_grid_type(const void* buf);
+uint32_t or_leaf_count(const void* buf);
+
+/* ReadAccessor::getValue / isActive / getDim(ijk, ray) per voxel */
+void or_get_value(const void* buf, const int32_t* ijk, uint64_t n, float* val, uint8_t* active, uint32_t* dim);
+
+/* nn = NN sampler (rintf), val = trilinear, grad per grad_mode
+ * (1 FD6 of trilinear * 0.5/voxelSize, 2 GradStencil central difference, 3 trilinear analytic, index space).
+ * world=1 maps xyz with the FP64 inverse map (Grid::worldToIndex). */
+void or_sample(const void* buf, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
+               int grad_mode);
+/* FP64 evaluation: corner values decoded in FP64 (code*quantum+min) and FP64 lerps over
+ * the same FP32 positions; used to split arithmetic error from codec error. */
+void or_sample_f64(const void* buf, const float* xyz, uint64_t n, double* val, double* grad_fd6);
+
+/* GradStencil at integer voxels: gradient, normSqGrad (Godunov), laplacian, zeroCrossing */
+void or_grad_stencil(const void* buf, const int32_t* ijk, uint64_t n, float* grad, float* normsq, float* lap,
+                     uint8_t* zc);
+
+/* world rays -> worldToIndexF -> zeroCrossing (+ normalised world-space trilinear-gradient normal) */
+void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
+                      float* t, float* v, float* normal);
+
+/* Emission-absorption fog march: FP32 HDDA/step structure, FP64 (L,T); steps per ray. */
+void or_fog_march(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma, double* L,
+                  double* T, uint32_t* steps);
+/* FP64 adjoint of or_fog_march: accumulates dLoss/dvalue into grad (leafCount*512 doubles,
+ * leaf order of the buffer) given dLoss/dL and dLoss/dT per ray (dT may be NULL). */
+void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma,
+                          const double* dL, const double* dT, double* grad);
+
+/* Known-answer helpers */
+void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,
+                   float* nexts, int32_t* voxels, uint8_t* ok);
+int or_ray_clip(const float* eye, const float* dir, float t0, float t1, const int32_t* bbox, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

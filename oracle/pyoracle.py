@@ -1,0 +1,295 @@
+"""ctypes front-end for the two checkers in this folder (test infrastructure only).
+
+* ``Oracle`` wraps ``liboracle.so``: the C restatement (oracle.c) of NanoVDB's
+  query path.
+* ``Reference`` wraps ``_ref/libnanoref.so``: the reference NanoVDB host code
+  compiled from the reference's own headers (ref_wrap.cpp).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+import this module; the product package never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libnanoref.so")
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_i = ctypes.c_int
+_f = ctypes.c_float
+_d = ctypes.c_double
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def aligned_empty(nbytes):
+    """Uninitialised uint8 array whose data is 32-byte aligned (NanoVDB.h:69-74)."""
+    raw = np.empty(nbytes + 32, np.uint8)
+    off = (-raw.ctypes.data) % 32
+    return raw[off:off + nbytes]
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        L = ctypes.CDLL(path)
+        self.L = L
+        L.or_validate.argtypes = [_vp, _u64]
+        L.or_grid_type.argtypes = [_vp]
+        L.or_grid_type.restype = ctypes.c_uint32
+        L.or_leaf_count.argtypes = [_vp]
+        L.or_leaf_count.restype = ctypes.c_uint32
+        L.or_get_value.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp]
+        L.or_sample.argtypes = [_vp, _vp, _u64, _i, _vp, _vp, _vp, _i]
+        L.or_sample_f64.argtypes = [_vp, _vp, _u64, _vp, _vp]
+        L.or_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
+        L.or_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]
+        L.or_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
+        L.or_fog_march_adjoint.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
+        L.or_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
+        L.or_ray_clip.argtypes = [_vp, _vp, _f, _f, _vp, _vp]
+
+    def validate(self, buf):
+        return self.L.or_validate(_p(buf), buf.nbytes)
+
+    def leaf_count(self, buf):
+        return self.L.or_leaf_count(_p(buf))
+
+    def get_value(self, buf, ijk):
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        n = ijk.shape[0]
+        val = np.empty(n, np.float32)
+        act = np.empty(n, np.uint8)
+        dim = np.empty(n, np.uint32)
+        self.L.or_get_value(_p(buf), _p(ijk), n, _p(val), _p(act), _p(dim))
+        return val, act, dim
+
+    def sample(self, buf, xyz, world=False, grad_mode=1):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        nn = np.empty(n, np.float32)
+        val = np.empty(n, np.float32)
+        grad = np.empty((n, 3), np.float32)
+        self.L.or_sample(_p(buf), _p(xyz), n, int(world), _p(nn), _p(val), _p(grad), grad_mode)
+        return nn, val, grad
+
+    def sample_f64(self, buf, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        val = np.empty(n, np.float64)
+        grad = np.empty((n, 3), np.float64)
+        self.L.or_sample_f64(_p(buf), _p(xyz), n, _p(val), _p(grad))
+        return val, grad
+
+    def grad_stencil(self, buf, ijk):
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        n = ijk.shape[0]
+        g = np.empty((n, 3), np.float32)
+        ns = np.empty(n, np.float32)
+        lap = np.empty(n, np.float32)
+        zc = np.empty(n, np.uint8)
+        self.L.or_grad_stencil(_p(buf), _p(ijk), n, _p(g), _p(ns), _p(lap), _p(zc))
+        return g, ns, lap, zc
+
+    def zero_crossing(self, buf, eye, dir):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        hit = np.empty(n, np.uint8)
+        ijk = np.empty((n, 3), np.int32)
+        t = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        nrm = np.empty((n, 3), np.float32)
+        self.L.or_zero_crossing(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm))
+        return hit, ijk, t, v, nrm
+
+    def fog_march(self, buf, eye, dir, dt, sigma):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        L = np.empty(n, np.float64)
+        T = np.empty(n, np.float64)
+        steps = np.empty(n, np.uint32)
+        self.L.or_fog_march(_p(buf), _p(eye), _p(dir), n, dt, sigma, _p(L), _p(T), _p(steps))
+        return L, T, steps
+
+    def fog_march_adjoint(self, buf, eye, dir, dt, sigma, dL, dT=None):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        dL = np.ascontiguousarray(dL, np.float64)
+        dT = None if dT is None else np.ascontiguousarray(dT, np.float64)
+        grad = np.zeros(self.leaf_count(buf) * 512, np.float64)
+        self.L.or_fog_march_adjoint(_p(buf), _p(eye), _p(dir), n, dt, sigma, _p(dL), _p(dT), _p(grad))
+        return grad
+
+    def hdda_trace(self, eye, dir, t0, t1, dim, nsteps):
+        eye = np.asarray(eye, np.float32)
+        dir = np.asarray(dir, np.float32)
+        times = np.empty(nsteps + 1, np.float32)
+        nexts = np.empty(nsteps + 1, np.float32)
+        vox = np.empty((nsteps + 1, 3), np.int32)
+        ok = np.empty(nsteps + 1, np.uint8)
+        self.L.or_hdda_trace(_p(eye), _p(dir), t0, t1, dim, nsteps, _p(times), _p(nexts), _p(vox), _p(ok))
+        return times, nexts, vox, ok
+
+    def ray_clip(self, eye, dir, t0, t1, bbox):
+        out = np.empty(2, np.float32)
+        hit = self.L.or_ray_clip(_p(np.asarray(eye, np.float32)), _p(np.asarray(dir, np.float32)), t0, t1,
+                                 _p(np.asarray(bbox, np.int32)), _p(out))
+        return bool(hit), float(out[0]), float(out[1])
+
+
+class Reference:
+    """The reference NanoVDB host code (compiled from the reference headers)."""
+
+    FLOAT, FP4, FP8, FP16, FPN = 0, 1, 2, 3, 4
+    VOXEL_FN = ctypes.CFUNCTYPE(ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+
+    def __init__(self, path=REF_SO):
+        L = ctypes.CDLL(path)
+        self.L = L
+        pu = ctypes.POINTER(_u64)
+        L.ref_hw_threads.restype = ctypes.c_uint
+        L.ref_free.argtypes = [_vp]
+        L.ref_level_set_sphere.restype = _vp
+        L.ref_level_set_sphere.argtypes = [_i, _d, _d, _d, _d, _d, _d, _f, pu]
+        L.ref_level_set_torus.restype = _vp
+        L.ref_level_set_torus.argtypes = [_i, _d, _d, _d, _d, _d, _d, _d, _f, pu]
+        L.ref_fog_volume_sphere.restype = _vp
+        L.ref_fog_volume_sphere.argtypes = [_d, _d, _d, _d, _d, _d, pu]
+        L.ref_grid_from_function.restype = _vp
+        L.ref_grid_from_function.argtypes = [self.VOXEL_FN, _vp, _vp, _f, _i, pu]
+        L.ref_quantize.restype = _vp
+        L.ref_quantize.argtypes = [_vp, _i, _f, pu]
+        L.ref_get_value.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _i]
+        L.ref_sample.argtypes = [_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _i]
+        L.ref_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
+        L.ref_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _i]
+        L.ref_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp, _vp, _vp, _i]
+        L.ref_ray_gen.argtypes = [_i, _i, _f, _f, _f, _f, _vp, _vp]
+        L.ref_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
+        L.ref_ray_clip.argtypes = [_vp, _vp, _f, _f, _vp, _vp]
+
+    def hw_threads(self):
+        return int(self.L.ref_hw_threads())
+
+    def _take(self, ptr, n):
+        if not ptr:
+            raise RuntimeError("reference grid construction failed")
+        res = aligned_empty(n)
+        ctypes.memmove(res.ctypes.data, ptr, n)
+        self.L.ref_free(ptr)
+        return res
+
+    def level_set_sphere(self, radius, center=(0, 0, 0), voxel=1.0, half_width=3.0, kind=0, tol=1e-3):
+        n = _u64()
+        p = self.L.ref_level_set_sphere(kind, radius, *center, voxel, half_width, tol, ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def level_set_torus(self, R, r, center=(0, 0, 0), voxel=1.0, half_width=3.0, kind=0, tol=1e-3):
+        n = _u64()
+        p = self.L.ref_level_set_torus(kind, R, r, *center, voxel, half_width, tol, ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def fog_volume_sphere(self, radius, center=(0, 0, 0), voxel=1.0, half_width=3.0):
+        n = _u64()
+        p = self.L.ref_fog_volume_sphere(radius, *center, voxel, half_width, ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def grid_from_function(self, fn, bbox, background=0.0, grid_class=2):
+        """fn(i, j, k) -> float, evaluated by build::Grid::operator()(func, bbox)."""
+        cb = self.VOXEL_FN(lambda i, j, k, _ctx: fn(i, j, k))
+        n = _u64()
+        p = self.L.ref_grid_from_function(cb, None, _p(np.asarray(bbox, np.int32)), background, grid_class,
+                                          ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def quantize(self, buf, kind, tol=1e-3):
+        n = _u64()
+        p = self.L.ref_quantize(_p(buf), kind, tol, ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def get_value(self, buf, ijk, nthreads=1):
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        n = ijk.shape[0]
+        val = np.empty(n, np.float32)
+        act = np.empty(n, np.uint8)
+        dim = np.empty(n, np.uint32)
+        self.L.ref_get_value(_p(buf), _p(ijk), n, _p(val), _p(act), _p(dim), nthreads)
+        return val, act, dim
+
+    def sample(self, buf, xyz, world=False, grad_mode=1, nthreads=1, nn=True, val=True, grad=True):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        o_nn = np.empty(n, np.float32) if nn else None
+        o_val = np.empty(n, np.float32) if val else None
+        o_grad = np.empty((n, 3), np.float32) if grad else None
+        self.L.ref_sample(_p(buf), _p(xyz), n, int(world), _p(o_nn), _p(o_val), _p(o_grad), grad_mode, nthreads)
+        return o_nn, o_val, o_grad
+
+    def grad_stencil(self, buf, ijk):
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        n = ijk.shape[0]
+        g = np.empty((n, 3), np.float32)
+        ns = np.empty(n, np.float32)
+        lap = np.empty(n, np.float32)
+        zc = np.empty(n, np.uint8)
+        self.L.ref_grad_stencil(_p(buf), _p(ijk), n, _p(g), _p(ns), _p(lap), _p(zc))
+        return g, ns, lap, zc
+
+    def zero_crossing(self, buf, eye, dir, nthreads=1):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        hit = np.empty(n, np.uint8)
+        ijk = np.empty((n, 3), np.int32)
+        t = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        nrm = np.empty((n, 3), np.float32)
+        self.L.ref_zero_crossing(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm), nthreads)
+        return hit, ijk, t, v, nrm
+
+    def fog_march(self, buf, eye, dir, dt, sigma, nthreads=1, fp32=True, fp64=True):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        L = np.empty(n, np.float32) if fp32 else None
+        T = np.empty(n, np.float32) if fp32 else None
+        L64 = np.empty(n, np.float64) if fp64 else None
+        T64 = np.empty(n, np.float64) if fp64 else None
+        steps = np.empty(n, np.uint32)
+        self.L.ref_fog_march(_p(buf), _p(eye), _p(dir), n, dt, sigma, _p(L), _p(T), _p(L64), _p(T64), _p(steps),
+                             nthreads)
+        return L, T, L64, T64, steps
+
+    def ray_gen(self, w, h, dimz, center=(0, 0, 0)):
+        eye = np.empty((w * h, 3), np.float32)
+        dir = np.empty((w * h, 3), np.float32)
+        self.L.ref_ray_gen(w, h, dimz, *center, _p(eye), _p(dir))
+        return eye, dir
+
+    def hdda_trace(self, eye, dir, t0, t1, dim, nsteps):
+        eye = np.asarray(eye, np.float32)
+        dir = np.asarray(dir, np.float32)
+        times = np.empty(nsteps + 1, np.float32)
+        nexts = np.empty(nsteps + 1, np.float32)
+        vox = np.empty((nsteps + 1, 3), np.int32)
+        ok = np.empty(nsteps + 1, np.uint8)
+        self.L.ref_hdda_trace(_p(eye), _p(dir), t0, t1, dim, nsteps, _p(times), _p(nexts), _p(vox), _p(ok))
+        return times, nexts, vox, ok
+
+    def ray_clip(self, eye, dir, t0, t1, bbox):
+        out = np.empty(2, np.float32)
+        hit = self.L.ref_ray_clip(_p(np.asarray(eye, np.float32)), _p(np.asarray(dir, np.float32)), t0, t1,
+                                  _p(np.asarray(bbox, np.int32)), _p(out))
+        return bool(hit), float(out[0]), float(out[1])
+
+
+def have_reference():
+    return os.path.exists(REF_SO)

@@ -1,0 +1,75 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU; run on the GPU box")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from pyoracle import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def ref():
+    from pyoracle import Reference, have_reference
+    if not have_reference():
+        pytest.skip("oracle/_ref/libnanoref.so not built (needs the reference sources)")
+    return Reference()
+
+
+@pytest.fixture(scope="session")
+def sphere(ref):
+    """createLevelSetSphere<float>(100, 0, 1, 3): the level-set ray-hit grid."""
+    return ref.level_set_sphere(100.0, (0, 0, 0), 1.0, 3.0)
+
+
+@pytest.fixture(scope="session")
+def torus(ref):
+    """createLevelSetTorus<float>(300, 80, 0, 1, 3): the point-query grid."""
+    return ref.level_set_torus(300.0, 80.0, (0, 0, 0), 1.0, 3.0)
+
+
+@pytest.fixture(scope="session")
+def fog_sphere(ref):
+    """createFogVolumeSphere<float>(60, (3,-2,5), 1, 3): fog with active interior tiles."""
+    return ref.fog_volume_sphere(60.0, (3.0, -2.0, 5.0), 1.0, 3.0)
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+def points_near(oracle, grid, n, seed, bbox_pad=4, sigma=2.0):
+    """Half uniform in the (padded) index bbox, half Gaussian around active voxels."""
+    from pyoracle import Oracle  # noqa: F401
+    r = rng(seed)
+    lo, hi = grid_bbox(grid)
+    lo = lo - bbox_pad
+    hi = hi + bbox_pad
+    a = r.random((n // 2, 3)) * (hi - lo + 1) + lo
+    cand = r.integers(lo, hi + 1, size=(n * 4, 3)).astype(np.int32)
+    _, act, _ = oracle.get_value(grid, cand)
+    act_pts = cand[act.astype(bool)]
+    if len(act_pts) == 0:
+        act_pts = cand
+    pick = act_pts[r.integers(0, len(act_pts), size=n - n // 2)]
+    b = pick + r.normal(0.0, sigma, size=pick.shape)
+    return np.concatenate([a, b]).astype(np.float32)
+
+
+def grid_bbox(buf):
+    tree = 672
+    off_root = int(np.frombuffer(buf[tree + 24:tree + 32].tobytes(), np.int64)[0])
+    root = tree + off_root
+    bb = np.frombuffer(buf[root:root + 24].tobytes(), np.int32)
+    return bb[:3].astype(np.int64), bb[3:].astype(np.int64)
