@@ -384,13 +384,16 @@ __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* _
                 const float tb = fminf(h.t0, t1);
                 if (on && ta < tb) {
                     float kf = ceilf(ta / dt);
-                    for (float t = kf * dt; t < tb; kf += 1.0f, t = kf * dt) {
+                    for (float t = kf * dt; t < tb; t = kf * dt) {
                         float x[3];
                         rayAt(R, t, x);
                         const float rho = trilinear(acc, x[0], x[1], x[2]);
                         const float a = expf(-sigma * rho * dt);
                         L += T * (1.0f - a) * rho;
                         T *= a;
+                        const float kn = kf + 1.0f;
+                        if (kn == kf) break; // step index beyond 2^24: float cannot advance it
+                        kf = kn;
                     }
                 }
                 if (!valid) break;

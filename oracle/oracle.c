@@ -664,7 +664,12 @@ static void march_structure(const G* g, const float* eye, const float* dir, floa
         const float tb = fminf(h.t0, t1);
         if (on && ta < tb) {
             float kf = ceilf(ta / dt);
-            for (float t = kf * dt; t < tb; kf += 1.0f, t = kf * dt) fn(g, &R, t, ctx);
+            for (float t = kf * dt; t < tb; t = kf * dt) {
+                fn(g, &R, t, ctx);
+                const float kn = kf + 1.0f;
+                if (kn == kf) break; /* step index beyond 2^24: float cannot advance it */
+                kf = kn;
+            }
         }
         if (!valid) break;
         ray_at(&R, h.t0 + Delta, p);

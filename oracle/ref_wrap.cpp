@@ -123,7 +123,7 @@ void fogMarch(const GridT* grid, const math::Ray<float>& wRay, float dt, float s
         const float tb = math::Min(hdda.time(), t1);
         if (on && ta < tb) {
             float kf = std::ceil(ta / dt);
-            for (float t = kf * dt; t < tb; kf += 1.0f, t = kf * dt) {
+            for (float t = kf * dt; t < tb; t = kf * dt) {
                 const Vec3f x = R(t);
                 RealT rho;
                 if constexpr (sizeof(RealT) == 4) {
@@ -146,6 +146,9 @@ void fogMarch(const GridT* grid, const math::Ray<float>& wRay, float dt, float s
                 L += T * (RealT(1) - a) * rho;
                 T *= a;
                 ++nsteps;
+                const float kn = kf + 1.0f;
+                if (kn == kf) break; // step index beyond 2^24: float cannot advance it
+                kf = kn;
             }
         }
         if (!valid) break;
