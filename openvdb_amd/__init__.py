@@ -5,6 +5,6 @@ host side: grid upload (``DeviceGrid``) and batched queries (``ops``).
 """
 from ._lib import LIB_PATH, NvqError, lib  # noqa: F401
 from .grid import DeviceGrid, aligned_buffer  # noqa: F401
-from . import ops  # noqa: F401
+from . import ops, tools  # noqa: F401
 
-__all__ = ["DeviceGrid", "NvqError", "aligned_buffer", "ops", "lib", "LIB_PATH"]
+__all__ = ["DeviceGrid", "NvqError", "aligned_buffer", "ops", "tools", "lib", "LIB_PATH"]

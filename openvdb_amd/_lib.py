@@ -15,6 +15,7 @@ _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
 _i = ctypes.c_int
 _f = ctypes.c_float
+_d = ctypes.c_double
 
 
 class GridInfo(ctypes.Structure):
@@ -43,6 +44,15 @@ SIGNATURES = {
     "nvq_grad_stencil": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_zero_crossing": ([_vp, _vp, _vp, _u64, _u32, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp], _i),
+    # host builder (include/nvq_build.h)
+    "nvqb_level_set_sphere": ([_d, _vp, _d, _d, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
+    "nvqb_level_set_torus": ([_d, _d, _vp, _d, _d, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
+    "nvqb_fog_volume_sphere": ([_d, _vp, _d, _d, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
+    "nvqb_fog_fbm": ([_vp, _u32, _d, _u64, _f, _f, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
+    "nvqb_fbm_density": ([_vp, _u64, _u32, _d, _u64, _f, _f, _vp], _i),
+    "nvqb_quantize": ([_vp, _u64, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
+    "nvqb_camera_rays": ([_u32, _u32, _vp, _vp, _vp, _f, _vp, _vp], _i),
+    "nvqb_free": ([_vp], None),
 }
 
 _lib = None

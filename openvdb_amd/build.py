@@ -14,19 +14,34 @@ LIB = os.path.join(HERE, "libnvq.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["nvq_kernels.hip"]
-HOST_SOURCES = []
+HOST_SOURCES = ["nvq_build.cpp"]  # host-only C++ (grid builder), compiled by g++
+OBJ_DIR = os.path.join(HERE, "_build")
+HOST_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-pthread", "-Wall", "-Wextra",
+              "-I", os.path.join(ROOT, "include")]
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES + HOST_SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def build(verbose=False):
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs + ["-pthread"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,93 @@
+"""Host grid producers and camera rays (libnvq.so's builder, include/nvq_build.h).
+
+Names mirror the reference's tools (tools/CreatePrimitives.h, tools/CreateNanoGrid.h):
+each returns a 32-byte aligned uint8 numpy array holding one NanoVDB grid buffer,
+ready for ``DeviceGrid``.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+
+GRID_TYPE = {"float": 1, "fp4": 13, "fp8": 14, "fp16": 15, "fpn": 16}
+
+
+def _gt(grid_type):
+    if isinstance(grid_type, int):
+        return grid_type
+    return GRID_TYPE[grid_type.lower()]
+
+
+def _take(ptr, nbytes):
+    raw = np.empty(nbytes.value + 32, np.uint8)
+    off = (-raw.ctypes.data) % 32
+    out = raw[off:off + nbytes.value]
+    ctypes.memmove(out.ctypes.data, ptr, nbytes.value)
+    lib().nvqb_free(ptr)
+    return out
+
+
+def _call(fn, *args):
+    p = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    check(fn(*args, ctypes.byref(p), ctypes.byref(n)), fn.__name__)
+    return _take(p, n)
+
+
+def _d3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def create_level_set_sphere(radius=100.0, center=(0.0, 0.0, 0.0), voxel_size=1.0, half_width=3.0,
+                            grid_type="float", tol=1e-3):
+    """createLevelSetSphere (tools/CreatePrimitives.h:1016-1100)."""
+    return _call(lib().nvqb_level_set_sphere, radius, _d3(center), voxel_size, half_width, _gt(grid_type), tol)
+
+
+def create_level_set_torus(major_radius=100.0, minor_radius=50.0, center=(0.0, 0.0, 0.0), voxel_size=1.0,
+                           half_width=3.0, grid_type="float", tol=1e-3):
+    """createLevelSetTorus (tools/CreatePrimitives.h:1186-1243)."""
+    return _call(lib().nvqb_level_set_torus, major_radius, minor_radius, _d3(center), voxel_size, half_width,
+                 _gt(grid_type), tol)
+
+
+def create_fog_volume_sphere(radius=100.0, center=(0.0, 0.0, 0.0), voxel_size=1.0, half_width=3.0):
+    """createFogVolumeSphere<float> (tools/CreatePrimitives.h:1103-1130)."""
+    return _call(lib().nvqb_fog_volume_sphere, radius, _d3(center), voxel_size, half_width)
+
+
+def create_fog_fbm(bbox, octaves=6, base_freq=1.0 / 64.0, seed=7, amplitude=1.0, threshold=0.15, grid_type="float",
+                   tol=1e-3):
+    """Fog volume of clamped fBm improved-Perlin noise filled like build::Grid::operator()(func, bbox)."""
+    bb = (ctypes.c_int32 * 6)(*[int(x) for x in bbox])
+    return _call(lib().nvqb_fog_fbm, bb, octaves, base_freq, seed, amplitude, threshold, _gt(grid_type), tol)
+
+
+def fbm_density(ijk, octaves=6, base_freq=1.0 / 64.0, seed=7, amplitude=1.0, threshold=0.15):
+    ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+    out = np.empty(len(ijk), np.float32)
+    check(lib().nvqb_fbm_density(ijk.ctypes.data_as(ctypes.c_void_p), len(ijk), octaves, base_freq, seed, amplitude,
+                                 threshold, out.ctypes.data_as(ctypes.c_void_p)), "nvqb_fbm_density")
+    return out
+
+
+def quantize(float_grid, grid_type, tol=1e-3):
+    """createNanoGrid<NanoGrid<float>, Fp4|Fp8|Fp16|FpN> (tools/CreateNanoGrid.h:1331-1474)."""
+    buf = np.ascontiguousarray(float_grid)
+    return _call(lib().nvqb_quantize, buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes, _gt(grid_type), tol)
+
+
+def camera_rays(width, height, eye, look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=45.0):
+    """Pinhole rays with the RayGenOp convention (examples/ex_raytrace_level_set/common.h:91-125)."""
+    n = width * height
+    e = np.empty((n, 3), np.float32)
+    d = np.empty((n, 3), np.float32)
+    check(lib().nvqb_camera_rays(width, height, _f3(eye), _f3(look_at), _f3(up), fov_deg,
+                                 e.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p)),
+          "nvqb_camera_rays")
+    return e, d

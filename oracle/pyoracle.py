@@ -165,6 +165,8 @@ class Reference:
         L.ref_fog_volume_sphere.argtypes = [_d, _d, _d, _d, _d, _d, pu]
         L.ref_grid_from_function.restype = _vp
         L.ref_grid_from_function.argtypes = [self.VOXEL_FN, _vp, _vp, _f, _i, pu]
+        L.ref_grid_from_dense.restype = _vp
+        L.ref_grid_from_dense.argtypes = [_vp, _vp, _f, _i, pu]
         L.ref_quantize.restype = _vp
         L.ref_quantize.argtypes = [_vp, _i, _f, pu]
         L.ref_get_value.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _i]
@@ -208,6 +210,14 @@ class Reference:
         n = _u64()
         p = self.L.ref_grid_from_function(cb, None, _p(np.asarray(bbox, np.int32)), background, grid_class,
                                           ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def grid_from_dense(self, dense, bbox, background=0.0, grid_class=2):
+        """build::Grid::operator()(func, bbox) with func reading dense[x][y][z] over bbox."""
+        dense = np.ascontiguousarray(dense, np.float32)
+        n = _u64()
+        p = self.L.ref_grid_from_dense(_p(dense), _p(np.asarray(bbox, np.int32)), background, grid_class,
+                                       ctypes.byref(n))
         return self._take(p, n.value)
 
     def quantize(self, buf, kind, tol=1e-3):

@@ -178,7 +178,7 @@ void* ref_level_set_sphere(int kind, double radius, double cx, double cy, double
     case kFp8: return copyOut(tools::createLevelSetSphere<Fp8>(radius, c, voxel, hw), bytes);
     case kFp16: return copyOut(tools::createLevelSetSphere<Fp16>(radius, c, voxel, hw), bytes);
     case kFpN:
-        return copyOut(tools::createLevelSetSphere<FpN>(radius, c, voxel, hw, Vec3d(0), "sphere_ls",
+        return copyOut(tools::createLevelSetSphere<FpN>(radius, c, voxel, hw, Vec3d(0), "sphere_ls_FpN",
                                                         tools::StatsMode::Default, CheckMode::Default, tol), bytes);
     }
     return nullptr;
@@ -213,6 +213,18 @@ void* ref_grid_from_function(ref_voxel_fn fn, void* ctx, const int32_t* bbox, fl
     tools::build::Grid<float> grid(background, "func", GridClass(gridClass));
     const CoordBBox b(Coord(bbox[0], bbox[1], bbox[2]), Coord(bbox[3], bbox[4], bbox[5]));
     grid([&](const Coord& ijk) { return fn(ijk[0], ijk[1], ijk[2], ctx); }, b);
+    return copyOut(tools::createNanoGrid(grid), bytes);
+}
+
+// Same, with the functor reading a dense array over bbox (x-major, z fastest).
+void* ref_grid_from_dense(const float* dense, const int32_t* bbox, float background, int gridClass, uint64_t* bytes)
+{
+    tools::build::Grid<float> grid(background, "func", GridClass(gridClass));
+    const CoordBBox b(Coord(bbox[0], bbox[1], bbox[2]), Coord(bbox[3], bbox[4], bbox[5]));
+    const int64_t ny = bbox[4] - bbox[1] + 1, nz = bbox[5] - bbox[2] + 1;
+    grid([&](const Coord& ijk) {
+        return dense[((int64_t(ijk[0]) - bbox[0]) * ny + (ijk[1] - bbox[1])) * nz + (ijk[2] - bbox[2])];
+    }, b);
     return copyOut(tools::createNanoGrid(grid), bytes);
 }
 
