@@ -6,7 +6,10 @@
 // XCDs (hardware dispatch is round-robin over XCDs) walks a contiguous band of
 // tiles and keeps its own L2 warm with the nodes that band touches.
 #include "nvq_device.h"
+#include "nvq_internal.h"
 #include "../../include/nvq.h"
+
+#include <cstdlib>
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -634,6 +637,15 @@ int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, fl
 {
     if (!grid || (n && !xyz) || (grad && (grad_mode < 1 || grad_mode > 3))) return NVQ_ERR_INVALID_ARG;
     if (n == 0) return NVQ_OK;
+    // Large batches go through spatial binning + LDS bricks (bit-identical results);
+    // NVQ_SAMPLE_DIRECT=1 forces the lane-per-query kernel.
+    static const bool forceDirect = std::getenv("NVQ_SAMPLE_DIRECT") && std::getenv("NVQ_SAMPLE_DIRECT")[0] == '1';
+    if (!forceDirect && sampleBinnedSupported(grid->view, n)) {
+        const int st = sampleBinned(grid->view, xyz, n, world, nn, val, grad, grad_mode, S(stream));
+        if (st == 0) return NVQ_OK;
+        if (st == -2) return NVQ_ERR_HIP; // scratch allocation failed
+        return st == -3 ? NVQ_ERR_UNSUPPORTED_TYPE : NVQ_ERR_HIP;
+    }
     return launchTyped<RunSample>(grid->view.type, grid->view, xyz, n, world, nn, val, grad, grad_mode, S(stream));
 }
 

@@ -207,3 +207,30 @@ def test_fog_march_sphere(nvq, fog_sphere, oracle):
     assert steps.max() > 100
     np.testing.assert_allclose(L, oL, rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(T, oT, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("grad_mode", [1, 2, 3])
+def test_sample_binned_large_batch(nvq, d_torus, torus, oracle, grad_mode):
+    """Batches of >= 2^20 points take the binned LDS-brick path; results stay bit-exact,
+    including points outside the bbox (border bins) and far away (accessor fallback)."""
+    r = rng(100 + grad_mode)
+    xyz = points_near(oracle, torus, (1 << 20) + 12345, seed=100 + grad_mode)
+    far = r.normal(0, 3e3, size=(5000, 3)).astype(np.float32)
+    xyz = np.concatenate([xyz, far])
+    nn, val, grad = nvq.ops.sample(d_torus, _t(xyz), grad_mode=grad_mode)
+    onn, oval, ograd = oracle.sample(torus, xyz, grad_mode=grad_mode)
+    _check_bits("nn", nn, onn)
+    _check_bits("trilinear", val, oval)
+    _check_bits(f"grad{grad_mode}", grad, ograd)
+
+
+@pytest.mark.parametrize("kind", [3, 4])
+def test_sample_binned_quantised_world(nvq, ref, oracle, kind):
+    grid = ref.level_set_sphere(60.0, (5.0, -7.0, 2.0), 0.5, 3.0, kind=kind, tol=1e-3)
+    d = nvq.DeviceGrid(grid)
+    xyz = points_near(oracle, grid, 1 << 20, seed=7 + kind) * np.float32(0.5)  # world space (voxel 0.5)
+    nn, val, grad = nvq.ops.sample(d, _t(xyz), world=True, grad_mode=1)
+    onn, oval, ograd = oracle.sample(grid, xyz, world=True, grad_mode=1)
+    _check_bits("nn", nn, onn)
+    _check_bits("trilinear", val, oval)
+    _check_bits("fd6", grad, ograd)
