@@ -67,7 +67,11 @@ const char* nvq_status_string(int status);
  * deviceGrid<T>() returns nullptr on a type mismatch, this returns
  * NVQ_ERR_UNSUPPORTED_TYPE / NVQ_ERR_BAD_MAGIC / ... and leaves *out NULL. */
 int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid** out);
-/* Free the device copy (waits for the device). */
+/* Same as nvq_grid_upload, but copies into caller-owned device memory `device_dst` (at least
+ * `bytes`, 32-byte aligned) instead of allocating; nvq_grid_free then leaves that memory alone.
+ * Lets a framework own the buffer, e.g. to expose the leaf values as a trainable tensor. */
+int nvq_grid_upload_into(const void* host_buf, uint64_t bytes, void* device_dst, void* stream, nvq_grid** out);
+/* Free the grid handle and, for nvq_grid_upload, the device copy (waits for the device). */
 int nvq_grid_free(nvq_grid* grid);
 int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info);
 
@@ -110,6 +114,21 @@ int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, 
  * may be NULL.  image_width as for nvq_zero_crossing. */
 int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                   float dt, float sigma_t, float* radiance, float* transmittance, void* stream);
+
+/* Adjoint of nvq_fog_march (no counterpart in the reference): dLoss/dvalue for every leaf value
+ * of a Float grid, given the forward outputs radiance[n] (and transmittance[n] when
+ * dL_dtransmittance is given) and the loss gradients per ray.  Contributions are ADDED to
+ * grad_accum[leafCount*512] (leaf order of the buffer, voxel offset n = x<<6|y<<3|z within the
+ * leaf) as int64 fixed point with 2^-fixed_shift resolution (range +-2^(63-fixed_shift)); the
+ * sum is deterministic for any schedule.  The caller zeroes grad_accum before the first call
+ * and may accumulate several calls (e.g. cameras).  Transmittance is recomputed by re-marching
+ * each ray; no per-step state is stored. */
+int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                      float dt, float sigma_t, const float* radiance, const float* transmittance,
+                      const float* dL_dradiance, const float* dL_dtransmittance, int64_t* grad_accum, int fixed_shift,
+                      void* stream);
+/* out[i] = float(double(accum[i]) * 2^-fixed_shift) */
+int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,7 @@ class GridInfo(ctypes.Structure):
 SIGNATURES = {
     "nvq_status_string": ([_i], ctypes.c_char_p),
     "nvq_grid_upload": ([_vp, _u64, _vp, ctypes.POINTER(_vp)], _i),
+    "nvq_grid_upload_into": ([_vp, _u64, _vp, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_grid_free": ([_vp], _i),
     "nvq_grid_get_info": ([_vp, ctypes.POINTER(GridInfo)], _i),
     "nvq_get_value": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp], _i),
@@ -44,6 +45,8 @@ SIGNATURES = {
     "nvq_grad_stencil": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_zero_crossing": ([_vp, _vp, _vp, _u64, _u32, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp], _i),
+    "nvq_fog_march_bwd": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
+    "nvq_grad_fixed_to_float": ([_vp, _u64, _i, _vp, _vp], _i),
     # host builder (include/nvq_build.h)
     "nvqb_level_set_sphere": ([_d, _vp, _d, _d, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
     "nvqb_level_set_torus": ([_d, _d, _vp, _d, _d, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnvq.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["nvq_kernels.hip", "nvq_binned.hip"]
+HIP_SOURCES = ["nvq_kernels.hip", "nvq_binned.hip", "nvq_adjoint.hip"]
 HOST_SOURCES = ["nvq_build.cpp"]  # host-only C++ (grid builder), compiled by g++
 OBJ_DIR = os.path.join(HERE, "_build")
 HOST_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-pthread", "-Wall", "-Wextra",
@@ -29,17 +29,30 @@ def _run(cmd, verbose):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=False):
+def _stale(obj, src):
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")] + \
+        [os.path.join(ROOT, "include", h) for h in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > os.path.getmtime(obj) for d in deps)
+
+
+def build(verbose=False, incremental=False):
+    """Compile every source (incremental=True skips objects newer than their sources and headers)."""
     os.makedirs(OBJ_DIR, exist_ok=True)
     objs = []
     for src in HOST_SOURCES:
         obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            continue
+        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            continue
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs + ["-pthread"], verbose)
     os.replace(tmp, LIB)

@@ -1,0 +1,140 @@
+// Adjoint of the emission-absorption fog march with respect to the leaf values of a
+// Float grid (no counterpart in the reference, which has no differentiable path).
+//
+// Forward (per ray, steps k = 0..N-1 from the shared marchSteps structure):
+//   rho_k = trilinear(x_k), a_k = exp(-s*rho_k) with s = sigma*dt,
+//   L = sum_k T_k (1-a_k) rho_k,  T_{k+1} = T_k a_k,  T_0 = 1.
+// Adjoint:
+//   dLoss/drho_k = gL * (T_k [(1-a_k) + rho_k s a_k] - s R_k) - gT * s * T_N,
+//   R_k = sum_{j>k} T_j (1-a_j) rho_j = L - L_{<=k}.
+// The backward kernel re-marches each ray with the identical FP32 step structure and
+// arithmetic, so T_k and the running radiance L_{<=k} are recomputed bit for bit instead of
+// stored per step; L and T_N come from the forward outputs.  dLoss/drho_k is spread over the
+// eight trilinear corners of step k with weights (1-u|u)(1-v|v)(1-w|w).
+//
+// Deterministic reduction: contributions are first summed per ray over runs of consecutive
+// steps that share a trilinear cell (a segmented reduction keyed by the corner voxel), then
+// added into an int64 fixed-point accumulator per leaf value (key = leaf index * 512 + voxel
+// offset) with integer atomics.  Integer addition is associative, so the result does not
+// depend on thread scheduling; the accumulator is converted through FP64.
+#include "nvq_device.h"
+#include "nvq_internal.h"
+#include "../../include/nvq.h"
+
+#include <hip/hip_runtime.h>
+
+namespace nvq {
+namespace {
+
+struct Slots {
+    int32_t key[8];
+    float acc[8];
+};
+
+__device__ __forceinline__ void flush(Slots& s, unsigned long long* __restrict__ accum, float scale)
+{
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (s.key[c] >= 0 && s.acc[c] != 0.0f) {
+            const long long q = __float2ll_rn(s.acc[c] * scale);
+            if (q != 0) atomicAdd(accum + s.key[c], static_cast<unsigned long long>(q));
+        }
+        s.acc[c] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const float* __restrict__ eye,
+                                                          const float* __restrict__ dir, Sched s, float dt, float sigma,
+                                                          const float* __restrict__ Lfwd, const float* __restrict__ Tfwd,
+                                                          const float* __restrict__ gLin, const float* __restrict__ gTin,
+                                                          unsigned long long* __restrict__ accum, float scale)
+{
+    uint64_t i;
+    if (!schedule(s, i)) return;
+    const float gL = gLin ? gLin[i] : 0.0f, gT = gTin ? gTin[i] : 0.0f;
+    if (gL == 0.0f && gT == 0.0f) return;
+    Accessor<GT_FLOAT> acc(g);
+    const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
+    const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    const float Ltot = Lfwd[i];
+    const float sdt = sigma * dt;
+    const float tTerm = gT * sdt * Tfwd[i];
+    float L = 0.0f, T = 1.0f;
+    Slots sl;
+    int32_t cell[3] = {0x7fffffff, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        sl.key[c] = -1;
+        sl.acc[c] = 0.0f;
+    }
+    marchSteps(acc, g, e, d, dt, [&](const float x[3]) {
+        const float fx = floorf(x[0]), fy = floorf(x[1]), fz = floorf(x[2]);
+        const int32_t ci = int32_t(fx), cj = int32_t(fy), ck = int32_t(fz);
+        Cell C;
+        if (ci != cell[0] || cj != cell[1] || ck != cell[2]) {
+            flush(sl, accum, scale);
+            cell[0] = ci;
+            cell[1] = cj;
+            cell[2] = ck;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const Probe p = acc.probe(ci + a, cj + b, ck + c);
+                        C.v[a][b][c] = p.value;
+                        sl.key[a * 4 + b * 2 + c] =
+                            p.leaf ? int32_t(uint32_t((p.leaf - g.leaf0) / 2144u) * 512u + p.n) : -1;
+                    }
+        } else {
+            stencil8(acc, ci, cj, ck, C);
+        }
+        const float u = x[0] - fx, v = x[1] - fy, w = x[2] - fz;
+        const float rho = trilerp(C, u, v, w);
+        const float a = expf(-sigma * rho * dt);
+        L += T * (1.0f - a) * rho;        // same operations as the forward march
+        const float R = Ltot - L;         // radiance of the later steps
+        const float gk = gL * (T * ((1.0f - a) + rho * sdt * a) - sdt * R) - tTerm;
+        T *= a;
+        const float wx[2] = {1.0f - u, u}, wy[2] = {1.0f - v, v}, wz[2] = {1.0f - w, w};
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) sl.acc[aa * 4 + b * 2 + c] += gk * (wx[aa] * wy[b] * wz[c]);
+    });
+    flush(sl, accum, scale);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fixed_to_float(const long long* __restrict__ in, uint64_t n, double inv,
+                                                           float* __restrict__ out)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) out[i] = float(double(in[i]) * inv);
+}
+
+} // namespace
+
+int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint64_t n, uint32_t width, float dt,
+                     float sigma, const float* L, const float* T, const float* gL, const float* gT, int64_t* accum,
+                     int shift, hipStream_t s)
+{
+    const Sched sc = makeSched(n, width);
+    const float scale = ldexpf(1.0f, shift);
+    hipLaunchKernelGGL(k_fog_march_bwd, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
+                       gT, reinterpret_cast<unsigned long long*>(accum), scale);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hipStream_t s)
+{
+    const uint64_t b = (count + kBlock - 1) / kBlock;
+    const uint32_t blocks = uint32_t(b < 1 ? 1 : (b > (1u << 20) ? (1u << 20) : b));
+    hipLaunchKernelGGL(k_fixed_to_float, dim3(blocks), dim3(kBlock), 0, s, reinterpret_cast<const long long*>(accum),
+                       count, ldexp(1.0, -shift), out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+} // namespace nvq

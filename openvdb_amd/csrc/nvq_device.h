@@ -428,4 +428,48 @@ struct HDDA {
     }
 };
 
+// ------------------------------------------------------------------ fog-march step structure
+// HDDA over cells of size >= 8 (the leaf level, as TreeMarcher, math/HDDA.h:450-536, with
+// Eps = 1e-6 and Delta = 0.01); a cell is marched when it lies in a leaf (getDim == 1) or in an
+// active tile; steps are globally aligned at t = k*dt with ta <= t < tb inside each marched
+// cell.  onStep(t) is called for every step in order; the forward march and its adjoint share
+// this routine, so both see the same steps bit for bit.
+template<class AccT, class F>
+__device__ __forceinline__ void marchSteps(AccT& acc, const GridView& g, const float e[3], const float d[3], float dt,
+                                           F&& onStep)
+{
+    Ray R = worldToIndexRay(g, e, d);
+    if (!rayClip(R, g.bbox) || double(R.t1) > 1e20) return;
+    const float Eps = 0.000001f, Delta = 0.01f;
+    const float t0 = R.t0 + Eps, t1 = R.t1 - Eps;
+    if (t0 > t1) return;
+    float p[3];
+    rayAt(R, t0, p);
+    Probe l = acc.probe(ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+    HDDA h;
+    h.init(R, t0, t1, int32_t(l.dim > 8u ? l.dim : 8u));
+    bool on = (l.dim == 1u) || l.active;
+    for (;;) {
+        const float ta = h.t0;
+        const bool valid = h.stepOnce();
+        const float tb = fminf(h.t0, t1);
+        if (on && ta < tb) {
+            float kf = ceilf(ta / dt);
+            for (float t = kf * dt; t < tb; t = kf * dt) {
+                float x[3];
+                rayAt(R, t, x);
+                onStep(x);
+                const float kn = kf + 1.0f;
+                if (kn == kf) break; // step index beyond 2^24: float cannot advance it
+                kf = kn;
+            }
+        }
+        if (!valid) break;
+        rayAt(R, h.t0 + Delta, p);
+        l = acc.probe(ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+        h.update(R, int32_t(l.dim > 8u ? l.dim : 8u));
+        on = (l.dim == 1u) || l.active;
+    }
+}
+
 } // namespace nvq

@@ -7,9 +7,68 @@
 
 namespace nvq {
 
+// ------------------------------------------------------------------ ray-tile scheduling
+// Rays of a row-major image are processed in 8x8 pixel tiles (one tile per 64-wide wavefront);
+// workgroups are remapped so each XCD (hardware dispatch is round-robin over the 8 XCDs) walks a
+// contiguous band of tiles and keeps its own L2 warm with the nodes that band touches.
+constexpr int kBlock = 256;
+
+struct Sched {
+    uint64_t n;
+    uint32_t width, height, tilesX;
+    uint64_t tiles;
+    uint32_t blocks;  // logical blocks
+    uint32_t perXcd;  // logical blocks per XCD
+};
+
+inline Sched makeSched(uint64_t n, uint32_t width)
+{
+    Sched s{};
+    s.n = n;
+    s.width = width;
+    if (width) {
+        s.height = uint32_t(n / width);
+        s.tilesX = (width + 7) / 8;
+        s.tiles = uint64_t(s.tilesX) * ((s.height + 7) / 8);
+        s.blocks = uint32_t((s.tiles + (kBlock / 64) - 1) / (kBlock / 64));
+    } else {
+        s.blocks = uint32_t((n + kBlock - 1) / kBlock);
+    }
+    s.perXcd = (s.blocks + 7) / 8;
+    return s;
+}
+
+__device__ __forceinline__ bool schedule(const Sched& s, uint64_t& idx)
+{
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = (b & 7u) * s.perXcd + (b >> 3);
+    if (lb >= s.blocks) return false;
+    if (s.width == 0) {
+        idx = uint64_t(lb) * kBlock + threadIdx.x;
+        return idx < s.n;
+    }
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t tile = uint64_t(lb) * (kBlock / 64) + wave;
+    if (tile >= s.tiles) return false;
+    const uint32_t tx = uint32_t(tile % s.tilesX), ty = uint32_t(tile / s.tilesX);
+    const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
+    if (px >= s.width || py >= s.height) return false;
+    idx = uint64_t(py) * s.width + px;
+    return true;
+}
+
+inline uint32_t launchBlocks(const Sched& s) { return s.perXcd * 8u; }
+
+
 // Large-batch point sampling through spatial bins and LDS bricks (nvq_binned.hip).
 bool sampleBinnedSupported(const GridView& g, uint64_t n);
 int sampleBinned(const GridView& g, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                  int mode, hipStream_t s);
+
+// Adjoint of the fog march into an int64 fixed-point accumulator (nvq_adjoint.hip).
+int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint64_t n, uint32_t width, float dt,
+                     float sigma, const float* L, const float* T, const float* gL, const float* gT, int64_t* accum,
+                     int shift, hipStream_t s);
+int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hipStream_t s);
 
 } // namespace nvq

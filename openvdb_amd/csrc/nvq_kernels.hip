@@ -23,62 +23,15 @@ struct nvq_grid {
     void* dbuf;
     uint64_t bytes;
     int device;
+    bool owns; // false when the device memory belongs to the caller (nvq_grid_upload_into)
     GridView view;
     nvq_grid_info info;
 };
 
 namespace {
 
-constexpr int kBlock = 256;
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
-
-// ------------------------------------------------------------------ scheduling
-struct Sched {
-    uint64_t n;
-    uint32_t width, height, tilesX;
-    uint64_t tiles;
-    uint32_t blocks;  // logical blocks
-    uint32_t perXcd;  // logical blocks per XCD
-};
-
-Sched makeSched(uint64_t n, uint32_t width)
-{
-    Sched s{};
-    s.n = n;
-    s.width = width;
-    if (width) {
-        s.height = uint32_t(n / width);
-        s.tilesX = (width + 7) / 8;
-        s.tiles = uint64_t(s.tilesX) * ((s.height + 7) / 8);
-        s.blocks = uint32_t((s.tiles + (kBlock / 64) - 1) / (kBlock / 64));
-    } else {
-        s.blocks = uint32_t((n + kBlock - 1) / kBlock);
-    }
-    s.perXcd = (s.blocks + 7) / 8;
-    return s;
-}
-
-__device__ __forceinline__ bool schedule(const Sched& s, uint64_t& idx)
-{
-    const uint32_t b = blockIdx.x;
-    const uint32_t lb = (b & 7u) * s.perXcd + (b >> 3);
-    if (lb >= s.blocks) return false;
-    if (s.width == 0) {
-        idx = uint64_t(lb) * kBlock + threadIdx.x;
-        return idx < s.n;
-    }
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint64_t tile = uint64_t(lb) * (kBlock / 64) + wave;
-    if (tile >= s.tiles) return false;
-    const uint32_t tx = uint32_t(tile % s.tilesX), ty = uint32_t(tile / s.tilesX);
-    const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
-    if (px >= s.width || py >= s.height) return false;
-    idx = uint64_t(py) * s.width + px;
-    return true;
-}
-
-inline uint32_t launchBlocks(const Sched& s) { return s.perXcd * 8u; }
 
 // ------------------------------------------------------------------ getValue
 template<uint32_t GT>
@@ -250,9 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_zero_crossing(GridView g, const floa
 }
 
 // ------------------------------------------------------------------ fog march
-// HDDA over cells of size >= 8 (leaf level, as TreeMarcher, math/HDDA.h:450-536, Eps 1e-6,
-// Delta 0.01); leaves and active tiles are marched at globally aligned steps t = k*dt with
-// ta <= t < tb; a = exp(-sigma*rho*dt), L += T*(1-a)*rho, T *= a.
+// Emission-absorption over the shared step structure (marchSteps): per step
+// a = exp(-sigma*rho*dt), L += T*(1-a)*rho, T *= a.
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* __restrict__ eye,
                                                       const float* __restrict__ dir, Sched s, float dt, float sigma,
@@ -263,44 +215,13 @@ __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* _
     Accessor<GT> acc(g);
     const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
     const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
-    Ray R = worldToIndexRay(g, e, d);
     float L = 0.0f, T = 1.0f;
-    if (rayClip(R, g.bbox) && !(double(R.t1) > 1e20)) {
-        const float Eps = 0.000001f, Delta = 0.01f;
-        const float t0 = R.t0 + Eps, t1 = R.t1 - Eps;
-        if (t0 <= t1) {
-            float p[3];
-            rayAt(R, t0, p);
-            Probe l = acc.probe(ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
-            HDDA h;
-            h.init(R, t0, t1, int32_t(l.dim > 8u ? l.dim : 8u));
-            bool on = (l.dim == 1u) || l.active;
-            for (;;) {
-                const float ta = h.t0;
-                const bool valid = h.stepOnce();
-                const float tb = fminf(h.t0, t1);
-                if (on && ta < tb) {
-                    float kf = ceilf(ta / dt);
-                    for (float t = kf * dt; t < tb; t = kf * dt) {
-                        float x[3];
-                        rayAt(R, t, x);
-                        const float rho = trilinear(acc, x[0], x[1], x[2]);
-                        const float a = expf(-sigma * rho * dt);
-                        L += T * (1.0f - a) * rho;
-                        T *= a;
-                        const float kn = kf + 1.0f;
-                        if (kn == kf) break; // step index beyond 2^24: float cannot advance it
-                        kf = kn;
-                    }
-                }
-                if (!valid) break;
-                rayAt(R, h.t0 + Delta, p);
-                l = acc.probe(ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
-                h.update(R, int32_t(l.dim > 8u ? l.dim : 8u));
-                on = (l.dim == 1u) || l.active;
-            }
-        }
-    }
+    marchSteps(acc, g, e, d, dt, [&](const float x[3]) {
+        const float rho = trilinear(acc, x[0], x[1], x[2]);
+        const float a = expf(-sigma * rho * dt);
+        L += T * (1.0f - a) * rho;
+        T *= a;
+    });
     if (Lout) Lout[i] = L;
     if (Tout) Tout[i] = T;
 }
@@ -439,7 +360,7 @@ const char* nvq_status_string(int status)
     }
 }
 
-int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid** out)
+static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, void* stream, nvq_grid** out)
 {
     if (!out) return NVQ_ERR_INVALID_ARG;
     *out = nullptr;
@@ -448,17 +369,18 @@ int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid
     if (st != NVQ_OK) return st;
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
-    void* d = nullptr;
-    if (hipMalloc(&d, bytes) != hipSuccess) return NVQ_ERR_HIP;
+    void* d = device_dst;
+    if (!d && hipMalloc(&d, bytes) != hipSuccess) return NVQ_ERR_HIP;
     if (hipMemcpyAsync(d, host_buf, bytes, hipMemcpyHostToDevice, S(stream)) != hipSuccess) {
-        (void)hipFree(d);
+        if (!device_dst) (void)hipFree(d);
         return NVQ_ERR_HIP;
     }
     nvq_grid* g = new (std::nothrow) nvq_grid;
     if (!g) {
-        (void)hipFree(d);
+        if (!device_dst) (void)hipFree(d);
         return NVQ_ERR_HIP;
     }
+    g->owns = device_dst == nullptr;
     g->dbuf = d;
     g->bytes = bytes;
     g->device = device;
@@ -500,11 +422,22 @@ int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid
     return NVQ_OK;
 }
 
+int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid** out)
+{
+    return uploadImpl(host_buf, bytes, nullptr, stream, out);
+}
+
+int nvq_grid_upload_into(const void* host_buf, uint64_t bytes, void* device_dst, void* stream, nvq_grid** out)
+{
+    if (!device_dst || reinterpret_cast<uintptr_t>(device_dst) % 32u) return NVQ_ERR_INVALID_ARG;
+    return uploadImpl(host_buf, bytes, device_dst, stream, out);
+}
+
 int nvq_grid_free(nvq_grid* grid)
 {
     if (!grid) return NVQ_ERR_INVALID_ARG;
     const hipError_t e1 = hipDeviceSynchronize();
-    const hipError_t e2 = hipFree(grid->dbuf);
+    const hipError_t e2 = grid->owns ? hipFree(grid->dbuf) : hipSuccess;
     delete grid;
     return (e1 == hipSuccess && e2 == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
 }
@@ -571,6 +504,31 @@ int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint
     const Sched sc = makeSched(n, image_width);
     return launchTyped<RunFogMarch>(grid->view.type, grid->view, eye, dir, sc, dt, sigma_t, radiance, transmittance,
                                     S(stream));
+}
+
+int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                      float dt, float sigma_t, const float* radiance, const float* transmittance,
+                      const float* dL_dradiance, const float* dL_dtransmittance, int64_t* grad_accum, int fixed_shift,
+                      void* stream)
+{
+    if (!grid || !grad_accum || (n && (!eye || !dir || !radiance)) || !(dt > 0.0f) || fixed_shift < 0 ||
+        fixed_shift > 62 || (dL_dtransmittance && !transmittance))
+        return NVQ_ERR_INVALID_ARG;
+    if (grid->view.type != GT_FLOAT) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
+    if (n == 0 || (!dL_dradiance && !dL_dtransmittance)) return NVQ_OK;
+    if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
+    return fogMarchBackward(grid->view, eye, dir, n, image_width, dt, sigma_t, radiance, transmittance, dL_dradiance,
+                            dL_dtransmittance, grad_accum, fixed_shift, S(stream)) == 0
+               ? NVQ_OK
+               : NVQ_ERR_HIP;
+}
+
+int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream)
+{
+    if ((count && (!accum || !out)) || fixed_shift < 0 || fixed_shift > 62) return NVQ_ERR_INVALID_ARG;
+    if (count == 0) return NVQ_OK;
+    return fixedToFloat(accum, count, fixed_shift, out, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
 }
 
 } // extern "C"

@@ -38,21 +38,33 @@ class DeviceGrid:
     """An uploaded grid.  ``host_buffer`` is a uint8 numpy array holding the NanoVDB buffer."""
 
     def __init__(self, host_buffer, stream=None):
+        import torch
         buf = np.ascontiguousarray(host_buffer).reshape(-1).view(np.uint8)
         if buf.ctypes.data % 32:
             buf = aligned_buffer(buf)
         self._host = buf
+        # device memory owned by torch (caching allocator, 256-byte aligned); the grid wraps it
+        self.device_bytes = torch.empty(buf.nbytes, dtype=torch.uint8, device="cuda")
         handle = ctypes.c_void_p()
         s = _stream_ptr(stream)
-        check(lib().nvq_grid_upload(buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes, s, ctypes.byref(handle)),
-              "nvq_grid_upload")
+        check(lib().nvq_grid_upload_into(buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes,
+                                         ctypes.c_void_p(self.device_bytes.data_ptr()), s, ctypes.byref(handle)),
+              "nvq_grid_upload_into")
         self.handle = handle
-        # the copy is asynchronous; keep the host bytes alive until it is done
-        import torch
-        torch.cuda.synchronize()
+        torch.cuda.synchronize()  # the copy is asynchronous: keep the host bytes alive until it is done
         info = GridInfo()
         check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
         self.info = info
+
+    def leaf_values(self):
+        """[leafCount, 512] float32 view of the leaf values inside the device buffer (Float grids).
+        Writing to it changes the grid; it is the parameter tensor of the differentiable march."""
+        if self.info.grid_type != 1:
+            raise ValueError("leaf_values() needs a Float grid")
+        import torch
+        leaf_off = 672 + int(np.frombuffer(self._host[672:680].tobytes(), np.int64)[0])
+        f32 = self.device_bytes.view(torch.float32)
+        return torch.as_strided(f32, (self.leaf_count, 512), (2144 // 4, 1), (leaf_off + 96) // 4)
 
     @property
     def grid_type(self):

@@ -112,3 +112,56 @@ def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):
     check(lib().nvq_fog_march(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t, _ptr(L), _ptr(T),
                               _stream()), "nvq_fog_march")
     return L, T
+
+
+def fog_march_backward(grid, eye, dir, radiance, grad_radiance, transmittance=None, grad_transmittance=None, dt=0.5,
+                       sigma_t=0.05, image_width=0, accum=None, fixed_shift=40):
+    """Adjoint of ``fog_march``: adds dLoss/dvalue (int64 fixed point, 2^-fixed_shift) into
+    ``accum`` [leafCount*512] and returns it (allocated and zeroed when None)."""
+    eye = _points(eye, torch.float32, "eye")
+    dir = _points(dir, torch.float32, "dir")
+    n = eye.shape[0]
+    if accum is None:
+        accum = torch.zeros(grid.leaf_count * 512, dtype=torch.int64, device=eye.device)
+    gr = None if grad_radiance is None else grad_radiance.contiguous().float()
+    gt = None if grad_transmittance is None else grad_transmittance.contiguous().float()
+    check(lib().nvq_fog_march_bwd(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t,
+                                  _ptr(radiance.contiguous()), _ptr(None if transmittance is None else transmittance.contiguous()),
+                                  _ptr(gr), _ptr(gt), _ptr(accum), fixed_shift, _stream()), "nvq_fog_march_bwd")
+    return accum
+
+
+def fixed_to_float(accum, fixed_shift=40):
+    out = torch.empty(accum.shape, dtype=torch.float32, device=accum.device)
+    check(lib().nvq_grad_fixed_to_float(_ptr(accum), accum.numel(), fixed_shift, _ptr(out), _stream()),
+          "nvq_grad_fixed_to_float")
+    return out
+
+
+class FogMarch(torch.autograd.Function):
+    """Differentiable emission-absorption render: (radiance, transmittance) = march(values).
+
+    ``values`` must be ``grid.leaf_values()`` (the grid reads it in place); the backward pass
+    returns dLoss/dvalues of the same shape, reduced deterministically.
+    """
+
+    @staticmethod
+    def forward(ctx, values, grid, eye, dir, dt, sigma_t, image_width, fixed_shift):
+        L, T = fog_march(grid, eye, dir, dt, sigma_t, image_width)
+        ctx.save_for_backward(eye, dir, L, T)
+        ctx.grid, ctx.dt, ctx.sigma, ctx.width, ctx.shift = grid, dt, sigma_t, image_width, fixed_shift
+        ctx.shape = values.shape
+        return L, T
+
+    @staticmethod
+    def backward(ctx, gL, gT):
+        eye, dir, L, T = ctx.saved_tensors
+        accum = fog_march_backward(ctx.grid, eye, dir, L, gL, T, gT, ctx.dt, ctx.sigma, ctx.width,
+                                   fixed_shift=ctx.shift)
+        g = fixed_to_float(accum, ctx.shift).view(ctx.shape)
+        return g, None, None, None, None, None, None, None
+
+
+def fog_render(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0, fixed_shift=40):
+    """Differentiable fog march over ``grid.leaf_values()``; returns (radiance, transmittance)."""
+    return FogMarch.apply(grid.leaf_values(), grid, eye, dir, dt, sigma_t, image_width, fixed_shift)

@@ -186,3 +186,45 @@ def test_fog_march_constant_density_closed_form(oracle, ref):
     assert steps[0] == len(ts) == 128
     np.testing.assert_allclose(T[0], np.prod(a), rtol=1e-14)
     np.testing.assert_allclose(L[0], np.sum(Tk * (1 - a) * dens), rtol=1e-14)
+
+
+def _leaf_value_offsets(buf, oracle):
+    """Byte offset of every leaf value (leaf order, n = x<<6|y<<3|z) of a Float grid."""
+    leaf0 = 672 + int(np.frombuffer(buf[672:680].tobytes(), np.int64)[0])
+    cnt = oracle.leaf_count(buf)
+    return leaf0 + 96 + (np.arange(cnt)[:, None] * 2144 + np.arange(512)[None, :] * 4)
+
+
+def test_fog_adjoint_matches_central_differences(oracle, ref):
+    """The FP64 oracle adjoint (dLoss/dvalue) against central differences of the FP64 oracle
+    march, h = 1e-3, on voxels that the rays actually sample.  Pins the adjoint by calculus:
+    no reference implementation exists."""
+    grid = ref.fog_volume_sphere(9.0, (0.5, -0.25, 0.75), 1.0, 3.0)
+    r = rng(5)
+    n = 64
+    eye = np.tile(np.array([[0.0, 0.0, 40.0]], np.float32), (n, 1)) + r.normal(0, 1.5, (n, 3)).astype(np.float32)
+    tgt = r.uniform(-6, 6, (n, 3))
+    d = tgt - eye
+    dir = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    wL = r.normal(0, 1, n)
+    wT = r.normal(0, 1, n)
+    g = oracle.fog_march_adjoint(grid, eye, dir, 0.5, 0.05, wL, wT)
+    offs = _leaf_value_offsets(grid, oracle).reshape(-1)
+    cand = np.nonzero(np.abs(g) > 1e-3 * np.abs(g).max())[0]
+    pick = r.choice(cand, size=min(40, len(cand)), replace=False)
+    h = 1e-3
+
+    def loss(buf):
+        L, T, _ = oracle.fog_march(buf, eye, dir, 0.5, 0.05)
+        return float(np.dot(wL, L) + np.dot(wT, T))
+
+    for k in pick:
+        b = grid.copy()
+        v = np.frombuffer(grid[offs[k]:offs[k] + 4].tobytes(), np.float32)[0]
+        b[offs[k]:offs[k] + 4] = np.frombuffer(np.float32(v + h).tobytes(), np.uint8)
+        lp = loss(b)
+        b[offs[k]:offs[k] + 4] = np.frombuffer(np.float32(v - h).tobytes(), np.uint8)
+        lm = loss(b)
+        dh = float(np.float32(v + h)) - float(np.float32(v - h))
+        fd = (lp - lm) / dh
+        assert abs(fd - g[k]) <= 2e-5 * max(1.0, abs(g[k])) + 1e-6 * np.abs(g).max(), (k, fd, g[k])
