@@ -115,6 +115,18 @@ int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, 
 int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                   float dt, float sigma_t, float* radiance, float* transmittance, void* stream);
 
+/* One persistent launch over k <= 32 grids of any leaf codecs and classes.  host_grids[k] and
+ * host_ray_offsets[k+1] / host_image_width[k] are HOST arrays; the rays of grid j are
+ * eye/dir[host_ray_offsets[j] .. host_ray_offsets[j+1]), a row-major image of width
+ * host_image_width[j] (its ray count must be a multiple of the width; width 0 = rows of 64).
+ * Level-set grids run zeroCrossing (hit, ijk, t -> out_a, v -> out_b, normal); fog-volume grids
+ * run the emission-absorption march (radiance -> out_a, transmittance -> out_b).  Wavefronts
+ * pull 8x8 ray tiles from one device work counter (the persistent-warp launch of
+ * examples/ex_raytrace_iso_surface/common.h:166-189).  hit / ijk / normal may be NULL. */
+int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                    const uint32_t* host_image_width, const float* eye, const float* dir, float dt, float sigma_t,
+                    uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal, void* stream);
+
 /* Adjoint of nvq_fog_march (no counterpart in the reference): dLoss/dvalue for every leaf value
  * of a Float grid, given the forward outputs radiance[n] (and transmittance[n] when
  * dL_dtransmittance is given) and the loss gradients per ray.  Contributions are ADDED to

@@ -67,8 +67,8 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
         sl.key[c] = -1;
         sl.acc[c] = 0.0f;
     }
-    marchSteps(acc, g, e, d, dt, [&](const float x[3]) {
-        const float fx = floorf(x[0]), fy = floorf(x[1]), fz = floorf(x[2]);
+    marchSteps(acc, g, e, d, dt, [&](float x0, float x1, float x2) {
+        const float fx = floorf(x0), fy = floorf(x1), fz = floorf(x2);
         const int32_t ci = int32_t(fx), cj = int32_t(fy), ck = int32_t(fz);
         Cell C;
         if (ci != cell[0] || cj != cell[1] || ck != cell[2]) {
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
         } else {
             stencil8(acc, ci, cj, ck, C);
         }
-        const float u = x[0] - fx, v = x[1] - fy, w = x[2] - fz;
+        const float u = x0 - fx, v = x1 - fy, w = x2 - fz;
         const float rho = trilerp(C, u, v, w);
         const float a = expf(-sigma * rho * dt);
         L += T * (1.0f - a) * rho;        // same operations as the forward march

@@ -351,11 +351,33 @@ __device__ __forceinline__ bool rayClip(Ray& r, const int32_t* bbox)
 
 __device__ __forceinline__ int32_t ifloor(float x) { return int32_t(floorf(x)); }
 
+// Per-axis state is kept in scalar members: selecting among array elements by the minimum axis
+// would be folded by the compiler into a dynamically indexed array and spilled to scratch.
 struct HDDA {
     int32_t dim;
     float t0, t1;
-    int32_t voxel[3], step[3];
-    float delta[3], next[3];
+    int32_t vx, vy, vz;    // current voxel / node origin
+    int32_t sx, sy, sz;    // step direction per axis
+    float dx, dy, dz;      // delta time per axis
+    float nx, ny, nz;      // next crossing time per axis
+
+    __device__ __forceinline__ static void initAxis(float dir, float inv, float pos, int32_t vox, int32_t d, float tStart,
+                                                    int32_t& st, float& de, float& nxt)
+    {
+        de = 0.0f;
+        if (dir == 0.0f) {
+            nxt = kFltMax;
+            st = 0;
+        } else if (inv > 0.0f) {
+            st = 1;
+            nxt = tStart + (float(vox + d) - pos) * inv;
+            de = inv;
+        } else {
+            st = -1;
+            nxt = tStart + (float(vox) - pos) * inv;
+            de = -inv;
+        }
+    }
 
     __device__ __forceinline__ void init(const Ray& r, float startTime, float maxTime, int32_t d)
     {
@@ -364,67 +386,60 @@ struct HDDA {
         t1 = maxTime;
         float pos[3];
         rayAt(r, t0, pos);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            voxel[a] = ifloor(pos[a]) & ~(d - 1);
-            delta[a] = 0.0f;
-            if (r.dir[a] == 0.0f) {
-                next[a] = kFltMax;
-                step[a] = 0;
-            } else if (r.inv[a] > 0.0f) {
-                step[a] = 1;
-                next[a] = t0 + (float(voxel[a] + d) - pos[a]) * r.inv[a];
-                delta[a] = r.inv[a];
-            } else {
-                step[a] = -1;
-                next[a] = t0 + (float(voxel[a]) - pos[a]) * r.inv[a];
-                delta[a] = -r.inv[a];
-            }
-        }
+        vx = ifloor(pos[0]) & ~(d - 1);
+        vy = ifloor(pos[1]) & ~(d - 1);
+        vz = ifloor(pos[2]) & ~(d - 1);
+        initAxis(r.dir[0], r.inv[0], pos[0], vx, d, t0, sx, dx, nx);
+        initAxis(r.dir[1], r.inv[1], pos[1], vy, d, t0, sy, dy, ny);
+        initAxis(r.dir[2], r.inv[2], pos[2], vz, d, t0, sz, dz, nz);
+    }
+
+    __device__ __forceinline__ static int32_t clampVox(float pos, int32_t vox, int32_t oldDim, int32_t d)
+    {
+        const int32_t vmax = (vox + (oldDim - 1)) & ~(d - 1), vmin = vox & ~(d - 1);
+        int32_t v = ifloor(pos) & ~(d - 1);
+        v = v < vmax ? v : vmax;
+        return v > vmin ? v : vmin;
+    }
+
+    __device__ __forceinline__ static void nextAxis(int32_t st, int32_t vox, float pos, float inv, int32_t d, float tStart,
+                                                    float& nxt)
+    {
+        if (st == 0) return;
+        nxt = tStart + (float(vox) - pos) * inv;
+        if (st > 0) nxt += float(d) * inv;
     }
 
     __device__ __forceinline__ bool update(const Ray& r, int32_t d)
     {
         if (dim == d) return false;
-        int32_t vmax[3], vmin[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            vmax[a] = (voxel[a] + (dim - 1)) & ~(d - 1);
-            vmin[a] = voxel[a] & ~(d - 1);
-        }
-        dim = d;
         float pos[3];
         rayAt(r, t0, pos);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int32_t v = ifloor(pos[a]) & ~(d - 1);
-            v = v < vmax[a] ? v : vmax[a];
-            v = v > vmin[a] ? v : vmin[a];
-            voxel[a] = v;
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (step[a] == 0) continue;
-            next[a] = t0 + (float(voxel[a]) - pos[a]) * r.inv[a];
-            if (step[a] > 0) next[a] += float(d) * r.inv[a];
-        }
+        vx = clampVox(pos[0], vx, dim, d);
+        vy = clampVox(pos[1], vy, dim, d);
+        vz = clampVox(pos[2], vz, dim, d);
+        dim = d;
+        nextAxis(sx, vx, pos[0], r.inv[0], d, t0, nx);
+        nextAxis(sy, vy, pos[1], r.inv[1], d, t0, ny);
+        nextAxis(sz, vz, pos[2], r.inv[2], d, t0, nz);
         return true;
     }
 
+    __device__ __forceinline__ bool advance(float& nxt, float de, int32_t& vox, int32_t st)
+    {
+        if (nxt <= t0) nxt += t0 - 0.999999f * nxt + 1.0e-6f; // ENFORCE_FORWARD_STEPPING, math/HDDA.h:163-169
+        t0 = nxt;
+        nxt += float(dim) * de;
+        vox += dim * st;
+        return t0 <= t1;
+    }
+
+    // MinIndex (math/Math.h:294-309) then step along that axis
     __device__ __forceinline__ bool stepOnce()
     {
-        // MinIndex, math/Math.h:294-309
-        const int a = (next[0] < next[1] && next[0] < next[2]) ? 0 : (next[1] < next[2] ? 1 : 2);
-        float na = a == 0 ? next[0] : (a == 1 ? next[1] : next[2]);
-        const float da = a == 0 ? delta[0] : (a == 1 ? delta[1] : delta[2]);
-        const int32_t sa = a == 0 ? step[0] : (a == 1 ? step[1] : step[2]);
-        if (na <= t0) na += t0 - 0.999999f * na + 1.0e-6f; // ENFORCE_FORWARD_STEPPING, math/HDDA.h:163-169
-        t0 = na;
-        na += float(dim) * da;
-        if (a == 0) { next[0] = na; voxel[0] += dim * sa; }
-        else if (a == 1) { next[1] = na; voxel[1] += dim * sa; }
-        else { next[2] = na; voxel[2] += dim * sa; }
-        return t0 <= t1;
+        if (nx < ny && nx < nz) return advance(nx, dx, vx, sx);
+        if (ny < nz) return advance(ny, dy, vy, sy);
+        return advance(nz, dz, vz, sz);
     }
 };
 
@@ -432,7 +447,7 @@ struct HDDA {
 // HDDA over cells of size >= 8 (the leaf level, as TreeMarcher, math/HDDA.h:450-536, with
 // Eps = 1e-6 and Delta = 0.01); a cell is marched when it lies in a leaf (getDim == 1) or in an
 // active tile; steps are globally aligned at t = k*dt with ta <= t < tb inside each marched
-// cell.  onStep(t) is called for every step in order; the forward march and its adjoint share
+// cell.  onStep(x, y, z) gets every step's index-space position, in order; the forward march and its adjoint share
 // this routine, so both see the same steps bit for bit.
 template<class AccT, class F>
 __device__ __forceinline__ void marchSteps(AccT& acc, const GridView& g, const float e[3], const float d[3], float dt,
@@ -456,9 +471,10 @@ __device__ __forceinline__ void marchSteps(AccT& acc, const GridView& g, const f
         if (on && ta < tb) {
             float kf = ceilf(ta / dt);
             for (float t = kf * dt; t < tb; t = kf * dt) {
-                float x[3];
-                rayAt(R, t, x);
-                onStep(x);
+                const float x0 = __builtin_fmaf(t, R.dir[0], R.eye[0]);
+                const float x1 = __builtin_fmaf(t, R.dir[1], R.eye[1]);
+                const float x2 = __builtin_fmaf(t, R.dir[2], R.eye[2]);
+                onStep(x0, x1, x2);
                 const float kn = kf + 1.0f;
                 if (kn == kf) break; // step index beyond 2^24: float cannot advance it
                 kf = kn;

@@ -65,6 +65,24 @@ bool sampleBinnedSupported(const GridView& g, uint64_t n);
 int sampleBinned(const GridView& g, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                  int mode, hipStream_t s);
 
+// Persistent multi-grid ray queries (nvq_multi.hip)
+constexpr uint32_t kMaxGrids = 32;
+constexpr uint32_t kOpZeroCrossing = 0, kOpFogMarch = 1;
+struct MultiSlot {
+    const GridView* view; // device copy of the grid's view
+    uint64_t rayBase;     // first ray of this grid in the concatenated arrays
+    uint64_t rayEnd;      // one past its last ray
+    uint32_t tileBase;    // first 8x8 tile of this grid in the global tile order
+    uint32_t width, height, tilesX, op;
+};
+struct MultiArgs {
+    MultiSlot slot[kMaxGrids];
+    uint32_t k;
+    uint32_t totalTiles;
+};
+int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
+               int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
+
 // Adjoint of the fog march into an int64 fixed-point accumulator (nvq_adjoint.hip).
 int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint64_t n, uint32_t width, float dt,
                      float sigma, const float* L, const float* T, const float* gL, const float* gT, int64_t* accum,

@@ -8,6 +8,7 @@
 #include "nvq_device.h"
 #include "nvq_internal.h"
 #include "nvq_sample.h"
+#include "nvq_rays.h"
 #include "../../include/nvq.h"
 
 #include <cstdlib>
@@ -21,6 +22,7 @@ using namespace nvq;
 
 struct nvq_grid {
     void* dbuf;
+    GridView* dview; // device copy of `view` (persistent multi-grid queries)
     uint64_t bytes;
     int device;
     bool owns; // false when the device memory belongs to the caller (nvq_grid_upload_into)
@@ -115,9 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_stencil(GridView g, const int32
     }
 }
 
-// ------------------------------------------------------------------ zeroCrossing
-// math/HDDA.h:189-214 over the device accessor; normal from the trilinear gradient at ray(t),
-// Grid::indexToWorldGradF (Map::applyIJTF, math/Math.h:1758-1763), normalised.
+// ------------------------------------------------------------------ zeroCrossing / fog march
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_zero_crossing(GridView g, const float* __restrict__ eye,
                                                           const float* __restrict__ dir, Sched s,
@@ -127,84 +127,9 @@ __global__ __launch_bounds__(kBlock) void k_zero_crossing(GridView g, const floa
 {
     uint64_t i;
     if (!schedule(s, i)) return;
-    Accessor<GT> acc(g);
-    const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
-    const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
-    Ray ray = worldToIndexRay(g, e, d);
-    bool found = false;
-    int32_t c[3] = {0, 0, 0};
-    float v = 0.0f, t = 0.0f;
-    if (rayClip(ray, g.bbox) && !(double(ray.t1) > 1e20)) {
-        const float Delta = 1.0001f;
-        float p[3];
-        rayAt(ray, ray.t0, p);
-        c[0] = ifloor(p[0]);
-        c[1] = ifloor(p[1]);
-        c[2] = ifloor(p[2]);
-        const Probe p0 = acc.probe(c[0], c[1], c[2]);
-        HDDA h;
-        h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
-        const float v0 = p0.value;
-        while (!found && h.stepOnce()) {
-            rayAt(ray, h.t0 + Delta, p);
-            c[0] = ifloor(p[0]);
-            c[1] = ifloor(p[1]);
-            c[2] = ifloor(p[2]);
-            const Probe q = acc.probe(c[0], c[1], c[2]);
-            h.update(ray, int32_t(q.dim));
-            if (h.dim > 1 || !q.active) continue;
-            while (h.stepOnce()) { // in the narrow band
-                const Probe r = acc.probe(h.voxel[0], h.voxel[1], h.voxel[2]);
-                if (!r.active) break;
-                v = r.value;
-                if (v * v0 < 0) {
-                    c[0] = h.voxel[0];
-                    c[1] = h.voxel[1];
-                    c[2] = h.voxel[2];
-                    t = h.t0;
-                    found = true;
-                    break;
-                }
-            }
-        }
-    }
-    if (hit) hit[i] = found ? 1 : 0;
-    if (ijk) {
-        ijk[3 * i] = found ? c[0] : 0;
-        ijk[3 * i + 1] = found ? c[1] : 0;
-        ijk[3 * i + 2] = found ? c[2] : 0;
-    }
-    if (tout) tout[i] = found ? t : 0.0f;
-    if (vout) vout[i] = found ? v : 0.0f;
-    if (normal) {
-        float nr[3] = {0.0f, 0.0f, 0.0f};
-        if (found) {
-            float p[3], gi[3], gw[3];
-            rayAt(ray, t, p);
-            const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
-            Cell C;
-            stencil8(acc, int32_t(fx), int32_t(fy), int32_t(fz), C);
-            trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, gi);
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                gw[a] = __builtin_fmaf(gi[0], g.invMatF[a], __builtin_fmaf(gi[1], g.invMatF[3 + a], gi[2] * g.invMatF[6 + a]));
-            const float len = sqrtf(gw[0] * gw[0] + gw[1] * gw[1] + gw[2] * gw[2]);
-            if (len > 0.0f) {
-                const float sc = 1.0f / len;
-                nr[0] = sc * gw[0];
-                nr[1] = sc * gw[1];
-                nr[2] = sc * gw[2];
-            }
-        }
-        normal[3 * i] = nr[0];
-        normal[3 * i + 1] = nr[1];
-        normal[3 * i + 2] = nr[2];
-    }
+    zeroCrossingRay<GT>(g, eye, dir, i, hit, ijk, tout, vout, normal);
 }
 
-// ------------------------------------------------------------------ fog march
-// Emission-absorption over the shared step structure (marchSteps): per step
-// a = exp(-sigma*rho*dt), L += T*(1-a)*rho, T *= a.
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* __restrict__ eye,
                                                       const float* __restrict__ dir, Sched s, float dt, float sigma,
@@ -212,18 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* _
 {
     uint64_t i;
     if (!schedule(s, i)) return;
-    Accessor<GT> acc(g);
-    const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
-    const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
-    float L = 0.0f, T = 1.0f;
-    marchSteps(acc, g, e, d, dt, [&](const float x[3]) {
-        const float rho = trilinear(acc, x[0], x[1], x[2]);
-        const float a = expf(-sigma * rho * dt);
-        L += T * (1.0f - a) * rho;
-        T *= a;
-    });
-    if (Lout) Lout[i] = L;
-    if (Tout) Tout[i] = T;
+    fogMarchRay<GT>(g, eye, dir, i, dt, sigma, Lout, Tout);
 }
 
 // ------------------------------------------------------------------ host helpers
@@ -382,6 +296,7 @@ static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, vo
     }
     g->owns = device_dst == nullptr;
     g->dbuf = d;
+    g->dview = nullptr;
     g->bytes = bytes;
     g->device = device;
     const uint8_t* tree = b + kGridSize;
@@ -418,6 +333,13 @@ static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, vo
     for (int i = 0; i < 3; ++i) in.voxel_size[i] = v.voxelSize[i];
     in.background = v.background;
     in.device_buffer = d;
+    if (hipMalloc(reinterpret_cast<void**>(&g->dview), sizeof(GridView)) != hipSuccess ||
+        hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, S(stream)) != hipSuccess) {
+        if (g->dview) (void)hipFree(g->dview);
+        if (g->owns) (void)hipFree(d);
+        delete g;
+        return NVQ_ERR_HIP;
+    }
     *out = g;
     return NVQ_OK;
 }
@@ -438,6 +360,7 @@ int nvq_grid_free(nvq_grid* grid)
     if (!grid) return NVQ_ERR_INVALID_ARG;
     const hipError_t e1 = hipDeviceSynchronize();
     const hipError_t e2 = grid->owns ? hipFree(grid->dbuf) : hipSuccess;
+    if (grid->dview) (void)hipFree(grid->dview);
     delete grid;
     return (e1 == hipSuccess && e2 == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
 }
@@ -529,6 +452,39 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
     if ((count && (!accum || !out)) || fixed_shift < 0 || fixed_shift > 62) return NVQ_ERR_INVALID_ARG;
     if (count == 0) return NVQ_OK;
     return fixedToFloat(accum, count, fixed_shift, out, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
+}
+
+int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                    const uint32_t* host_image_width, const float* eye, const float* dir, float dt, float sigma_t,
+                    uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal, void* stream)
+{
+    if (!host_grids || !host_ray_offsets || k == 0 || k > kMaxGrids || !(dt > 0.0f)) return NVQ_ERR_INVALID_ARG;
+    MultiArgs A{};
+    A.k = k;
+    uint64_t tiles = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        const nvq_grid* g = host_grids[j];
+        const uint64_t n = host_ray_offsets[j + 1] - host_ray_offsets[j];
+        if (!g || host_ray_offsets[j + 1] < host_ray_offsets[j]) return NVQ_ERR_INVALID_ARG;
+        const uint32_t given = host_image_width ? host_image_width[j] : 0u;
+        if (given && n % given) return NVQ_ERR_INVALID_ARG;
+        const uint32_t w = given ? given : 64u; // linear batches: rows of 64 rays
+        MultiSlot& S = A.slot[j];
+        S.view = g->dview;
+        S.rayBase = host_ray_offsets[j];
+        S.rayEnd = host_ray_offsets[j + 1];
+        S.width = w;
+        S.height = uint32_t((n + w - 1) / w);
+        S.tilesX = (w + 7) / 8;
+        S.tileBase = uint32_t(tiles);
+        S.op = g->info.grid_class == 2u ? kOpFogMarch : kOpZeroCrossing;
+        if (n && (!eye || !dir)) return NVQ_ERR_INVALID_ARG;
+        tiles += uint64_t(S.tilesX) * ((S.height + 7) / 8);
+    }
+    if (tiles >= (uint64_t(1) << 31)) return NVQ_ERR_INVALID_ARG;
+    A.totalTiles = uint32_t(tiles);
+    if (tiles == 0) return NVQ_OK;
+    return multiQuery(A, eye, dir, dt, sigma_t, hit, ijk, out_a, out_b, normal, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
 }
 
 } // extern "C"

@@ -1,0 +1,82 @@
+// Persistent multi-grid ray query: one launch answers the rays of up to kMaxGrids grids of
+// mixed leaf codecs and classes.  Wavefronts pull 8x8 ray tiles from one device work counter
+// until every grid's tiles are taken (the persistent-warp pattern of the reference's
+// iso-surface renderer, examples/ex_raytrace_iso_surface/common.h:79-110, 166-189), so small
+// and large grids share the machine without per-grid launches or tail effects.  Level-set
+// grids run zeroCrossing, fog volumes run the emission-absorption march.
+#include "nvq_device.h"
+#include "nvq_internal.h"
+#include "nvq_rays.h"
+
+#include <hip/hip_runtime.h>
+
+namespace nvq {
+namespace {
+
+template<uint32_t GT>
+__device__ __forceinline__ void runRay(const GridView& g, uint32_t op, const float* eye, const float* dir, uint64_t i,
+                                       float dt, float sigma, uint8_t* hit, int32_t* ijk, float* a, float* b,
+                                       float* normal)
+{
+    if (op == kOpZeroCrossing) zeroCrossingRay<GT>(g, eye, dir, i, hit, ijk, a, b, normal);
+    else fogMarchRay<GT>(g, eye, dir, i, dt, sigma, a, b);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float* __restrict__ eye,
+                                                        const float* __restrict__ dir, float dt, float sigma,
+                                                        uint32_t* __restrict__ counter, uint8_t* __restrict__ hit,
+                                                        int32_t* __restrict__ ijk, float* __restrict__ a,
+                                                        float* __restrict__ b, float* __restrict__ normal)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1u);
+        t = __shfl(t, 0, 64);
+        if (t >= A.totalTiles) break; // every wave reaches this once the counter passes the last tile
+        uint32_t j = 0;
+        while (j + 1 < A.k && A.slot[j + 1].tileBase <= t) ++j;
+        const MultiSlot& S = A.slot[j];
+        const uint32_t local = t - S.tileBase;
+        const uint32_t tx = local % S.tilesX, ty = local / S.tilesX;
+        const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
+        const uint64_t i = S.rayBase + uint64_t(py) * S.width + px;
+        // lanes outside the image or past a ragged last row idle; the wave stays together
+        if (px < S.width && py < S.height && i < S.rayEnd) {
+            const GridView g = *S.view;
+            switch (g.type) {
+            case GT_FLOAT: runRay<GT_FLOAT>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            case GT_FP4: runRay<GT_FP4>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            case GT_FP8: runRay<GT_FP8>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            case GT_FP16: runRay<GT_FP16>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            default: runRay<GT_FPN>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            }
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+} // namespace
+
+int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
+               int32_t* ijk, float* a, float* b, float* normal, hipStream_t s)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    uint32_t* counter = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&counter), 256, s) != hipSuccess) return -2;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
+    const uint64_t waves = A.totalTiles; // one wave per tile is the most that can be busy
+    uint32_t blocks = uint32_t(cus) * 8u;
+    if (uint64_t(blocks) * (kBlock / 64) > waves) blocks = uint32_t((waves + (kBlock / 64) - 1) / (kBlock / 64));
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_multi_query, dim3(blocks), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter, hit, ijk, a,
+                       b, normal);
+    const bool ok = hipGetLastError() == hipSuccess;
+    if (hipFreeAsync(counter, s) != hipSuccess) return -3;
+    return ok ? 0 : -4;
+}
+
+} // namespace nvq

@@ -9,6 +9,7 @@ Each function mirrors one reference call shape, applied to a batch:
 * ``grad_stencil``     GradStencil                                  (math/Stencils.h:455-576)
 * ``zero_crossing``    Ray::worldToIndexF + math::zeroCrossing      (math/HDDA.h:189-214)
 * ``fog_march``        emission-absorption march with HDDA node skipping
+* ``multi_query``      one persistent launch over many grids (zeroCrossing / fog march per grid)
 
 All tensors must be CUDA tensors on the grid's device; work is queued on the
 current torch stream.
@@ -112,6 +113,31 @@ def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):
     check(lib().nvq_fog_march(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t, _ptr(L), _ptr(T),
                               _stream()), "nvq_fog_march")
     return L, T
+
+
+def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.05):
+    """One persistent launch over ``grids`` (<= 32, any codecs).  The rays of grid j are
+    ``eye/dir[offsets[j]:offsets[j+1]]``; ``image_widths[j]`` (0 = linear batch) makes them a
+    row-major image.  Returns (hit, ijk, a, b, normal): level sets give hit/ijk/t/v/normal,
+    fog volumes give radiance in ``a`` and transmittance in ``b``."""
+    eye = _points(eye, torch.float32, "eye")
+    dir = _points(dir, torch.float32, "dir")
+    k = len(grids)
+    if len(offsets) != k + 1:
+        raise ValueError("offsets must have len(grids) + 1 entries")
+    n = eye.shape[0]
+    dev = eye.device
+    hs = (ctypes.c_void_p * max(k, 1))(*[g.handle.value for g in grids])
+    offs = (ctypes.c_uint64 * (k + 1))(*[int(o) for o in offsets])
+    widths = None if image_widths is None else (ctypes.c_uint32 * max(k, 1))(*[int(w) for w in image_widths])
+    hit = torch.zeros(n, dtype=torch.uint8, device=dev)
+    ijk = torch.zeros((n, 3), dtype=torch.int32, device=dev)
+    a = torch.zeros(n, dtype=torch.float32, device=dev)
+    b = torch.zeros(n, dtype=torch.float32, device=dev)
+    nrm = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    check(lib().nvq_multi_query(hs, k, offs, widths, _ptr(eye), _ptr(dir), dt, sigma_t, _ptr(hit), _ptr(ijk),
+                                _ptr(a), _ptr(b), _ptr(nrm), _stream()), "nvq_multi_query")
+    return hit, ijk, a, b, nrm
 
 
 def fog_march_backward(grid, eye, dir, radiance, grad_radiance, transmittance=None, grad_transmittance=None, dt=0.5,
