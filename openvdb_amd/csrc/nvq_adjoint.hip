@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
     const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
     const float Ltot = Lfwd[i];
     const float sdt = sigma * dt;
-    const float tTerm = gT * sdt * Tfwd[i];
+    const float tTerm = gTin ? gT * sdt * Tfwd[i] : 0.0f; // Tfwd may be NULL when no dL/dT is given
     float L = 0.0f, T = 1.0f;
     Slots sl;
     int32_t cell[3] = {0x7fffffff, 0, 0};
