@@ -10,7 +10,12 @@ gradient.  One step = one pass over both batches.  Inputs are resident in HBM
 before the timed region.  value = queries / second over the whole step.
 
 Also reported (same JSON line, "extra"): level-set ray hits (512x512 camera,
-sphere R=100) and the fog march (1024x1024 rays through the 512^3 fog) in rays/s.
+sphere R=100) and the fog march (1024x1024 rays through the 512^3 fog) in rays/s;
+the differentiable march (256^3 fBm fog seed 7 against a seed-11 target, 64
+cameras on a radius-600 shell x 256^2 rays, L2 image loss) as the time ratio of
+forward+adjoint to forward; and the quantised batched query (16 level-set and fog
+grids of bbox 64^3..1024^3, each as Fp16 / Fp8 / FpN(1e-3), ~2^24 rays per codec
+in one persistent launch) in rays/s per codec.
 
 The CPU baseline times the reference host code (oracle/_ref/libnanoref.so:
 ReadAccessor + SampleFromVoxels<Acc,1> + the same FD6 gradient) on a bounded
@@ -41,6 +46,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     p.add_argument("--no-extra", action="store_true", help="skip the ray workloads")
     p.add_argument("--cpu-log2-points", type=int, default=21)
+    p.add_argument("--no-diff", action="store_true", help="skip the differentiable-march workload")
+    p.add_argument("--no-multi", action="store_true", help="skip the quantised multi-grid workload")
     return p.parse_args()
 
 
@@ -112,6 +119,125 @@ def timed_launches(torch, fns, steps, warmup):
     wall = time.perf_counter() - t0
     per = [sum(ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(steps)) for i in range(len(fns))]
     return wall, per
+
+
+def shell_cameras(nvq, count, res, centre, radius, seed, fov=45.0):
+    """`count` pinhole cameras (RayGenOp convention) on a seeded sphere shell looking at `centre`."""
+    r = np.random.default_rng(seed)
+    v = r.normal(size=(count, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    eyes, dirs = [], []
+    for k in range(count):
+        eye = tuple(float(x) for x in np.asarray(centre) + radius * v[k])
+        up = (0.0, 1.0, 0.0) if abs(v[k][1]) < 0.9 else (1.0, 0.0, 0.0)
+        e, d = nvq.tools.camera_rays(res, res, eye, tuple(float(c) for c in centre), up, fov)
+        eyes.append(e)
+        dirs.append(d)
+    return np.concatenate(eyes), np.concatenate(dirs)
+
+
+def diff_march(torch, nvq, steps):
+    """Differentiable march: forward time vs forward + L2-loss adjoint time (one step each)."""
+    box = [0, 0, 0, 255, 255, 255]
+    fp = nvq.DeviceGrid(nvq.tools.create_fog_fbm(box, seed=7))
+    ft = nvq.DeviceGrid(nvq.tools.create_fog_fbm(box, seed=11))
+    e, d = shell_cameras(nvq, 64, 256, (128.0, 128.0, 128.0), 600.0, seed=2024)
+    E, D = torch.from_numpy(e).cuda(), torch.from_numpy(d).cuda()
+    n = E.shape[0]
+    target, _ = nvq.ops.fog_march(ft, E, D, 0.5, 0.05, image_width=256)
+    accum = torch.zeros(fp.leaf_count * 512, dtype=torch.int64, device="cuda")
+    grad = torch.empty(fp.leaf_count * 512, dtype=torch.float32, device="cuda")
+
+    def fwd():
+        return nvq.ops.fog_march(fp, E, D, 0.5, 0.05, image_width=256)
+
+    def fwd_bwd():
+        L, T = fwd()
+        gL = 2.0 * (L - target)
+        accum.zero_()
+        nvq.ops.fog_march_backward(fp, E, D, L, gL, None, None, 0.5, 0.05, image_width=256, accum=accum)
+        nvq.ops.fixed_to_float(accum, out=grad)
+
+    wf, pf = timed_launches(torch, [fwd], steps, 2)
+    wb, pb = timed_launches(torch, [fwd_bwd], steps, 2)
+    L, _ = fwd()
+    return {"rays": n, "leaves": fp.leaf_count, "fwd_ms": 1e3 * wf / steps, "fwd_bwd_ms": 1e3 * wb / steps,
+            "ratio": wb / wf, "fwd_rays_per_s": n * steps / wf,
+            "loss": float(((L - target) ** 2).sum().item()), "grad_nonzero": int((accum != 0).sum().item())}
+
+
+def multi_grid_set(nvq):
+    """16 grids of uneven size: level-set spheres and tori with bboxes ~64^3..1024^3 and fBm fogs."""
+    from concurrent.futures import ThreadPoolExecutor
+    specs = [("sphere", 29.0), ("sphere", 60.0), ("sphere", 90.0), ("sphere", 180.0), ("sphere", 250.0),
+             ("sphere", 508.0), ("torus", (24.0, 8.0)), ("torus", (50.0, 14.0)), ("torus", (100.0, 30.0)),
+             ("torus", (200.0, 60.0)), ("torus", (380.0, 120.0)), ("fog", (64, 3)), ("fog", (96, 5)),
+             ("fog", (128, 7)), ("fog", (192, 9)), ("fog", (256, 13))]
+
+    def build(spec):
+        kind, p = spec
+        if kind == "sphere":
+            return nvq.tools.create_level_set_sphere(p, (0.0, 0.0, 0.0), 1.0, 3.0)
+        if kind == "torus":
+            return nvq.tools.create_level_set_torus(p[0], p[1], (0.0, 0.0, 0.0), 1.0, 3.0)
+        return nvq.tools.create_fog_fbm([0, 0, 0, p[0] - 1, p[0] - 1, p[0] - 1], seed=p[1])
+
+    with ThreadPoolExecutor(8) as ex:
+        return specs, list(ex.map(build, specs))
+
+
+def multi_query_bench(torch, nvq, steps, total_log2=24):
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.time()
+    specs, grids = multi_grid_set(nvq)
+    log(f"C5: 16 float grids built in {time.time() - t0:.1f}s")
+    # rays per grid proportional to the bbox surface area, as square images of side a multiple of 8
+    boxes = []
+    for g in grids:
+        d = nvq.DeviceGrid(g)
+        boxes.append(d.index_bbox)
+        d.close()
+    area = np.array([2 * ((b[3] - b[0] + 1) * (b[4] - b[1] + 1) + (b[4] - b[1] + 1) * (b[5] - b[2] + 1) +
+                          (b[3] - b[0] + 1) * (b[5] - b[2] + 1)) for b in boxes], np.float64)
+    share = area / area.sum() * (1 << total_log2)
+    sides = [max(8, int(round(np.sqrt(s) / 8.0)) * 8) for s in share]
+    eyes, dirs, offs = [], [], [0]
+    r = np.random.default_rng(77)
+    for b, side in zip(boxes, sides):
+        lo, hi = np.array(b[:3], np.float64), np.array(b[3:], np.float64) + 1.0
+        c = 0.5 * (lo + hi)
+        diag = float(np.linalg.norm(hi - lo))
+        v = r.normal(size=3)
+        v /= np.linalg.norm(v)
+        e, d = nvq.tools.camera_rays(side, side, tuple(c + 1.3 * diag * v), tuple(c), (0.0, 1.0, 0.0)
+                                     if abs(v[1]) < 0.9 else (1.0, 0.0, 0.0), 45.0)
+        eyes.append(e)
+        dirs.append(d)
+        offs.append(offs[-1] + side * side)
+    E = torch.from_numpy(np.concatenate(eyes)).cuda()
+    D = torch.from_numpy(np.concatenate(dirs)).cuda()
+    n = E.shape[0]
+    out = {"rays": n, "grid_rays": [s * s for s in sides], "grids": [f"{k}:{p}" for k, p in specs], "codecs": {}}
+    for codec in ("fp16", "fp8", "fpn"):
+        t0 = time.time()
+        with ThreadPoolExecutor(8) as ex:
+            q = list(ex.map(lambda g: nvq.tools.quantize(g, codec, 1e-3), grids))
+        dg = [nvq.DeviceGrid(x) for x in q]
+        del q
+        log(f"C5 {codec}: quantised and uploaded in {time.time() - t0:.1f}s")
+        res = {}
+
+        def run():
+            res["r"] = nvq.ops.multi_query(dg, E, D, offs, sides, 0.5, 0.05)
+
+        w, p = timed_launches(torch, [run], steps, 1)
+        hit = res["r"][0]
+        out["codecs"][codec] = {"rays_per_s": n * steps / w, "ms": 1e3 * w / steps, "kernel_ms": p[0] / steps,
+                                "bytes": int(sum(g.nbytes for g in dg)), "hits": int(hit.sum().item())}
+        for g in dg:
+            g.close()
+        log(f"C5 {codec}: {out['codecs'][codec]['rays_per_s']/1e6:.1f} M rays/s")
+    return out
 
 
 def cpu_baseline(grids_np, pts_np, log2n):
@@ -244,6 +370,12 @@ def main():
         extra["c3_mean_transmittance"] = float(T.mean().item())
         log(f"C1 {extra['c1_rays_per_s']/1e9:.2f} G rays/s ({extra['c1_hits']} hits); "
             f"C3 {extra['c3_rays_per_s']/1e6:.1f} M rays/s")
+        if not args.no_diff:
+            extra["c4"] = diff_march(torch, nvq, 5)
+            log(f"C4: fwd {extra['c4']['fwd_ms']:.2f} ms, fwd+bwd {extra['c4']['fwd_bwd_ms']:.2f} ms, "
+                f"ratio {extra['c4']['ratio']:.2f}")
+        if not args.no_multi:
+            extra["c5"] = multi_query_bench(torch, nvq, 3)
 
     cpu = None
     if not args.no_cpu:

@@ -157,8 +157,11 @@ def fog_march_backward(grid, eye, dir, radiance, grad_radiance, transmittance=No
     return accum
 
 
-def fixed_to_float(accum, fixed_shift=40):
-    out = torch.empty(accum.shape, dtype=torch.float32, device=accum.device)
+def fixed_to_float(accum, fixed_shift=40, out=None):
+    if out is None:
+        out = torch.empty(accum.shape, dtype=torch.float32, device=accum.device)
+    elif out.numel() != accum.numel() or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 tensor with accum's element count")
     check(lib().nvq_grad_fixed_to_float(_ptr(accum), accum.numel(), fixed_shift, _ptr(out), _stream()),
           "nvq_grad_fixed_to_float")
     return out
