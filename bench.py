@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--cpu-log2-points", type=int, default=21)
     p.add_argument("--no-diff", action="store_true", help="skip the differentiable-march workload")
     p.add_argument("--no-multi", action="store_true", help="skip the quantised multi-grid workload")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="also run one elementwise copy of MIB MiB (reads and writes MIB MiB) to scale HBM counters")
     return p.parse_args()
 
 
@@ -304,6 +306,14 @@ def main():
         outs.append((torch.empty(n, device="cuda"), torch.empty(n, device="cuda"), torch.empty((n, 3), device="cuda")))
     torch.cuda.synchronize()
     log(f"inputs ready: 2 x 2^{args.log2_points} points")
+
+    if args.copy_calibration:
+        src = torch.rand(args.copy_calibration << 18, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
+        del src, dst
 
     lib = nvq.lib()
     import ctypes
