@@ -12,10 +12,10 @@
 // stored per step; L and T_N come from the forward outputs.  dLoss/drho_k is spread over the
 // eight trilinear corners of step k with weights (1-u|u)(1-v|v)(1-w|w).
 //
-// Deterministic reduction: contributions are first summed per ray over runs of consecutive
-// steps that share a trilinear cell (a segmented reduction keyed by the corner voxel), then
-// added into an int64 fixed-point accumulator per leaf value (key = leaf index * 512 + voxel
-// offset) with integer atomics.  Integer addition is associative, so the result does not
+// Deterministic reduction: contributions are first summed per ray and per corner voxel over
+// the run of consecutive steps whose trilinear cells contain that voxel (a segmented reduction
+// in the sliding corner window), then added into an int64 fixed-point accumulator per leaf
+// value (key = leaf index * 512 + voxel offset) with integer atomics.  Integer addition is associative, so the result does not
 // depend on thread scheduling; the accumulator is converted through FP64.
 #include "nvq_device.h"
 #include "nvq_internal.h"
@@ -25,23 +25,6 @@
 
 namespace nvq {
 namespace {
-
-struct Slots {
-    int32_t key[8];
-    float acc[8];
-};
-
-__device__ __forceinline__ void flush(Slots& s, unsigned long long* __restrict__ accum, float scale)
-{
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        if (s.key[c] >= 0 && s.acc[c] != 0.0f) {
-            const long long q = __float2ll_rn(s.acc[c] * scale);
-            if (q != 0) atomicAdd(accum + s.key[c], static_cast<unsigned long long>(q));
-        }
-        s.acc[c] = 0.0f;
-    }
-}
 
 __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const float* __restrict__ eye,
                                                           const float* __restrict__ dir, Sched s, float dt, float sigma,
@@ -60,38 +43,16 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
     const float sdt = sigma * dt;
     const float tTerm = gTin ? gT * sdt * Tfwd[i] : 0.0f; // Tfwd may be NULL when no dL/dT is given
     float L = 0.0f, T = 1.0f;
-    Slots sl;
-    int32_t cell[3] = {0x7fffffff, 0, 0};
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        sl.key[c] = -1;
-        sl.acc[c] = 0.0f;
-    }
+    CellWindow<true> W;
+    auto sink = [&](int32_t key, float sum) {
+        const long long q = __float2ll_rn(sum * scale);
+        if (q != 0) atomicAdd(accum + key, static_cast<unsigned long long>(q));
+    };
     marchSteps(acc, g, e, d, dt, [&](float x0, float x1, float x2) {
         const float fx = floorf(x0), fy = floorf(x1), fz = floorf(x2);
-        const int32_t ci = int32_t(fx), cj = int32_t(fy), ck = int32_t(fz);
-        Cell C;
-        if (ci != cell[0] || cj != cell[1] || ck != cell[2]) {
-            flush(sl, accum, scale);
-            cell[0] = ci;
-            cell[1] = cj;
-            cell[2] = ck;
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const Probe p = acc.probe(ci + a, cj + b, ck + c);
-                        C.v[a][b][c] = p.value;
-                        sl.key[a * 4 + b * 2 + c] =
-                            p.leaf ? int32_t(uint32_t((p.leaf - g.leaf0) / 2144u) * 512u + p.n) : -1;
-                    }
-        } else {
-            stencil8(acc, ci, cj, ck, C);
-        }
+        W.moveTo(acc, g, int32_t(fx), int32_t(fy), int32_t(fz), sink);
         const float u = x0 - fx, v = x1 - fy, w = x2 - fz;
-        const float rho = trilerp(C, u, v, w);
+        const float rho = trilerp(W.c, u, v, w);
         const float a = expf(-sigma * rho * dt);
         L += T * (1.0f - a) * rho;        // same operations as the forward march
         const float R = Ltot - L;         // radiance of the later steps
@@ -103,9 +64,9 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
-                for (int c = 0; c < 2; ++c) sl.acc[aa * 4 + b * 2 + c] += gk * (wx[aa] * wy[b] * wz[c]);
+                for (int c = 0; c < 2; ++c) W.sum[aa * 4 + b * 2 + c] += gk * (wx[aa] * wy[b] * wz[c]);
     });
-    flush(sl, accum, scale);
+    W.flush(sink);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fixed_to_float(const long long* __restrict__ in, uint64_t n, double inv,

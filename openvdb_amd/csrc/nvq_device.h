@@ -9,6 +9,8 @@
 // lookups are bit-identical to the reference host code.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <utility>
 #include <stdint.h>
 
 namespace nvq {
@@ -271,6 +273,133 @@ __device__ __forceinline__ float trilinear(AccT& acc, float x, float y, float z)
     stencil8(acc, int32_t(fx), int32_t(fy), int32_t(fz), c);
     return trilerp(c, x - fx, y - fy, z - fz);
 }
+
+// ------------------------------------------------------------------ sliding corner window
+// The eight corner values of the current trilinear cell, kept across the steps of a march.  A
+// move to a cell whose offset is -1, 0 or +1 on every axis keeps the shared corners and looks
+// up only the new ones; larger jumps reload all eight.  Values are ReadAccessor::getValue
+// results, so the sample does not depend on the window.  With KEYS, every corner also carries
+// its leaf-value index (leaf index * 512 + voxel offset, -1 outside leaves; Float leaves) and a
+// partial sum that is handed to `sink(key, sum)` exactly once, when the corner leaves the window:
+// the cells sharing a corner voxel form a convex 2x2x2 block, so a ray visits them in one run.
+struct NoSink {
+    __device__ __forceinline__ void operator()(int32_t, float) const {}
+};
+
+// f(std::integral_constant<int, I>) for I = 0..N-1, so register arrays are indexed by constants
+template<class F, int... I>
+__device__ __forceinline__ void staticForImpl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template<int N, class F>
+__device__ __forceinline__ void staticFor(F&& f)
+{
+    staticForImpl(f, std::make_integer_sequence<int, N>{});
+}
+
+// slot k (0..3) among those whose `bit` is clear (slot s = a*4 + b*2 + e)
+__host__ __device__ constexpr int clearSlot(int k, int bit)
+{
+    return bit == 4 ? k : (bit == 2 ? (((k & 2) << 1) | (k & 1)) : (k << 1));
+}
+
+template<bool KEYS>
+struct CellWindow {
+    int32_t ci, cj, ck;
+    Cell c;
+    int32_t key[KEYS ? 8 : 1];
+    float sum[KEYS ? 8 : 1];
+
+    __device__ __forceinline__ CellWindow() : ci(0x7fffffff), cj(0), ck(0)
+    {
+        if constexpr (KEYS) {
+            staticFor<8>([&](auto S) {
+                key[S] = -1;
+                sum[S] = 0.0f;
+            });
+        }
+    }
+
+    template<int S, class Sink>
+    __device__ __forceinline__ void drop(Sink& sink)
+    {
+        if constexpr (KEYS) {
+            if (key[S] >= 0 && sum[S] != 0.0f) sink(key[S], sum[S]);
+            key[S] = -1;
+            sum[S] = 0.0f;
+        }
+    }
+
+    // shift the window by d in {-1, 0, +1} along the axis of slot bit BIT (4 = x, 2 = y, 1 = z).
+    // Both slots of a pair are written unconditionally with selected values (the kept corner; the
+    // vacated slot is reloaded afterwards), so every register index stays a constant.
+    template<int BIT, class Sink>
+    __device__ __forceinline__ void shift(int32_t d, Sink& sink)
+    {
+        if (d == 0) return;
+        const bool up = d > 0;
+        staticFor<4>([&](auto K) {
+            constexpr int lo = clearSlot(decltype(K)::value, BIT), hi = lo | BIT;
+            constexpr int la = lo >> 2, lb = (lo >> 1) & 1, le = lo & 1;
+            constexpr int ha = hi >> 2, hb = (hi >> 1) & 1, he = hi & 1;
+            const float kept = up ? c.v[ha][hb][he] : c.v[la][lb][le];
+            c.v[la][lb][le] = kept;
+            c.v[ha][hb][he] = kept;
+            if constexpr (KEYS) {
+                const int32_t kOut = up ? key[lo] : key[hi], kKeep = up ? key[hi] : key[lo];
+                const float sOut = up ? sum[lo] : sum[hi], sKeep = up ? sum[hi] : sum[lo];
+                if (kOut >= 0 && sOut != 0.0f) sink(kOut, sOut);
+                key[lo] = up ? kKeep : -1;
+                sum[lo] = up ? sKeep : 0.0f;
+                key[hi] = up ? -1 : kKeep;
+                sum[hi] = up ? 0.0f : sKeep;
+            }
+        });
+    }
+
+    template<class AccT, class Sink>
+    __device__ __forceinline__ void moveTo(AccT& acc, const GridView& g, int32_t ni, int32_t nj, int32_t nk, Sink& sink)
+    {
+        const int32_t dx = ni - ci, dy = nj - cj, dz = nk - ck;
+        if ((dx | dy | dz) == 0) return;
+        uint32_t need = 0xffu;
+        if (uint32_t(dx + 1) > 2u || uint32_t(dy + 1) > 2u || uint32_t(dz + 1) > 2u) {
+            staticFor<8>([&](auto S) { drop<decltype(S)::value>(sink); });
+        } else {
+            shift<4>(dx, sink);
+            shift<2>(dy, sink);
+            shift<1>(dz, sink);
+            // new corner (a,b,e) was an old corner iff a+dx, b+dy and e+dz all lie in {0,1}
+            // (a = 1 is slots 0xf0, b = 1 is 0xcc, e = 1 is 0xaa)
+            const uint32_t mx = dx == 0 ? 0u : (dx > 0 ? 0xf0u : 0x0fu);
+            const uint32_t my = dy == 0 ? 0u : (dy > 0 ? 0xccu : 0x33u);
+            const uint32_t mz = dz == 0 ? 0u : (dz > 0 ? 0xaau : 0x55u);
+            need = mx | my | mz;
+        }
+        ci = ni;
+        cj = nj;
+        ck = nk;
+        staticFor<8>([&](auto S) {
+            constexpr int s = decltype(S)::value, a = s >> 2, b = (s >> 1) & 1, e = s & 1;
+            if (!(need & (1u << s))) return;
+            if constexpr (KEYS) {
+                const Probe p = acc.probe(ni + a, nj + b, nk + e);
+                c.v[a][b][e] = p.value;
+                key[s] = p.leaf ? int32_t(uint32_t((p.leaf - g.leaf0) / 2144u) * 512u + p.n) : -1;
+                sum[s] = 0.0f;
+            } else {
+                c.v[a][b][e] = acc.getValue(ni + a, nj + b, nk + e);
+            }
+        });
+    }
+
+    template<class Sink>
+    __device__ __forceinline__ void flush(Sink& sink)
+    {
+        staticFor<8>([&](auto S) { drop<decltype(S)::value>(sink); });
+    }
+};
 
 // TrilinearSampler::gradient, math/SampleFromVoxels.h:232-271 (index space)
 __device__ __forceinline__ void trilinearGradient(const Cell& s, float u, float v, float w, float out[3])

@@ -100,8 +100,12 @@ __device__ __forceinline__ void fogMarchRay(const GridView& g, const float* __re
     const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
     const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
     float L = 0.0f, T = 1.0f;
+    CellWindow<false> W;
+    NoSink none;
     marchSteps(acc, g, e, d, dt, [&](float x0, float x1, float x2) {
-        const float rho = trilinear(acc, x0, x1, x2);
+        const float fx = floorf(x0), fy = floorf(x1), fz = floorf(x2);
+        W.moveTo(acc, g, int32_t(fx), int32_t(fy), int32_t(fz), none);
+        const float rho = trilerp(W.c, x0 - fx, x1 - fy, x2 - fz);
         const float a = expf(-sigma * rho * dt);
         L += T * (1.0f - a) * rho;
         T *= a;
