@@ -1,0 +1,119 @@
+"""Quantised grids (Fp16 / Fp8 / FpN at tolerance 1e-3) in one persistent multi-grid launch:
+per-grid errors against the FP64 oracle.
+
+(i)  arithmetic error: GPU FP32 decode + sample vs an FP64 evaluation of the same codes
+     (code * quantum + min in FP64, oracle ``sample_f64`` / ``fog_march``);
+(ii) codec error: FP64 evaluation of the quantised grid vs FP64 evaluation of its FP32
+     source grid.  NanoVDB's encoders bound it in closed form: |decoded - source| <= quantum/2
+     per leaf for Fp8/Fp16 (tools/CreateNanoGrid.h:1331-1474, round to nearest code) and <= the
+     tolerance for FpN (absolute-difference oracle); a trilinear sample is a convex
+     combination of eight voxels, so it inherits the same bound.
+"""
+import numpy as np
+import pytest
+
+from conftest import points_near, rng
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 1e-3
+GRIDS = [("sphere", 29.0), ("sphere", 90.0), ("torus", (50.0, 14.0)), ("torus", (100.0, 30.0)),
+         ("fog", (64, 3)), ("fog", (128, 7))]
+LEAF_BYTES = {"fp16": 1120, "fp8": 608}
+
+
+def _nvq():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import openvdb_amd
+    return openvdb_amd
+
+
+@pytest.fixture(scope="module")
+def sources():
+    nvq = _nvq()
+    out = []
+    for kind, p in GRIDS:
+        if kind == "sphere":
+            out.append(nvq.tools.create_level_set_sphere(p, (0.0, 0.0, 0.0), 1.0, 3.0))
+        elif kind == "torus":
+            out.append(nvq.tools.create_level_set_torus(p[0], p[1], (0.0, 0.0, 0.0), 1.0, 3.0))
+        else:
+            out.append(nvq.tools.create_fog_fbm([0, 0, 0, p[0] - 1, p[0] - 1, p[0] - 1], seed=p[1]))
+    return out
+
+
+def _max_half_quantum(buf, leaf_bytes):
+    tree = 672
+    off_leaf = int(np.frombuffer(buf[tree:tree + 8].tobytes(), np.int64)[0])
+    count = int(np.frombuffer(buf[tree + 32:tree + 36].tobytes(), np.uint32)[0])
+    base = tree + off_leaf
+    idx = base + np.arange(count, dtype=np.int64) * leaf_bytes + 84
+    q = np.frombuffer(buf, np.uint8)[idx[:, None] + np.arange(4)].copy().view(np.float32).ravel()
+    return 0.5 * float(q.max())
+
+
+def _camera(nvq, buf, side, seed):
+    from conftest import grid_bbox
+    lo, hi = grid_bbox(buf)
+    lo = lo.astype(np.float64)
+    hi = hi.astype(np.float64) + 1.0
+    c = 0.5 * (lo + hi)
+    v = rng(seed).normal(size=3)
+    v /= np.linalg.norm(v)
+    eye = c + 1.3 * np.linalg.norm(hi - lo) * v
+    up = (0.0, 1.0, 0.0) if abs(v[1]) < 0.9 else (1.0, 0.0, 0.0)
+    return nvq.tools.camera_rays(side, side, tuple(eye), tuple(c), up, 45.0)
+
+
+@pytest.mark.parametrize("codec", ["fp16", "fp8", "fpn"])
+def test_quantised_errors_per_grid(oracle, sources, codec):
+    nvq = _nvq()
+    quant = [nvq.tools.quantize(s, codec, TOL) for s in sources]
+    dgrids = [nvq.DeviceGrid(q) for q in quant]
+    side = 40
+    cams = [_camera(nvq, s, side, 31 + j) for j, s in enumerate(sources)]
+    eye = np.concatenate([c[0] for c in cams]).astype(np.float32)
+    dir = np.concatenate([c[1] for c in cams]).astype(np.float32)
+    offs = np.arange(len(sources) + 1, dtype=np.uint64) * side * side
+    E, D = torch.from_numpy(eye).cuda(), torch.from_numpy(dir).cuda()
+    hit, ijk, a, b, nrm = nvq.ops.multi_query(dgrids, E, D, offs, [side] * len(sources), 0.5, 0.05)
+    hit, ijk, a = hit.cpu().numpy(), ijk.cpu().numpy(), a.cpu().numpy()
+    for j, ((kind, p), src, q, dg) in enumerate(zip(GRIDS, sources, quant, dgrids)):
+        # point samples: (i) and (ii) with the closed-form codec bound
+        xyz = points_near(oracle, src, 20000, seed=500 + j)
+        _, val, _ = nvq.ops.sample(dg, torch.from_numpy(xyz).cuda(), nn=False, grad_mode=0)
+        val = val.cpu().numpy().astype(np.float64)
+        vq, _ = oracle.sample_f64(q, xyz)
+        vs, _ = oracle.sample_f64(src, xyz)
+        arith = float(np.abs(val - vq).max())
+        codec_err = float(np.abs(vq - vs).max())
+        bound = TOL if codec == "fpn" else _max_half_quantum(q, LEAF_BYTES[codec])
+        scale = max(1.0, float(np.abs(vq).max()))
+        lo, hi = int(offs[j]), int(offs[j + 1])
+        if kind == "fog":
+            Lq, _, steps = oracle.fog_march(q, eye[lo:hi], dir[lo:hi], 0.5, 0.05)
+            Ls, _, _ = oracle.fog_march(src, eye[lo:hi], dir[lo:hi], 0.5, 0.05)
+            assert steps.max() > 50
+            ray_arith = float(np.max(np.abs(a[lo:hi] - Lq) / np.maximum(np.abs(Lq), 1e-6)))
+            ray_codec = float(np.max(np.abs(Lq - Ls) / np.maximum(np.abs(Ls), 1e-6)))
+            ray_txt = f"radiance rel err (i) {ray_arith:.2e} (ii) {ray_codec:.2e}"
+            assert ray_arith <= 1e-5
+        else:
+            oh, oijk, ot, _, _ = oracle.zero_crossing(q, eye[lo:hi], dir[lo:hi])
+            sh, sijk, st, _, _ = oracle.zero_crossing(src, eye[lo:hi], dir[lo:hi])
+            assert np.array_equal(hit[lo:hi], oh) and np.array_equal(ijk[lo:hi], oijk)
+            assert np.array_equal(a[lo:hi].view(np.uint32), ot.view(np.uint32))
+            both = (oh == 1) & (sh == 1)
+            moved = int((np.any(oijk != sijk, axis=1) & both).sum()) + int((oh != sh).sum())
+            dt = float(np.abs(ot[both] - st[both]).max()) if both.any() else 0.0
+            ray_txt = (f"hits {int(oh.sum())}/{hi - lo}, (i) hit voxels and t bit-exact, (ii) {moved} rays "
+                       f"change hit voxel, max |dt| {dt:.2e}")
+            assert int(oh.sum()) > 100
+        print(f"[{codec}] {kind}{p}: sample (i) {arith:.2e} (ii) {codec_err:.2e} <= bound {bound:.2e}; {ray_txt}",
+              flush=True)
+        assert arith <= 2e-6 * scale
+        assert codec_err <= bound * (1.0 + 1e-5) + 1e-7
+    for g in dgrids:
+        g.close()
