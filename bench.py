@@ -69,10 +69,12 @@ def make_points(torch, nvq, grid, n, gen, surface):
 
 
 def torus_surface(torch, gen, m, R=300.0, r=80.0):
+    """Points on the torus surface; createLevelSetTorus lays the ring in the xz-plane
+    (tools/CreatePrimitives.h:161), y is its axis."""
     th = torch.rand(m, generator=gen, device="cuda") * (2 * np.pi)
     ph = torch.rand(m, generator=gen, device="cuda") * (2 * np.pi)
     w = R + r * torch.cos(ph)
-    return torch.stack([w * torch.cos(th), w * torch.sin(th), r * torch.sin(ph)], 1)
+    return torch.stack([w * torch.cos(th), r * torch.sin(ph), w * torch.sin(th)], 1)
 
 
 def fog_surface(torch, nvq, grid, gen, m):

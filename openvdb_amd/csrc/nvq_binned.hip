@@ -11,9 +11,12 @@
 //   k_bin_sample   : one workgroup per bin stages the 20^3 brick of voxel values
 //                    around the bin in LDS (ReadAccessor::getValue semantics, so
 //                    results are bit-identical) and evaluates the bin's queries
-//                    from LDS; outputs go back to each query's original index.
+//                    from LDS; each query's results go to its original index as
+//                    one aligned 32-byte record {nn, val, grad xyz}, a whole memory
+//                    sector, instead of three partial-sector writes to the outputs.
 //   k_sample_list  : queries whose stencil leaves the brick (border bins) run
-//                    through the lane accessor.
+//                    through the lane accessor, into the same records.
+//   k_unpack       : records -> the requested outputs, coalesced.
 #include "nvq_device.h"
 #include "nvq_internal.h"
 #include "nvq_sample.h"
@@ -40,6 +43,19 @@ struct BinLayout {
     uint32_t nbins;
     uint32_t bits;   // key bits for the radix sort
 };
+
+struct alignas(32) Rec {
+    float4 a; // nn, val, grad x, grad y
+    float4 b; // grad z, unused
+};
+
+__device__ __forceinline__ void storeRec(Rec* __restrict__ rec, uint32_t id, float nn, float val, const float g[3])
+{
+    Rec r;
+    r.a = make_float4(nn, val, g[0], g[1]);
+    r.b = make_float4(g[2], 0.0f, 0.0f, 0.0f);
+    rec[id] = r;
+}
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t m, uint32_t v)
 {
@@ -108,8 +124,8 @@ template<uint32_t GT>
 __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L, const float* __restrict__ xyz,
                                                          int world, const uint32_t* __restrict__ sorted,
                                                          const uint32_t* __restrict__ begin,
-                                                         const uint32_t* __restrict__ end, float* __restrict__ nn,
-                                                         float* __restrict__ val, float* __restrict__ grad, int mode,
+                                                         const uint32_t* __restrict__ end, Rec* __restrict__ rec,
+                                                         bool wantNN, bool wantVal, int mode,
                                                          SampleConsts k, uint32_t* __restrict__ spill,
                                                          uint32_t* __restrict__ spillCount)
 {
@@ -157,12 +173,12 @@ __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L
             return trilerp(c, qx - gx, qy - gy, qz - gz);
         };
         const int32_t c0 = (ci * kBrick + cj) * kBrick + ck;
-        if (nn) {
+        float nnv = 0.0f, vv = 0.0f;
+        if (wantNN) {
             const int32_t ri = int32_t(rintf(p[0])) - o[0], rj = int32_t(rintf(p[1])) - o[1], rk = int32_t(rintf(p[2])) - o[2];
-            nn[id] = brick[(ri * kBrick + rj) * kBrick + rk];
+            nnv = brick[(ri * kBrick + rj) * kBrick + rk];
         }
-        if (val) val[id] = cell(p[0], p[1], p[2]);
-        if (!grad) continue;
+        if (wantVal) vv = cell(p[0], p[1], p[2]);
         float g3[3] = {0.0f, 0.0f, 0.0f};
         if (mode == 1) {
             g3[0] = (cell(p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f) - cell(p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f)) * k.fdx;
@@ -182,18 +198,15 @@ __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L
                     for (int e = 0; e < 2; ++e) C.v[a][bb][e] = brick[c0 + (a * kBrick + bb) * kBrick + e];
             trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, g3);
         }
-        grad[3 * uint64_t(id)] = g3[0];
-        grad[3 * uint64_t(id) + 1] = g3[1];
-        grad[3 * uint64_t(id) + 2] = g3[2];
+        storeRec(rec, id, nnv, vv, g3);
     }
 }
 
 template<uint32_t GT>
 __global__ __launch_bounds__(kThreads) void k_sample_list(GridView g, const float* __restrict__ xyz, int world,
                                                           const uint32_t* __restrict__ list,
-                                                          const uint32_t* __restrict__ count, float* __restrict__ nn,
-                                                          float* __restrict__ val, float* __restrict__ grad, int mode,
-                                                          SampleConsts k)
+                                                          const uint32_t* __restrict__ count, Rec* __restrict__ rec,
+                                                          bool wantNN, bool wantVal, int mode, SampleConsts k)
 {
     Accessor<GT> acc(g);
     const uint32_t m = *count;
@@ -201,20 +214,38 @@ __global__ __launch_bounds__(kThreads) void k_sample_list(GridView g, const floa
         const uint32_t id = list[j];
         float p[3];
         loadPoint(g, xyz, id, world, p);
-        sampleOne(acc, p, id, nn, val, grad, mode, k);
+        const SampleOut r = sampleEval(acc, p, wantNN, wantVal, mode, k);
+        storeRec(rec, id, r.nn, r.val, r.g);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec, uint32_t n, float* __restrict__ nn,
+                                                     float* __restrict__ val, float* __restrict__ grad)
+{
+    const uint32_t stride = gridDim.x * kThreads;
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+        const float4 a = rec[i].a;
+        if (nn) nn[i] = a.x;
+        if (val) val[i] = a.y;
+        if (grad) {
+            const float gz = rec[i].b.x;
+            grad[3 * uint64_t(i)] = a.z;
+            grad[3 * uint64_t(i) + 1] = a.w;
+            grad[3 * uint64_t(i) + 2] = gz;
+        }
     }
 }
 
 template<uint32_t GT>
 void launchSample(const GridView& g, const BinLayout& L, const float* xyz, int world, const uint32_t* sorted,
-                  const uint32_t* begin, const uint32_t* end, float* nn, float* val, float* grad, int mode,
+                  const uint32_t* begin, const uint32_t* end, Rec* rec, bool wantNN, bool wantVal, int mode,
                   uint32_t* spill, uint32_t* spillCount, hipStream_t s)
 {
     const SampleConsts k = sampleConsts(g);
-    hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kThreads), 0, s, g, L, xyz, world, sorted, begin, end, nn,
-                       val, grad, mode, k, spill, spillCount);
-    hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, xyz, world, spill, spillCount, nn, val,
-                       grad, mode, k);
+    hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kThreads), 0, s, g, L, xyz, world, sorted, begin, end, rec,
+                       wantNN, wantVal, mode, k, spill, spillCount);
+    hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, xyz, world, spill, spillCount, rec,
+                       wantNN, wantVal, mode, k);
 }
 
 bool makeLayout(const GridView& g, BinLayout& L)
@@ -255,11 +286,12 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
                                            static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
                                            static_cast<uint32_t*>(nullptr), n, 0, int(L.bits), s) != hipSuccess)
         return -4;
-    // scratch: keys[2n] | idx[2n] | begin[m] | end[m] | spillCount | spill[n] | sort temp
+    // scratch: keys[2n] | idx[2n] | begin[m] | end[m] | spillCount | spill[n] | rec[n] | sort temp
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t oKeys = 0, oKeys2 = up(oKeys + 4ull * n), oIdx = up(oKeys2 + 4ull * n), oIdx2 = up(oIdx + 4ull * n),
                  oBegin = up(oIdx2 + 4ull * n), oEnd = up(oBegin + 4ull * m), oCount = up(oEnd + 4ull * m),
-                 oSpill = up(oCount + 4), oTemp = up(oSpill + 4ull * n), total = oTemp + up(sortBytes);
+                 oSpill = up(oCount + 4), oRec = up(oSpill + 4ull * n), oTemp = up(oRec + sizeof(Rec) * size_t(n)),
+                 total = oTemp + up(sortBytes);
     keepPoolMemory();
     char* base = nullptr;
     if (hipMallocAsync(reinterpret_cast<void**>(&base), total, s) != hipSuccess) return -2;
@@ -271,8 +303,11 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     uint32_t* end = reinterpret_cast<uint32_t*>(base + oEnd);
     uint32_t* spillCount = reinterpret_cast<uint32_t*>(base + oCount);
     uint32_t* spill = reinterpret_cast<uint32_t*>(base + oSpill);
+    Rec* rec = reinterpret_cast<Rec*>(base + oRec);
     void* temp = base + oTemp;
     int st = 0;
+    const bool wantNN = nn != nullptr, wantVal = val != nullptr;
+    const int gm = grad ? mode : 0;
     const uint32_t gridN = std::min<uint32_t>((n + kThreads - 1) / kThreads, 1u << 16);
     const uint32_t zeroWords = uint32_t(oCount / 4 - oBegin / 4 + 1); // begin[], end[] and spillCount
     hipLaunchKernelGGL(k_fill_u32, dim3((zeroWords + kThreads - 1) / kThreads), dim3(kThreads), 0, s, begin, zeroWords,
@@ -282,13 +317,16 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
         st = -4;
     hipLaunchKernelGGL(k_bin_bounds, dim3(gridN), dim3(kThreads), 0, s, keys2, n, begin, end);
     switch (g.type) {
-    case GT_FLOAT: launchSample<GT_FLOAT>(g, L, xyz, world, idx2, begin, end, nn, val, grad, mode, spill, spillCount, s); break;
-    case GT_FP4: launchSample<GT_FP4>(g, L, xyz, world, idx2, begin, end, nn, val, grad, mode, spill, spillCount, s); break;
-    case GT_FP8: launchSample<GT_FP8>(g, L, xyz, world, idx2, begin, end, nn, val, grad, mode, spill, spillCount, s); break;
-    case GT_FP16: launchSample<GT_FP16>(g, L, xyz, world, idx2, begin, end, nn, val, grad, mode, spill, spillCount, s); break;
-    case GT_FPN: launchSample<GT_FPN>(g, L, xyz, world, idx2, begin, end, nn, val, grad, mode, spill, spillCount, s); break;
+    case GT_FLOAT: launchSample<GT_FLOAT>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FP4: launchSample<GT_FP4>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FP8: launchSample<GT_FP8>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FP16: launchSample<GT_FP16>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FPN: launchSample<GT_FPN>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
     default: st = -3;
     }
+    if (st == 0)
+        hipLaunchKernelGGL(k_unpack, dim3(std::min<uint32_t>((n + kThreads - 1) / kThreads, 1u << 14)), dim3(kThreads), 0,
+                           s, rec, n, nn, val, grad);
     if (hipGetLastError() != hipSuccess) st = -4;
     if (hipFreeAsync(base, s) != hipSuccess) st = -5;
     return st;

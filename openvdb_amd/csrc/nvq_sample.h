@@ -37,25 +37,30 @@ __device__ __forceinline__ float fd6Sample(Accessor<GT>& acc, float qx, float qy
     return trilerp(c, qx - fx, qy - fy, qz - fz);
 }
 
-// Evaluate one query at index-space p; writes the requested outputs at index i.
+// Result of one point query; fields that were not asked for hold 0.
+struct SampleOut {
+    float nn, val, g[3];
+};
+
+// Evaluate one query at index-space p: the nearest value when wantNN, the trilinear value when
+// wantVal, and the gradient of the given mode (0 = none).
 template<uint32_t GT>
-__device__ __forceinline__ void sampleOne(Accessor<GT>& acc, const float p[3], uint64_t i, float* __restrict__ nn,
-                                          float* __restrict__ val, float* __restrict__ grad, int mode,
-                                          const SampleConsts& k)
+__device__ __forceinline__ SampleOut sampleEval(Accessor<GT>& acc, const float p[3], bool wantNN, bool wantVal, int mode,
+                                                const SampleConsts& k)
 {
+    SampleOut r{0.0f, 0.0f, {0.0f, 0.0f, 0.0f}};
     const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
     const int32_t i0 = int32_t(fx), j0 = int32_t(fy), k0 = int32_t(fz);
     const float u = p[0] - fx, v = p[1] - fy, w = p[2] - fz;
     Cell C;
     stencil8(acc, i0, j0, k0, C);
-    if (nn) {
+    if (wantNN) {
         // SampleFromVoxels<Acc,0>: getValue(Round(xyz)); Round uses rintf, which lands in this cell
         const int32_t ri = int32_t(rintf(p[0])) - i0, rj = int32_t(rintf(p[1])) - j0, rk = int32_t(rintf(p[2])) - k0;
-        nn[i] = ((ri | rj | rk) & ~1) ? acc.getValue(i0 + ri, j0 + rj, k0 + rk) : pick(C, ri, rj, rk);
+        r.nn = ((ri | rj | rk) & ~1) ? acc.getValue(i0 + ri, j0 + rj, k0 + rk) : pick(C, ri, rj, rk);
     }
-    if (val) val[i] = trilerp(C, u, v, w);
-    if (!grad) return;
-    float o[3] = {0.0f, 0.0f, 0.0f};
+    if (wantVal) r.val = trilerp(C, u, v, w);
+    float* o = r.g;
     if (mode == 1) {
         // FD6 of trilinear samples, python/cuda/PySampleFromVoxels.cu:42-47
         Cell B;
@@ -118,9 +123,23 @@ __device__ __forceinline__ void sampleOne(Accessor<GT>& acc, const float p[3], u
     } else if (mode == 3) {
         trilinearGradient(C, u, v, w, o);
     }
-    grad[3 * i] = o[0];
-    grad[3 * i + 1] = o[1];
-    grad[3 * i + 2] = o[2];
+    return r;
+}
+
+// Evaluate one query at index-space p; writes the requested outputs at index i.
+template<uint32_t GT>
+__device__ __forceinline__ void sampleOne(Accessor<GT>& acc, const float p[3], uint64_t i, float* __restrict__ nn,
+                                          float* __restrict__ val, float* __restrict__ grad, int mode,
+                                          const SampleConsts& k)
+{
+    const SampleOut r = sampleEval(acc, p, nn != nullptr, val != nullptr, grad ? mode : 0, k);
+    if (nn) nn[i] = r.nn;
+    if (val) val[i] = r.val;
+    if (grad) {
+        grad[3 * i] = r.g[0];
+        grad[3 * i + 1] = r.g[1];
+        grad[3 * i + 2] = r.g[2];
+    }
 }
 
 inline SampleConsts sampleConsts(const GridView& g)
