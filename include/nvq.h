@@ -5,8 +5,9 @@
  * "host_", every pointer argument is a DEVICE pointer.  A `stream` is a
  * hipStream_t passed as void* (NULL = the default stream).  Every call is
  * stream-ordered and asynchronous.  Every call returns NVQ_OK (0) or a negative
- * NVQ_ERR_* status.  Nothing throws across the ABI.  The library holds no
- * global mutable state: calls on distinct streams are thread-safe.  The caller
+ * NVQ_ERR_* status.  Nothing throws across the ABI.  Apart from the opt-in
+ * kernel profiler (nvq_profile_*), the library holds no global mutable state:
+ * calls on distinct streams are thread-safe.  The caller
  * owns every buffer and keeps the grid alive until work queued on it has
  * finished.
  *
@@ -141,6 +142,16 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
                       void* stream);
 /* out[i] = float(double(accum[i]) * 2^-fixed_shift) */
 int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream);
+
+/* Opt-in kernel timing for measurement tools (no counterpart in the reference).  While enabled,
+ * each call records HIP events on its stream around its dominant kernel: "k_sample" (direct
+ * point queries), "k_bin_sample" (binned point queries, n >= 2^20), "k_zero_crossing",
+ * "k_fog_march", "k_fog_march_bwd", "k_multi_query".  nvq_profile_read waits for the recorded
+ * launches and returns the summed milliseconds and launch count of one kernel name since the
+ * last reset.  Process-wide; thread-safe. */
+int nvq_profile_enable(int enable);
+int nvq_profile_read(const char* kernel, double* total_ms, uint64_t* launches);
+int nvq_profile_reset(void);
 
 #ifdef __cplusplus
 }

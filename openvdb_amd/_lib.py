@@ -48,6 +48,9 @@ SIGNATURES = {
     "nvq_multi_query": ([_vp, _u32, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march_bwd": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
     "nvq_grad_fixed_to_float": ([_vp, _u64, _i, _vp, _vp], _i),
+    "nvq_profile_enable": ([_i], _i),
+    "nvq_profile_read": ([ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_u64)], _i),
+    "nvq_profile_reset": ([], _i),
     # host builder (include/nvq_build.h)
     "nvqb_level_set_sphere": ([_d, _vp, _d, _d, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
     "nvqb_level_set_torus": ([_d, _d, _vp, _d, _d, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),

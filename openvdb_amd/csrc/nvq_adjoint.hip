@@ -19,6 +19,7 @@
 // depend on thread scheduling; the accumulator is converted through FP64.
 #include "nvq_device.h"
 #include "nvq_internal.h"
+#include "nvq_profile.h"
 #include "../../include/nvq.h"
 
 #include <hip/hip_runtime.h>
@@ -84,6 +85,7 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
 {
     const Sched sc = makeSched(n, width);
     const float scale = ldexpf(1.0f, shift);
+    ProfScope prof("k_fog_march_bwd", s);
     hipLaunchKernelGGL(k_fog_march_bwd, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
                        gT, reinterpret_cast<unsigned long long*>(accum), scale);
     return hipGetLastError() == hipSuccess ? 0 : -1;

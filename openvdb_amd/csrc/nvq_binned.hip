@@ -19,6 +19,7 @@
 //   k_unpack       : records -> the requested outputs, coalesced.
 #include "nvq_device.h"
 #include "nvq_internal.h"
+#include "nvq_profile.h"
 #include "nvq_sample.h"
 
 #include <hip/hip_runtime.h>
@@ -242,8 +243,11 @@ void launchSample(const GridView& g, const BinLayout& L, const float* xyz, int w
                   uint32_t* spill, uint32_t* spillCount, hipStream_t s)
 {
     const SampleConsts k = sampleConsts(g);
-    hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kThreads), 0, s, g, L, xyz, world, sorted, begin, end, rec,
-                       wantNN, wantVal, mode, k, spill, spillCount);
+    {
+        ProfScope prof("k_bin_sample", s);
+        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kThreads), 0, s, g, L, xyz, world, sorted, begin, end, rec,
+                           wantNN, wantVal, mode, k, spill, spillCount);
+    }
     hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, xyz, world, spill, spillCount, rec,
                        wantNN, wantVal, mode, k);
 }

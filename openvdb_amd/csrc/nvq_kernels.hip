@@ -7,6 +7,7 @@
 // tiles and keeps its own L2 warm with the nodes that band touches.
 #include "nvq_device.h"
 #include "nvq_internal.h"
+#include "nvq_profile.h"
 #include "nvq_sample.h"
 #include "nvq_rays.h"
 #include "../../include/nvq.h"
@@ -172,6 +173,7 @@ template<uint32_t GT> struct RunSample {
     static void run(const GridView& g, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                     int mode, hipStream_t s)
     {
+        ProfScope prof("k_sample", s);
         hipLaunchKernelGGL(k_sample<GT>, dim3(gridStrideBlocks(n)), dim3(kBlock), 0, s, g, xyz, n, world, nn, val, grad,
                            mode, sampleConsts(g));
     }
@@ -190,6 +192,7 @@ template<uint32_t GT> struct RunZeroCrossing {
     static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, uint8_t* hit, int32_t* ijk,
                     float* t, float* v, float* nrm, hipStream_t s)
     {
+        ProfScope prof("k_zero_crossing", s);
         hipLaunchKernelGGL(k_zero_crossing<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, hit, ijk, t,
                            v, nrm);
     }
@@ -198,6 +201,7 @@ template<uint32_t GT> struct RunFogMarch {
     static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, float dt, float sigma,
                     float* L, float* T, hipStream_t s)
     {
+        ProfScope prof("k_fog_march", s);
         hipLaunchKernelGGL(k_fog_march<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T);
     }
 };

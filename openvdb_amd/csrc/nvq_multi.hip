@@ -6,6 +6,7 @@
 // grids run zeroCrossing, fog volumes run the emission-absorption march.
 #include "nvq_device.h"
 #include "nvq_internal.h"
+#include "nvq_profile.h"
 #include "nvq_rays.h"
 
 #include <hip/hip_runtime.h>
@@ -72,6 +73,7 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     uint32_t blocks = uint32_t(cus) * 8u;
     if (uint64_t(blocks) * (kBlock / 64) > waves) blocks = uint32_t((waves + (kBlock / 64) - 1) / (kBlock / 64));
     if (blocks == 0) blocks = 1;
+    ProfScope prof("k_multi_query", s);
     hipLaunchKernelGGL(k_multi_query, dim3(blocks), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter, hit, ijk, a,
                        b, normal);
     const bool ok = hipGetLastError() == hipSuccess;
