@@ -131,6 +131,8 @@ __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L
                                                          uint32_t* __restrict__ spillCount)
 {
     __shared__ float brick[kBrickVox];
+    __shared__ const uint8_t* slotLeaf[64]; // leaf node of each 8^3 slot the brick overlaps, or null
+    __shared__ float slotValue[64];         // the slot's tile or background value when it has no leaf
     const uint32_t b = blockIdx.x;
     const uint32_t q0 = begin[b], q1 = end[b];
     if (q0 == q1) return;
@@ -138,14 +140,28 @@ __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L
                    bx = b / (uint32_t(L.nb[2]) * uint32_t(L.nb[1]));
     const int32_t o[3] = {L.lo[0] + (int32_t(bx) << kBinLog2) - 1, L.lo[1] + (int32_t(by) << kBinLog2) - 1,
                           L.lo[2] + (int32_t(bz) << kBinLog2) - 1};
-    {
-        // one z-row of the brick per thread at a time: consecutive voxels stay in one leaf
+    // The brick overlaps at most 4x4x4 aligned 8^3 slots; inside one slot every voxel is read from
+    // one leaf or, without a leaf, equals one tile or background value.  One tree walk per slot
+    // (64 lanes in parallel), then every brick voxel is an independent load (ReadAccessor::getValue
+    // semantics, NanoVDB.h:5370-5377).
+    const int32_t s0[3] = {o[0] >> 3, o[1] >> 3, o[2] >> 3};
+    const int32_t ns[3] = {((o[0] + kBrick - 1) >> 3) - s0[0] + 1, ((o[1] + kBrick - 1) >> 3) - s0[1] + 1,
+                           ((o[2] + kBrick - 1) >> 3) - s0[2] + 1};
+    if (threadIdx.x < uint32_t(ns[0] * ns[1] * ns[2])) {
+        const int32_t t = int32_t(threadIdx.x);
+        const int32_t sx = t / (ns[1] * ns[2]), sy = (t / ns[2]) % ns[1], sz = t % ns[2];
         Accessor<GT> acc(g);
-        for (int row = threadIdx.x; row < kBrick * kBrick; row += kThreads) {
-            const int32_t x = row / kBrick, y = row % kBrick;
-#pragma unroll 4
-            for (int z = 0; z < kBrick; ++z) brick[row * kBrick + z] = acc.getValue(o[0] + x, o[1] + y, o[2] + z);
-        }
+        const Probe pr = acc.probe((s0[0] + sx) << 3, (s0[1] + sy) << 3, (s0[2] + sz) << 3);
+        slotLeaf[(sx * 4 + sy) * 4 + sz] = pr.leaf;
+        slotValue[(sx * 4 + sy) * 4 + sz] = pr.value;
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < kBrickVox; v += kThreads) {
+        const int32_t x = v / (kBrick * kBrick), y = (v / kBrick) % kBrick, z = v % kBrick;
+        const int32_t gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
+        const int slot = (((gx >> 3) - s0[0]) * 4 + ((gy >> 3) - s0[1])) * 4 + ((gz >> 3) - s0[2]);
+        const uint8_t* leaf = slotLeaf[slot];
+        brick[v] = leaf ? Codec<GT>::get(leaf, Accessor<GT>::leafOffset(gx, gy, gz)) : slotValue[slot];
     }
     __syncthreads();
     for (uint32_t q = q0 + threadIdx.x; q < q1; q += kThreads) {
