@@ -32,6 +32,7 @@ namespace nvq {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kBinThreads = 1024;       // one bin per workgroup: 16 waves share the 32 KB brick
 constexpr int kBinLog2 = 4;
 constexpr int kBin = 1 << kBinLog2;     // bin edge in voxels
 constexpr int kBrick = kBin + 4;        // brick covers [bin-1, bin+kBin+2]
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* __restr
 }
 
 template<uint32_t GT>
-__global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L, const float* __restrict__ xyz,
+__global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayout L, const float* __restrict__ xyz,
                                                          int world, const uint32_t* __restrict__ sorted,
                                                          const uint32_t* __restrict__ begin,
                                                          const uint32_t* __restrict__ end, Rec* __restrict__ rec,
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L
         slotValue[(sx * 4 + sy) * 4 + sz] = pr.value;
     }
     __syncthreads();
-    for (int v = threadIdx.x; v < kBrickVox; v += kThreads) {
+    for (int v = threadIdx.x; v < kBrickVox; v += kBinThreads) {
         const int32_t x = v / (kBrick * kBrick), y = (v / kBrick) % kBrick, z = v % kBrick;
         const int32_t gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
         const int slot = (((gx >> 3) - s0[0]) * 4 + ((gy >> 3) - s0[1])) * 4 + ((gz >> 3) - s0[2]);
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_sample(GridView g, BinLayout L
         brick[v] = leaf ? Codec<GT>::get(leaf, Accessor<GT>::leafOffset(gx, gy, gz)) : slotValue[slot];
     }
     __syncthreads();
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kThreads) {
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBinThreads) {
         const uint32_t id = sorted[q];
         float p[3];
         loadPoint(g, xyz, id, world, p);
@@ -261,7 +262,7 @@ void launchSample(const GridView& g, const BinLayout& L, const float* xyz, int w
     const SampleConsts k = sampleConsts(g);
     {
         ProfScope prof("k_bin_sample", s);
-        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kThreads), 0, s, g, L, xyz, world, sorted, begin, end, rec,
+        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kBinThreads), 0, s, g, L, xyz, world, sorted, begin, end, rec,
                            wantNN, wantVal, mode, k, spill, spillCount);
     }
     hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, xyz, world, spill, spillCount, rec,
