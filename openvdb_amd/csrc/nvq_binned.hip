@@ -5,8 +5,11 @@
 // the last-level cache.  For large batches the queries are first sorted into
 // bins of 16^3 voxels (over the grid's active bounding box grown by 2 voxels;
 // points outside are clamped into the border bins):
-//   k_bin_keys     : bin key per point (after the FP64 world->index map when asked)
-//   radix sort     : (key, point index) pairs, rocPRIM/hipCUB DeviceRadixSort
+//   k_bin_keys     : bin key per point and the point itself {index-space xyz (after the FP64
+//                    world->index map when asked), original index}
+//   radix sort     : (key, point) pairs, rocPRIM/hipCUB DeviceRadixSort; the points travel
+//                    with their keys, so the per-bin pass reads them coalesced instead of
+//                    gathering 12 bytes per 128-byte cache line
 //   k_bin_bounds   : [begin, end) of every non-empty bin in the sorted order
 //   k_bin_sample   : one workgroup per bin stages the 20^3 brick of voxel values
 //                    around the bin in LDS (ReadAccessor::getValue semantics, so
@@ -44,6 +47,11 @@ struct BinLayout {
     int32_t nb[3];   // bins per axis
     uint32_t nbins;
     uint32_t bits;   // key bits for the radix sort
+};
+
+struct alignas(16) Pt {
+    float x, y, z; // index space
+    uint32_t id;   // original position in the batch
 };
 
 struct alignas(32) Rec {
@@ -100,14 +108,19 @@ __device__ __forceinline__ uint32_t binOf(const BinLayout& L, const float p[3])
 
 __global__ __launch_bounds__(kThreads) void k_bin_keys(GridView g, BinLayout L, const float* __restrict__ xyz,
                                                        uint32_t n, int world, uint32_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ idx)
+                                                       Pt* __restrict__ pts)
 {
     const uint32_t stride = gridDim.x * kThreads;
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
         float p[3];
         loadPoint(g, xyz, i, world, p);
         keys[i] = binOf(L, p);
-        idx[i] = i;
+        Pt t;
+        t.x = p[0];
+        t.y = p[1];
+        t.z = p[2];
+        t.id = i;
+        pts[i] = t;
     }
 }
 
@@ -123,8 +136,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* __restr
 }
 
 template<uint32_t GT>
-__global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayout L, const float* __restrict__ xyz,
-                                                         int world, const uint32_t* __restrict__ sorted,
+__global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayout L, const Pt* __restrict__ sorted,
                                                          const uint32_t* __restrict__ begin,
                                                          const uint32_t* __restrict__ end, Rec* __restrict__ rec,
                                                          bool wantNN, bool wantVal, int mode,
@@ -166,9 +178,9 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
     }
     __syncthreads();
     for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBinThreads) {
-        const uint32_t id = sorted[q];
-        float p[3];
-        loadPoint(g, xyz, id, world, p);
+        const Pt t = sorted[q];
+        const uint32_t id = t.id;
+        const float p[3] = {t.x, t.y, t.z};
         const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
         const int32_t ci = int32_t(fx) - o[0], cj = int32_t(fy) - o[1], ck = int32_t(fz) - o[2];
         // every FD6 sample cell lies in [cell-1, cell+2] (p +- 1 rounds at most one voxel further up),
@@ -255,14 +267,14 @@ __global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec
 }
 
 template<uint32_t GT>
-void launchSample(const GridView& g, const BinLayout& L, const float* xyz, int world, const uint32_t* sorted,
+void launchSample(const GridView& g, const BinLayout& L, const float* xyz, int world, const Pt* sorted,
                   const uint32_t* begin, const uint32_t* end, Rec* rec, bool wantNN, bool wantVal, int mode,
                   uint32_t* spill, uint32_t* spillCount, hipStream_t s)
 {
     const SampleConsts k = sampleConsts(g);
     {
         ProfScope prof("k_bin_sample", s);
-        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kBinThreads), 0, s, g, L, xyz, world, sorted, begin, end, rec,
+        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kBinThreads), 0, s, g, L, sorted, begin, end, rec,
                            wantNN, wantVal, mode, k, spill, spillCount);
     }
     hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, xyz, world, spill, spillCount, rec,
@@ -304,13 +316,13 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     const uint32_t n = uint32_t(n64), m = L.nbins;
     size_t sortBytes = 0;
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, static_cast<uint32_t*>(nullptr),
-                                           static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                                           static_cast<uint32_t*>(nullptr), n, 0, int(L.bits), s) != hipSuccess)
+                                           static_cast<uint32_t*>(nullptr), static_cast<Pt*>(nullptr),
+                                           static_cast<Pt*>(nullptr), n, 0, int(L.bits), s) != hipSuccess)
         return -4;
-    // scratch: keys[2n] | idx[2n] | begin[m] | end[m] | spillCount | spill[n] | rec[n] | sort temp
+    // scratch: keys[2n] | pts[2n] | begin[m] | end[m] | spillCount | spill[n] | rec[n] | sort temp
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t oKeys = 0, oKeys2 = up(oKeys + 4ull * n), oIdx = up(oKeys2 + 4ull * n), oIdx2 = up(oIdx + 4ull * n),
-                 oBegin = up(oIdx2 + 4ull * n), oEnd = up(oBegin + 4ull * m), oCount = up(oEnd + 4ull * m),
+    const size_t oKeys = 0, oKeys2 = up(oKeys + 4ull * n), oIdx = up(oKeys2 + 4ull * n),
+                 oIdx2 = up(oIdx + sizeof(Pt) * size_t(n)), oBegin = up(oIdx2 + sizeof(Pt) * size_t(n)), oEnd = up(oBegin + 4ull * m), oCount = up(oEnd + 4ull * m),
                  oSpill = up(oCount + 4), oRec = up(oSpill + 4ull * n), oTemp = up(oRec + sizeof(Rec) * size_t(n)),
                  total = oTemp + up(sortBytes);
     keepPoolMemory();
@@ -318,8 +330,8 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     if (hipMallocAsync(reinterpret_cast<void**>(&base), total, s) != hipSuccess) return -2;
     uint32_t* keys = reinterpret_cast<uint32_t*>(base + oKeys);
     uint32_t* keys2 = reinterpret_cast<uint32_t*>(base + oKeys2);
-    uint32_t* idx = reinterpret_cast<uint32_t*>(base + oIdx);
-    uint32_t* idx2 = reinterpret_cast<uint32_t*>(base + oIdx2);
+    Pt* idx = reinterpret_cast<Pt*>(base + oIdx);
+    Pt* idx2 = reinterpret_cast<Pt*>(base + oIdx2);
     uint32_t* begin = reinterpret_cast<uint32_t*>(base + oBegin);
     uint32_t* end = reinterpret_cast<uint32_t*>(base + oEnd);
     uint32_t* spillCount = reinterpret_cast<uint32_t*>(base + oCount);
