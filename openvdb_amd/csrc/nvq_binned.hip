@@ -14,16 +14,12 @@
 //   k_bin_sample   : one workgroup per bin stages the 20^3 brick of voxel values
 //                    around the bin in LDS (ReadAccessor::getValue semantics, so
 //                    results are bit-identical) and evaluates the bin's queries
-//                    from LDS.  Results are written in sorted order as aligned
-//                    32-byte records {nn, val, grad xyz} (coalesced), and each
-//                    query's sorted position goes to pos[original index] (4-byte
-//                    writes into an array small enough to stay in the
-//                    memory-side cache).
+//                    from LDS; each query's results go to its original index as
+//                    one aligned 32-byte record {nn, val, grad xyz}, a whole memory
+//                    sector, instead of three partial-sector writes to the outputs.
 //   k_sample_list  : queries whose stencil leaves the brick (border bins) run
 //                    through the lane accessor, into the same records.
-//   k_unpack       : out[i] = record[pos[i]] for the requested outputs: the
-//                    permutation back to the caller's order is a gather, which
-//                    HBM serves at a higher rate than a scatter of whole records.
+//   k_unpack       : records -> the requested outputs, coalesced.
 #include "nvq_device.h"
 #include "nvq_internal.h"
 #include "nvq_profile.h"
@@ -63,12 +59,12 @@ struct alignas(32) Rec {
     float4 b; // grad z, unused
 };
 
-__device__ __forceinline__ void storeRec(Rec* __restrict__ rec, uint32_t q, float nn, float val, const float g[3])
+__device__ __forceinline__ void storeRec(Rec* __restrict__ rec, uint32_t id, float nn, float val, const float g[3])
 {
     Rec r;
     r.a = make_float4(nn, val, g[0], g[1]);
     r.b = make_float4(g[2], 0.0f, 0.0f, 0.0f);
-    rec[q] = r;
+    rec[id] = r;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t m, uint32_t v)
@@ -143,7 +139,6 @@ template<uint32_t GT>
 __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayout L, const Pt* __restrict__ sorted,
                                                          const uint32_t* __restrict__ begin,
                                                          const uint32_t* __restrict__ end, Rec* __restrict__ rec,
-                                                         uint32_t* __restrict__ pos,
                                                          bool wantNN, bool wantVal, int mode,
                                                          SampleConsts k, uint32_t* __restrict__ spill,
                                                          uint32_t* __restrict__ spillCount)
@@ -184,7 +179,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
     __syncthreads();
     for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBinThreads) {
         const Pt t = sorted[q];
-        pos[t.id] = q;
+        const uint32_t id = t.id;
         const float p[3] = {t.x, t.y, t.z};
         const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
         const int32_t ci = int32_t(fx) - o[0], cj = int32_t(fy) - o[1], ck = int32_t(fz) - o[2];
@@ -192,7 +187,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
         // so the whole stencil is inside the brick iff the cell is in [1, kBrick-4]
         if (uint32_t(ci - 1) > uint32_t(kBrick - 5) || uint32_t(cj - 1) > uint32_t(kBrick - 5) ||
             uint32_t(ck - 1) > uint32_t(kBrick - 5)) {
-            spill[atomicAdd(spillCount, 1u)] = q;
+            spill[atomicAdd(spillCount, 1u)] = id;
             continue;
         }
         auto cell = [&](float qx, float qy, float qz) -> float {
@@ -233,12 +228,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
                     for (int e = 0; e < 2; ++e) C.v[a][bb][e] = brick[c0 + (a * kBrick + bb) * kBrick + e];
             trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, g3);
         }
-        storeRec(rec, q, nnv, vv, g3);
+        storeRec(rec, id, nnv, vv, g3);
     }
 }
 
 template<uint32_t GT>
-__global__ __launch_bounds__(kThreads) void k_sample_list(GridView g, const Pt* __restrict__ sorted,
+__global__ __launch_bounds__(kThreads) void k_sample_list(GridView g, const float* __restrict__ xyz, int world,
                                                           const uint32_t* __restrict__ list,
                                                           const uint32_t* __restrict__ count, Rec* __restrict__ rec,
                                                           bool wantNN, bool wantVal, int mode, SampleConsts k)
@@ -246,26 +241,24 @@ __global__ __launch_bounds__(kThreads) void k_sample_list(GridView g, const Pt* 
     Accessor<GT> acc(g);
     const uint32_t m = *count;
     for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < m; j += gridDim.x * kThreads) {
-        const uint32_t q = list[j];
-        const Pt t = sorted[q];
-        const float p[3] = {t.x, t.y, t.z};
+        const uint32_t id = list[j];
+        float p[3];
+        loadPoint(g, xyz, id, world, p);
         const SampleOut r = sampleEval(acc, p, wantNN, wantVal, mode, k);
-        storeRec(rec, q, r.nn, r.val, r.g);
+        storeRec(rec, id, r.nn, r.val, r.g);
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec, const uint32_t* __restrict__ pos,
-                                                     uint32_t n, float* __restrict__ nn, float* __restrict__ val,
-                                                     float* __restrict__ grad)
+__global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec, uint32_t n, float* __restrict__ nn,
+                                                     float* __restrict__ val, float* __restrict__ grad)
 {
     const uint32_t stride = gridDim.x * kThreads;
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-        const uint32_t q = pos[i];
-        const float4 a = rec[q].a;
+        const float4 a = rec[i].a;
         if (nn) nn[i] = a.x;
         if (val) val[i] = a.y;
         if (grad) {
-            const float gz = rec[q].b.x;
+            const float gz = rec[i].b.x;
             grad[3 * uint64_t(i)] = a.z;
             grad[3 * uint64_t(i) + 1] = a.w;
             grad[3 * uint64_t(i) + 2] = gz;
@@ -274,18 +267,18 @@ __global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec
 }
 
 template<uint32_t GT>
-void launchSample(const GridView& g, const BinLayout& L, const Pt* sorted, const uint32_t* begin, const uint32_t* end,
-                  Rec* rec, uint32_t* pos, bool wantNN, bool wantVal, int mode, uint32_t* spill, uint32_t* spillCount,
-                  hipStream_t s)
+void launchSample(const GridView& g, const BinLayout& L, const float* xyz, int world, const Pt* sorted,
+                  const uint32_t* begin, const uint32_t* end, Rec* rec, bool wantNN, bool wantVal, int mode,
+                  uint32_t* spill, uint32_t* spillCount, hipStream_t s)
 {
     const SampleConsts k = sampleConsts(g);
     {
         ProfScope prof("k_bin_sample", s);
-        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kBinThreads), 0, s, g, L, sorted, begin, end, rec, pos,
+        hipLaunchKernelGGL(k_bin_sample<GT>, dim3(L.nbins), dim3(kBinThreads), 0, s, g, L, sorted, begin, end, rec,
                            wantNN, wantVal, mode, k, spill, spillCount);
     }
-    hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, sorted, spill, spillCount, rec, wantNN,
-                       wantVal, mode, k);
+    hipLaunchKernelGGL(k_sample_list<GT>, dim3(1024), dim3(kThreads), 0, s, g, xyz, world, spill, spillCount, rec,
+                       wantNN, wantVal, mode, k);
 }
 
 bool makeLayout(const GridView& g, BinLayout& L)
@@ -326,12 +319,11 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
                                            static_cast<uint32_t*>(nullptr), static_cast<Pt*>(nullptr),
                                            static_cast<Pt*>(nullptr), n, 0, int(L.bits), s) != hipSuccess)
         return -4;
-    // scratch: keys[2n] | pts[2n] | begin[m] | end[m] | spillCount | spill[n] | rec[n] | pos[n] | sort temp
+    // scratch: keys[2n] | pts[2n] | begin[m] | end[m] | spillCount | spill[n] | rec[n] | sort temp
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t oKeys = 0, oKeys2 = up(oKeys + 4ull * n), oIdx = up(oKeys2 + 4ull * n),
                  oIdx2 = up(oIdx + sizeof(Pt) * size_t(n)), oBegin = up(oIdx2 + sizeof(Pt) * size_t(n)), oEnd = up(oBegin + 4ull * m), oCount = up(oEnd + 4ull * m),
-                 oSpill = up(oCount + 4), oRec = up(oSpill + 4ull * n), oPos = up(oRec + sizeof(Rec) * size_t(n)),
-                 oTemp = up(oPos + 4ull * n),
+                 oSpill = up(oCount + 4), oRec = up(oSpill + 4ull * n), oTemp = up(oRec + sizeof(Rec) * size_t(n)),
                  total = oTemp + up(sortBytes);
     keepPoolMemory();
     char* base = nullptr;
@@ -345,7 +337,6 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     uint32_t* spillCount = reinterpret_cast<uint32_t*>(base + oCount);
     uint32_t* spill = reinterpret_cast<uint32_t*>(base + oSpill);
     Rec* rec = reinterpret_cast<Rec*>(base + oRec);
-    uint32_t* pos = reinterpret_cast<uint32_t*>(base + oPos);
     void* temp = base + oTemp;
     int st = 0;
     const bool wantNN = nn != nullptr, wantVal = val != nullptr;
@@ -359,16 +350,16 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
         st = -4;
     hipLaunchKernelGGL(k_bin_bounds, dim3(gridN), dim3(kThreads), 0, s, keys2, n, begin, end);
     switch (g.type) {
-    case GT_FLOAT: launchSample<GT_FLOAT>(g, L, idx2, begin, end, rec, pos, wantNN, wantVal, gm, spill, spillCount, s); break;
-    case GT_FP4: launchSample<GT_FP4>(g, L, idx2, begin, end, rec, pos, wantNN, wantVal, gm, spill, spillCount, s); break;
-    case GT_FP8: launchSample<GT_FP8>(g, L, idx2, begin, end, rec, pos, wantNN, wantVal, gm, spill, spillCount, s); break;
-    case GT_FP16: launchSample<GT_FP16>(g, L, idx2, begin, end, rec, pos, wantNN, wantVal, gm, spill, spillCount, s); break;
-    case GT_FPN: launchSample<GT_FPN>(g, L, idx2, begin, end, rec, pos, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FLOAT: launchSample<GT_FLOAT>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FP4: launchSample<GT_FP4>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FP8: launchSample<GT_FP8>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FP16: launchSample<GT_FP16>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_FPN: launchSample<GT_FPN>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
     default: st = -3;
     }
     if (st == 0)
         hipLaunchKernelGGL(k_unpack, dim3(std::min<uint32_t>((n + kThreads - 1) / kThreads, 1u << 14)), dim3(kThreads), 0,
-                           s, rec, pos, n, nn, val, grad);
+                           s, rec, n, nn, val, grad);
     if (hipGetLastError() != hipSuccess) st = -4;
     if (hipFreeAsync(base, s) != hipSuccess) st = -5;
     return st;
