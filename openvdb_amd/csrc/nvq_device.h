@@ -386,7 +386,9 @@ struct CellWindow {
             if constexpr (KEYS) {
                 const Probe p = acc.probe(ni + a, nj + b, nk + e);
                 c.v[a][b][e] = p.value;
-                key[s] = p.leaf ? int32_t(uint32_t((p.leaf - g.leaf0) / 2144u) * 512u + p.n) : -1;
+                // leaf index = byte offset / 2144 (Float leaf size) = (offset / 32) / 67, exact and in
+                // 32 bits for the <= 2^22 leaves the adjoint accepts
+                key[s] = p.leaf ? int32_t((uint32_t(uint64_t(p.leaf - g.leaf0) >> 5) / 67u) * 512u + p.n) : -1;
                 sum[s] = 0.0f;
             } else {
                 c.v[a][b][e] = acc.getValue(ni + a, nj + b, nk + e);
