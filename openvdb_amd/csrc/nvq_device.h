@@ -250,11 +250,16 @@ __device__ __forceinline__ void stencil8(AccT& acc, int32_t i, int32_t j, int32_
             for (int e = 0; e < 2; ++e) c.v[a][b][e] = acc.getValue(i + a, j + b, k + e);
 }
 
-// c.v[a][b][e] for a,b,e in {0,1} without dynamic register indexing
+// c.v[a][b][e] for a,b,e in {0,1} without dynamic register indexing.  The empty asm keeps the
+// eight corners as register values: otherwise the optimiser rewrites the selects of elements into
+// one load through a selected address, which forces the cell out of registers into memory.
 __device__ __forceinline__ float pick(const Cell& c, int32_t a, int32_t b, int32_t e)
 {
-    const float x0 = b ? (e ? c.v[0][1][1] : c.v[0][1][0]) : (e ? c.v[0][0][1] : c.v[0][0][0]);
-    const float x1 = b ? (e ? c.v[1][1][1] : c.v[1][1][0]) : (e ? c.v[1][0][1] : c.v[1][0][0]);
+    float v000 = c.v[0][0][0], v001 = c.v[0][0][1], v010 = c.v[0][1][0], v011 = c.v[0][1][1];
+    float v100 = c.v[1][0][0], v101 = c.v[1][0][1], v110 = c.v[1][1][0], v111 = c.v[1][1][1];
+    asm("" : "+v"(v000), "+v"(v001), "+v"(v010), "+v"(v011), "+v"(v100), "+v"(v101), "+v"(v110), "+v"(v111));
+    const float x0 = b ? (e ? v011 : v010) : (e ? v001 : v000);
+    const float x1 = b ? (e ? v111 : v110) : (e ? v101 : v100);
     return a ? x1 : x0;
 }
 
