@@ -11,7 +11,8 @@ before the timed region.  value = queries / second over the whole step.
 
 Also reported (same JSON line, "extra"): level-set ray hits (512x512 camera,
 sphere R=100) and the fog march (1024x1024 rays through the 512^3 fog) in rays/s;
-the differentiable march (256^3 fBm fog seed 7 against a seed-11 target, 64
+the iso-surface example (sphere R=100 at (-20,0,0), 4096^2 pixels, persistent and
+tiled launches) in rays/s; the differentiable march (256^3 fBm fog seed 7 against a seed-11 target, 64
 cameras on a radius-600 shell x 256^2 rays, L2 image loss) as the time ratio of
 forward+adjoint to forward; and the quantised batched query (16 level-set and fog
 grids of bbox 64^3..1024^3, each as Fp16 / Fp8 / FpN(1e-3), ~2^24 rays per codec
@@ -406,6 +407,26 @@ def main():
         extra["c3_mean_transmittance"] = float(T.mean().item())
         log(f"C1 {extra['c1_rays_per_s']/1e9:.2f} G rays/s ({extra['c1_hits']} hits); "
             f"C3 {extra['c3_rays_per_s']/1e6:.1f} M rays/s")
+        # iso-surface example (examples/ex_raytrace_iso_surface): createLevelSetSphere(100, (-20,0,0)),
+        # 4096^2 pixels, iso = voxel size, RenderOp camera; persistent and tiled launches
+        iso_np = nvq.tools.create_level_set_sphere(100.0, (-20.0, 0.0, 0.0), 1.0, 3.0)
+        iso_g = nvq.DeviceGrid(iso_np)
+        from openvdb_amd._lib import IsoCamera
+        cd = nvq.tools.iso_camera(iso_np, 4096, 4096)
+        cam = IsoCamera(cd["width"], cd["height"], cd["iso"], cd["voxel_size"], cd["wbbox_dim_z"], cd["tan_half_fov"],
+                        (ctypes.c_float * 3)(*cd["origin"]))
+        img = torch.empty(4096 * 4096, device="cuda")
+        for persistent, kname in ((1, "k_iso_render_persistent"), (0, "k_iso_render")):
+            def ci(p=persistent):
+                st = lib.nvq_render_iso_surface(iso_g.handle, ctypes.byref(cam), p, P(img), None, None,
+                                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+                if st != 0:
+                    raise RuntimeError(f"nvq_render_iso_surface failed: {st}")
+            wi, _ = timed_launches(torch, [ci], 10, 2)
+            key = "iso_render_persistent" if persistent else "iso_render_tiled"
+            extra[key] = {"rays_per_s": 4096 * 4096 * 10 / wi, "kernel_ms": kernel_ms(kname)[0] / 10}
+        log(f"iso-surface 4096^2: persistent {extra['iso_render_persistent']['rays_per_s']/1e9:.2f} G rays/s, "
+            f"tiled {extra['iso_render_tiled']['rays_per_s']/1e9:.2f} G rays/s")
         if not args.no_diff:
             extra["c4"] = diff_march(torch, nvq, 5)
             log(f"C4: fwd {extra['c4']['fwd_ms']:.2f} ms, fwd+bwd {extra['c4']['fwd_bwd_ms']:.2f} ms, "

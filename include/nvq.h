@@ -107,6 +107,33 @@ int nvq_grad_stencil(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float
 int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                       uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream);
 
+/* math::isoCrossing (math/HDDA.h:225-248) for world-space rays: as nvq_zero_crossing, with the
+ * crossing taken where value - iso changes sign; v is the value at the hit voxel minus iso.
+ * nvq_zero_crossing is this call with iso = 0 (bit for bit). */
+int nvq_iso_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                     float iso, uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream);
+
+/* Camera of the iso-surface renderer (RenderOp, examples/ex_raytrace_iso_surface/common.h:18-37).
+ * nvqb_iso_camera (nvq_build.h) fills it from a grid the way RenderOp's constructor does. */
+typedef struct nvq_iso_camera {
+    int32_t width, height; /* image size in pixels */
+    float iso;             /* iso value (RenderOp: the voxel size) */
+    float voxel_size;      /* mDx */
+    float wbbox_dim_z;     /* mWBBoxDimZ: twice the z extent of the world bbox */
+    float tan_half_fov;    /* tanf(45 / 2 degrees), evaluated on the host */
+    float origin[3];       /* mWBBoxCenter + (0, 0, mWBBoxDimZ) */
+} nvq_iso_camera;
+
+/* Render a width x height image (RenderOp::renderPixel, common.h:70-78): per pixel one ray
+ * (getIndexRay, common.h:130-145), isoCrossing, and composite (common.h:147-155): depth
+ * (t * voxel_size) / (2 * wbbox_dim_z) where hit, else a checkerboard of 1.0 / 0.5 in 128-pixel
+ * squares.  persistent != 0 uses a persistent launch whose wavefronts pull 8x8 pixel tiles from a
+ * device counter (renderIsoSurfacePersistentKernel, common.h:166-189); otherwise one thread per
+ * pixel in 8x8 tiles.  `camera` is a HOST pointer; image[width*height] (row-major), optional
+ * hit[width*height] and t[width*height] (index-space hit time, 0 on miss) are device pointers. */
+int nvq_render_iso_surface(const nvq_grid* grid, const nvq_iso_camera* camera, int persistent, float* image,
+                           uint8_t* hit, float* t, void* stream);
+
 /* Emission-absorption march of world-space rays through a Float fog volume.  HDDA over the
  * tree at leaf-node granularity skips inactive tiles and background (math/HDDA.h:28-179,
  * 450-536); inside leaves and active tiles the ray is sampled trilinearly at globally aligned
@@ -148,7 +175,8 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
  * point queries), "k_bin_sample" (binned point queries, n >= 2^20), "k_zero_crossing",
  * "k_fog_march", "k_fog_march_bwd", "k_multi_query".  nvq_profile_read waits for the recorded
  * launches and returns the summed milliseconds and launch count of one kernel name since the
- * last reset.  Process-wide; thread-safe. */
+ * last reset.  The iso-surface renderer records "k_iso_render" / "k_iso_render_persistent".
+ * Process-wide; thread-safe. */
 int nvq_profile_enable(int enable);
 int nvq_profile_read(const char* kernel, double* total_ms, uint64_t* launches);
 int nvq_profile_reset(void);

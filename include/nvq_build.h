@@ -16,6 +16,8 @@
 
 #include <stdint.h>
 
+#include "nvq.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -61,6 +63,12 @@ int nvqb_camera_rays(uint32_t width, uint32_t height, const float eye[3], const 
                      float fov_deg, float* out_eye, float* out_dir);
 
 void nvqb_free(void* p);
+
+/* The camera of the iso-surface example's RenderOp (examples/ex_raytrace_iso_surface/common.h:26-37)
+ * for a width x height image of a NanoVDB grid buffer: voxel_size = float(voxelSize[0]), iso =
+ * voxel_size, wbbox_dim_z = float(worldBBox.dim()[2]) * 2, origin = float(worldBBox.min() +
+ * worldBBox.dim() * 0.5) + (0, 0, wbbox_dim_z), tan_half_fov = tanf(45 / 2 degrees). */
+int nvqb_iso_camera(const void* grid_buf, uint64_t bytes, int32_t width, int32_t height, nvq_iso_camera* out);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,18 @@ _f = ctypes.c_float
 _d = ctypes.c_double
 
 
+class IsoCamera(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("iso", ctypes.c_float),
+        ("voxel_size", ctypes.c_float),
+        ("wbbox_dim_z", ctypes.c_float),
+        ("tan_half_fov", ctypes.c_float),
+        ("origin", ctypes.c_float * 3),
+    ]
+
+
 class GridInfo(ctypes.Structure):
     _fields_ = [
         ("grid_type", ctypes.c_uint32),
@@ -44,6 +56,8 @@ SIGNATURES = {
     "nvq_sample": ([_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _vp], _i),
     "nvq_grad_stencil": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_zero_crossing": ([_vp, _vp, _vp, _u64, _u32, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "nvq_iso_crossing": ([_vp, _vp, _vp, _u64, _u32, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "nvq_render_iso_surface": ([_vp, ctypes.POINTER(IsoCamera), _i, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp], _i),
     "nvq_multi_query": ([_vp, _u32, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march_bwd": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
@@ -59,6 +73,7 @@ SIGNATURES = {
     "nvqb_fbm_density": ([_vp, _u64, _u32, _d, _u64, _f, _f, _vp], _i),
     "nvqb_quantize": ([_vp, _u64, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
     "nvqb_camera_rays": ([_u32, _u32, _vp, _vp, _vp, _f, _vp, _vp], _i),
+    "nvqb_iso_camera": ([_vp, _u64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(IsoCamera)], _i),
     "nvqb_free": ([_vp], None),
 }
 

@@ -1266,4 +1266,29 @@ int nvqb_camera_rays(uint32_t width, uint32_t height, const float eye[3], const 
     return NVQ_OK;
 }
 
+int nvqb_iso_camera(const void* grid_buf, uint64_t bytes, int32_t width, int32_t height, nvq_iso_camera* out)
+{
+    if (!grid_buf || bytes < 672 || width <= 0 || height <= 0 || !out) return NVQ_ERR_INVALID_ARG;
+    const uint8_t* b = static_cast<const uint8_t*>(grid_buf);
+    if (std::memcmp(b, "NanoVDB", 7) != 0) return NVQ_ERR_BAD_MAGIC;
+    double lo[3], hi[3], vs[3]; // GridData::mWorldBBox at 560, mVoxelSize at 608 (NanoVDB.h:1956-1957)
+    std::memcpy(lo, b + 560, sizeof(lo));
+    std::memcpy(hi, b + 584, sizeof(hi));
+    std::memcpy(vs, b + 608, sizeof(vs));
+    // BBox<Vec3d>::dim(): max - min when min < max on every axis, else 0 (math/Math.h:1907-1910)
+    const bool valid = lo[0] < hi[0] && lo[1] < hi[1] && lo[2] < hi[2];
+    double dim[3];
+    for (int a = 0; a < 3; ++a) dim[a] = valid ? hi[a] - lo[a] : 0.0;
+    out->width = width;
+    out->height = height;
+    out->voxel_size = float(vs[0]);
+    out->iso = out->voxel_size;
+    out->wbbox_dim_z = float(dim[2]) * 2;
+    const float off[3] = {0.0f, 0.0f, out->wbbox_dim_z};
+    for (int a = 0; a < 3; ++a) out->origin[a] = float(lo[a] + 0.5 * dim[a]) + off[a];
+    const float fov = 45.f;
+    out->tan_half_fov = std::tan(fov / 2 * 3.14159265358979323846f / 180.f);
+    return NVQ_OK;
+}
+
 } // extern "C"

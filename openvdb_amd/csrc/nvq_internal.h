@@ -83,6 +83,15 @@ struct MultiArgs {
 int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
                int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
 
+// Iso-surface renderer camera (nvq_iso_camera of nvq.h, field for field).
+struct IsoCamera {
+    int32_t width, height;
+    float iso, voxelSize, wbboxDimZ, tanHalfFov;
+    float origin[3];
+};
+int renderIso(const GridView& g, const IsoCamera& c, int persistent, float* image, uint8_t* hit, float* t,
+              hipStream_t s);
+
 // Adjoint of the fog march into an int64 fixed-point accumulator (nvq_adjoint.hip).
 int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint64_t n, uint32_t width, float dt,
                      float sigma, const float* L, const float* T, const float* gL, const float* gT, int64_t* accum,

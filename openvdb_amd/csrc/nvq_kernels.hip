@@ -121,14 +121,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_stencil(GridView g, const int32
 // ------------------------------------------------------------------ zeroCrossing / fog march
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_zero_crossing(GridView g, const float* __restrict__ eye,
-                                                          const float* __restrict__ dir, Sched s,
+                                                          const float* __restrict__ dir, Sched s, float iso,
                                                           uint8_t* __restrict__ hit, int32_t* __restrict__ ijk,
                                                           float* __restrict__ tout, float* __restrict__ vout,
                                                           float* __restrict__ normal)
 {
     uint64_t i;
     if (!schedule(s, i)) return;
-    zeroCrossingRay<GT>(g, eye, dir, i, hit, ijk, tout, vout, normal);
+    zeroCrossingRay<GT>(g, eye, dir, i, iso, hit, ijk, tout, vout, normal);
 }
 
 template<uint32_t GT>
@@ -189,12 +189,12 @@ template<uint32_t GT> struct RunGradStencil {
     }
 };
 template<uint32_t GT> struct RunZeroCrossing {
-    static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, uint8_t* hit, int32_t* ijk,
-                    float* t, float* v, float* nrm, hipStream_t s)
+    static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, float iso, uint8_t* hit,
+                    int32_t* ijk, float* t, float* v, float* nrm, hipStream_t s)
     {
         ProfScope prof("k_zero_crossing", s);
-        hipLaunchKernelGGL(k_zero_crossing<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, hit, ijk, t,
-                           v, nrm);
+        hipLaunchKernelGGL(k_zero_crossing<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, iso, hit,
+                           ijk, t, v, nrm);
     }
 };
 template<uint32_t GT> struct RunFogMarch {
@@ -418,7 +418,31 @@ int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, 
     if (n == 0) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
     const Sched sc = makeSched(n, image_width);
-    return launchTyped<RunZeroCrossing>(grid->view.type, grid->view, eye, dir, sc, hit, ijk, t, v, normal, S(stream));
+    return launchTyped<RunZeroCrossing>(grid->view.type, grid->view, eye, dir, sc, 0.0f, hit, ijk, t, v, normal,
+                                        S(stream));
+}
+
+int nvq_iso_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
+                     float iso, uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream)
+{
+    if (!grid || (n && (!eye || !dir))) return NVQ_ERR_INVALID_ARG;
+    if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
+    if (n == 0) return NVQ_OK;
+    if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
+    const Sched sc = makeSched(n, image_width);
+    return launchTyped<RunZeroCrossing>(grid->view.type, grid->view, eye, dir, sc, iso, hit, ijk, t, v, normal,
+                                        S(stream));
+}
+
+int nvq_render_iso_surface(const nvq_grid* grid, const nvq_iso_camera* camera, int persistent, float* image,
+                           uint8_t* hit, float* t, void* stream)
+{
+    if (!grid || !camera || !image || camera->width <= 0 || camera->height <= 0) return NVQ_ERR_INVALID_ARG;
+    if (uint64_t(camera->width) * uint64_t(camera->height) >= (uint64_t(1) << 31)) return NVQ_ERR_INVALID_ARG;
+    static_assert(sizeof(nvq_iso_camera) == sizeof(IsoCamera), "camera layouts differ");
+    IsoCamera c;
+    std::memcpy(&c, camera, sizeof(c));
+    return renderIso(grid->view, c, persistent, image, hit, t, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
 }
 
 int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width, float dt,

@@ -19,7 +19,7 @@ __device__ __forceinline__ void runRay(const GridView& g, uint32_t op, const flo
                                        float dt, float sigma, uint8_t* hit, int32_t* ijk, float* a, float* b,
                                        float* normal)
 {
-    if (op == kOpZeroCrossing) zeroCrossingRay<GT>(g, eye, dir, i, hit, ijk, a, b, normal);
+    if (op == kOpZeroCrossing) zeroCrossingRay<GT>(g, eye, dir, i, 0.0f, hit, ijk, a, b, normal);
     else fogMarchRay<GT>(g, eye, dir, i, dt, sigma, a, b);
 }
 

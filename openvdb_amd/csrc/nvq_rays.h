@@ -4,56 +4,66 @@
 
 namespace nvq {
 
-// ------------------------------------------------------------------ zeroCrossing
-// math/HDDA.h:189-214 over the device accessor; normal from the trilinear gradient at ray(t),
-// Grid::indexToWorldGradF (Map::applyIJTF, math/Math.h:1758-1763), normalised.
+// ------------------------------------------------------------------ isoCrossing / zeroCrossing
+// math/HDDA.h:225-248 (isoCrossing) over the device accessor on an index-space ray.  zeroCrossing
+// (math/HDDA.h:189-214) is the same loop with iso = 0: value - 0.0f is the value, bit for bit.
+// Returns the first voxel whose value - iso changes sign against the value at the entry voxel.
+template<uint32_t GT>
+__device__ __forceinline__ bool isoCrossingCore(Accessor<GT>& acc, const GridView& g, Ray& ray, float iso,
+                                                int32_t c[3], float& v, float& t)
+{
+    if (!rayClip(ray, g.bbox) || double(ray.t1) > 1e20) return false;
+    const float Delta = 1.0001f;
+    float p[3];
+    rayAt(ray, ray.t0, p);
+    c[0] = ifloor(p[0]);
+    c[1] = ifloor(p[1]);
+    c[2] = ifloor(p[2]);
+    const Probe p0 = acc.probe(c[0], c[1], c[2]);
+    HDDA h;
+    h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
+    const float v0 = p0.value - iso;
+    while (h.stepOnce()) {
+        rayAt(ray, h.t0 + Delta, p);
+        c[0] = ifloor(p[0]);
+        c[1] = ifloor(p[1]);
+        c[2] = ifloor(p[2]);
+        const Probe q = acc.probe(c[0], c[1], c[2]);
+        h.update(ray, int32_t(q.dim));
+        if (h.dim > 1 || !q.active) continue;
+        while (h.stepOnce()) { // in the narrow band
+            const Probe r = acc.probe(h.vx, h.vy, h.vz);
+            if (!r.active) break;
+            v = r.value - iso;
+            if (v * v0 < 0) {
+                c[0] = h.vx;
+                c[1] = h.vy;
+                c[2] = h.vz;
+                t = h.t0;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Batched ray query: Ray::worldToIndexF (math/Ray.h:258-268), isoCrossing, and the normal from
+// the trilinear gradient at ray(t), Grid::indexToWorldGradF (Map::applyIJTF,
+// math/Math.h:1758-1763), normalised.  v is the value at the hit voxel minus iso.
 template<uint32_t GT>
 __device__ __forceinline__ void zeroCrossingRay(const GridView& g, const float* __restrict__ eye,
-                                                const float* __restrict__ dir, uint64_t i, uint8_t* __restrict__ hit,
-                                                int32_t* __restrict__ ijk, float* __restrict__ tout,
-                                                float* __restrict__ vout, float* __restrict__ normal)
+                                                const float* __restrict__ dir, uint64_t i, float iso,
+                                                uint8_t* __restrict__ hit, int32_t* __restrict__ ijk,
+                                                float* __restrict__ tout, float* __restrict__ vout,
+                                                float* __restrict__ normal)
 {
     Accessor<GT> acc(g);
     const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
     const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
     Ray ray = worldToIndexRay(g, e, d);
-    bool found = false;
     int32_t c[3] = {0, 0, 0};
     float v = 0.0f, t = 0.0f;
-    if (rayClip(ray, g.bbox) && !(double(ray.t1) > 1e20)) {
-        const float Delta = 1.0001f;
-        float p[3];
-        rayAt(ray, ray.t0, p);
-        c[0] = ifloor(p[0]);
-        c[1] = ifloor(p[1]);
-        c[2] = ifloor(p[2]);
-        const Probe p0 = acc.probe(c[0], c[1], c[2]);
-        HDDA h;
-        h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
-        const float v0 = p0.value;
-        while (!found && h.stepOnce()) {
-            rayAt(ray, h.t0 + Delta, p);
-            c[0] = ifloor(p[0]);
-            c[1] = ifloor(p[1]);
-            c[2] = ifloor(p[2]);
-            const Probe q = acc.probe(c[0], c[1], c[2]);
-            h.update(ray, int32_t(q.dim));
-            if (h.dim > 1 || !q.active) continue;
-            while (h.stepOnce()) { // in the narrow band
-                const Probe r = acc.probe(h.vx, h.vy, h.vz);
-                if (!r.active) break;
-                v = r.value;
-                if (v * v0 < 0) {
-                    c[0] = h.vx;
-                    c[1] = h.vy;
-                    c[2] = h.vz;
-                    t = h.t0;
-                    found = true;
-                    break;
-                }
-            }
-        }
-    }
+    const bool found = isoCrossingCore<GT>(acc, g, ray, iso, c, v, t);
     if (hit) hit[i] = found ? 1 : 0;
     if (ijk) {
         ijk[3 * i] = found ? c[0] : 0;

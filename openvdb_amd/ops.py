@@ -8,6 +8,8 @@ Each function mirrors one reference call shape, applied to a batch:
 * ``sample``           SampleFromVoxels<Acc,0|1> + gradient modes   (math/SampleFromVoxels.h)
 * ``grad_stencil``     GradStencil                                  (math/Stencils.h:455-576)
 * ``zero_crossing``    Ray::worldToIndexF + math::zeroCrossing      (math/HDDA.h:189-214)
+* ``iso_crossing``     Ray::worldToIndexF + math::isoCrossing       (math/HDDA.h:225-248)
+* ``render_iso_surface`` the iso-surface example's RenderOp (examples/ex_raytrace_iso_surface/common.h)
 * ``fog_march``        emission-absorption march with HDDA node skipping
 * ``multi_query``      one persistent launch over many grids (zeroCrossing / fog march per grid)
 
@@ -102,6 +104,36 @@ def zero_crossing(grid, eye, dir, image_width=0):
     check(lib().nvq_zero_crossing(grid.handle, _ptr(eye), _ptr(dir), n, image_width, _ptr(hit), _ptr(ijk), _ptr(t),
                                   _ptr(v), _ptr(nrm), _stream()), "nvq_zero_crossing")
     return hit, ijk, t, v, nrm
+
+
+def iso_crossing(grid, eye, dir, iso, image_width=0):
+    eye = _points(eye, torch.float32, "eye")
+    dir = _points(dir, torch.float32, "dir")
+    n = eye.shape[0]
+    dev = eye.device
+    hit = torch.empty(n, dtype=torch.uint8, device=dev)
+    ijk = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    v = torch.empty(n, dtype=torch.float32, device=dev)
+    nrm = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    check(lib().nvq_iso_crossing(grid.handle, _ptr(eye), _ptr(dir), n, image_width, float(iso), _ptr(hit), _ptr(ijk),
+                                 _ptr(t), _ptr(v), _ptr(nrm), _stream()), "nvq_iso_crossing")
+    return hit, ijk, t, v, nrm
+
+
+def render_iso_surface(grid, camera, persistent=True, device="cuda"):
+    """Render the iso-surface example's image for ``camera`` (a dict from ``tools.iso_camera``).
+    Returns (image [h, w], hit [h, w], t [h, w])."""
+    from ._lib import IsoCamera
+    w, h = int(camera["width"]), int(camera["height"])
+    c = IsoCamera(w, h, camera["iso"], camera["voxel_size"], camera["wbbox_dim_z"], camera["tan_half_fov"],
+                  (ctypes.c_float * 3)(*camera["origin"]))
+    image = torch.empty((h, w), dtype=torch.float32, device=device)
+    hit = torch.empty((h, w), dtype=torch.uint8, device=device)
+    t = torch.empty((h, w), dtype=torch.float32, device=device)
+    check(lib().nvq_render_iso_surface(grid.handle, ctypes.byref(c), int(bool(persistent)), _ptr(image), _ptr(hit),
+                                       _ptr(t), _stream()), "nvq_render_iso_surface")
+    return image, hit, t
 
 
 def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):

@@ -91,3 +91,15 @@ def camera_rays(width, height, eye, look_at=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
                                  e.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p)),
           "nvqb_camera_rays")
     return e, d
+
+
+def iso_camera(grid, width, height):
+    """The iso-surface example's camera for a host grid buffer (RenderOp's constructor,
+    examples/ex_raytrace_iso_surface/common.h:26-37) as a dict of nvq_iso_camera's fields."""
+    from ._lib import IsoCamera
+    buf = np.ascontiguousarray(grid)
+    cam = IsoCamera()
+    check(lib().nvqb_iso_camera(buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes, int(width), int(height),
+                                ctypes.byref(cam)), "nvqb_iso_camera")
+    return {"width": cam.width, "height": cam.height, "iso": cam.iso, "voxel_size": cam.voxel_size,
+            "wbbox_dim_z": cam.wbbox_dim_z, "tan_half_fov": cam.tan_half_fov, "origin": list(cam.origin)}

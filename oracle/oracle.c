@@ -560,7 +560,9 @@ void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int d
 }
 
 /* ------------------------------------------------------------------ zeroCrossing, math/HDDA.h:189-214 */
-static int zero_crossing(const G* g, Ray* ray, int32_t ijk[3], float* v, float* t)
+/* math::isoCrossing, math/HDDA.h:225-248 (zeroCrossing, :189-214, is the iso = 0 case:
+ * value - 0.0f is the value bit for bit) */
+static int iso_crossing(const G* g, Ray* ray, float iso, int32_t ijk[3], float* v, float* t)
 {
     if (!ray_clip(ray, g->bbox) || (double)ray->t1 > 1e20) return 0;
     const float Delta = 1.0001f;
@@ -569,7 +571,7 @@ static int zero_crossing(const G* g, Ray* ray, int32_t ijk[3], float* v, float* 
     int32_t c[3] = {ifloor(p[0]), ifloor(p[1]), ifloor(p[2])};
     HDDA h;
     hdda_init(&h, ray, ray->t0, ray->t1, (int)lookup(g, c[0], c[1], c[2]).dim);
-    const float v0 = get_value(g, c[0], c[1], c[2]);
+    const float v0 = get_value(g, c[0], c[1], c[2]) - iso;
     while (hdda_step(&h)) {
         ray_at(ray, h.t0 + Delta, p);
         c[0] = ifloor(p[0]);
@@ -581,7 +583,7 @@ static int zero_crossing(const G* g, Ray* ray, int32_t ijk[3], float* v, float* 
         while (hdda_step(&h)) {
             const Look lv = lookup(g, h.voxel[0], h.voxel[1], h.voxel[2]);
             if (!lv.active) break;
-            *v = lv.value;
+            *v = lv.value - iso;
             if (*v * v0 < 0) {
                 ijk[0] = h.voxel[0];
                 ijk[1] = h.voxel[1];
@@ -597,6 +599,12 @@ static int zero_crossing(const G* g, Ray* ray, int32_t ijk[3], float* v, float* 
 void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
                       float* t, float* v, float* normal)
 {
+    or_iso_crossing(buf, eye, dir, n, 0.0f, hit, ijk, t, v, normal);
+}
+
+void or_iso_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, uint8_t* hit,
+                     int32_t* ijk, float* t, float* v, float* normal)
+{
     const G g = open_grid(buf);
     const uint8_t* m = g.buf + GRID_MAP;
     float IM[9];
@@ -605,7 +613,7 @@ void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
         Ray r = world_to_index_ray(&g, eye + 3 * i, dir + 3 * i);
         int32_t c[3] = {0, 0, 0};
         float vv = 0.0f, tt = 0.0f;
-        const int h = zero_crossing(&g, &r, c, &vv, &tt);
+        const int h = iso_crossing(&g, &r, iso, c, &vv, &tt);
         if (hit) hit[i] = (uint8_t)h;
         if (ijk) for (int a = 0; a < 3; ++a) ijk[3 * i + a] = h ? c[a] : 0;
         if (t) t[i] = h ? tt : 0.0f;
@@ -628,6 +636,38 @@ void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
             }
             for (int a = 0; a < 3; ++a) normal[3 * i + a] = nr[a];
         }
+    }
+}
+
+/* ------------------------------------------------------------------ iso-surface renderer
+ * RenderOp of examples/ex_raytrace_iso_surface/common.h: getIndexRay (:130-145), isoCrossing,
+ * composite (:147-155).  cam = {width, height, iso, dx, wbboxDimZ, tanHalfFov, origin[3]}. */
+void or_iso_render(const void* buf, int32_t width, int32_t height, float iso, float dx, float wbbox_dim_z,
+                   float tan_half_fov, const float origin[3], float* image, uint8_t* hit, float* t)
+{
+    const G g = open_grid(buf);
+    for (uint64_t i = 0; i < (uint64_t)width * (uint64_t)height; ++i) {
+        const uint32_t x = (uint32_t)(i % (uint32_t)width), y = (uint32_t)(i / (uint32_t)width);
+        const float u = ((float)x + 0.5f) / (float)width;
+        const float v = ((float)y + 0.5f) / (float)height;
+        const float aspect = (float)width / (float)height;
+        const float px = (2.f * u - 1.f) * tan_half_fov * aspect;
+        const float py = (2.f * v - 1.f) * tan_half_fov;
+        const float pz = -1.f;
+        /* Vec3::normalize: *this *= 1 / length(), length = Sqrt(x*x + y*y + z*z), math/Math.h:1423-1473 */
+        const float inv = 1.0f / sqrtf(px * px + py * py + pz * pz);
+        const float d[3] = {px * inv, py * inv, pz * inv};
+        Ray r = world_to_index_ray(&g, origin, d);
+        int32_t c[3] = {0, 0, 0};
+        float vv = 0.0f, tt = 0.0f;
+        const int h = iso_crossing(&g, &r, iso, c, &vv, &tt);
+        const float value = h ? (tt * dx) / (wbbox_dim_z * 2) : 0.0f;
+        const float alpha = h ? 1.0f : 0.0f;
+        const uint32_t mask = 1u << 7;
+        const float bg = ((x & mask) ^ (y & mask)) ? 1.0f : 0.5f;
+        image[i] = alpha * value + (1.0f - alpha) * bg;
+        if (hit) hit[i] = (uint8_t)h;
+        if (t) t[i] = h ? tt : 0.0f;
     }
 }
 

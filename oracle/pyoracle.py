@@ -49,6 +49,8 @@ class Oracle:
         L.or_sample_f64.argtypes = [_vp, _vp, _u64, _vp, _vp]
         L.or_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
         L.or_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]
+        L.or_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _vp]
+        L.or_iso_render.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f, _f, _f, _f, _vp, _vp, _vp, _vp]
         L.or_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
         L.or_fog_march_adjoint.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
         L.or_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
@@ -107,6 +109,29 @@ class Oracle:
         nrm = np.empty((n, 3), np.float32)
         self.L.or_zero_crossing(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm))
         return hit, ijk, t, v, nrm
+
+    def iso_crossing(self, buf, eye, dir, iso):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        hit = np.empty(n, np.uint8)
+        ijk = np.empty((n, 3), np.int32)
+        t = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        nrm = np.empty((n, 3), np.float32)
+        self.L.or_iso_crossing(_p(buf), _p(eye), _p(dir), n, iso, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm))
+        return hit, ijk, t, v, nrm
+
+    def iso_render(self, buf, cam):
+        """cam: dict with width, height, iso, voxel_size, wbbox_dim_z, tan_half_fov, origin."""
+        w, h = int(cam["width"]), int(cam["height"])
+        image = np.empty(w * h, np.float32)
+        hit = np.empty(w * h, np.uint8)
+        t = np.empty(w * h, np.float32)
+        origin = np.ascontiguousarray(cam["origin"], np.float32)
+        self.L.or_iso_render(_p(buf), w, h, cam["iso"], cam["voxel_size"], cam["wbbox_dim_z"], cam["tan_half_fov"],
+                             _p(origin), _p(image), _p(hit), _p(t))
+        return image, hit, t
 
     def fog_march(self, buf, eye, dir, dt, sigma):
         eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
@@ -173,6 +198,7 @@ class Reference:
         L.ref_sample.argtypes = [_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _i]
         L.ref_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
         L.ref_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _i]
+        L.ref_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _i]
         L.ref_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp, _vp, _vp, _i]
         L.ref_ray_gen.argtypes = [_i, _i, _f, _f, _f, _f, _vp, _vp]
         L.ref_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
@@ -264,6 +290,17 @@ class Reference:
         nrm = np.empty((n, 3), np.float32)
         self.L.ref_zero_crossing(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm), nthreads)
         return hit, ijk, t, v, nrm
+
+    def iso_crossing(self, buf, eye, dir, iso, nthreads=1):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        hit = np.empty(n, np.uint8)
+        ijk = np.empty((n, 3), np.int32)
+        t = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        self.L.ref_iso_crossing(_p(buf), _p(eye), _p(dir), n, iso, _p(hit), _p(ijk), _p(t), _p(v), nthreads)
+        return hit, ijk, t, v
 
     def fog_march(self, buf, eye, dir, dt, sigma, nthreads=1, fp32=True, fp64=True):
         eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)

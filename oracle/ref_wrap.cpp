@@ -367,6 +367,34 @@ int ref_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
     });
 }
 
+// math::isoCrossing (math/HDDA.h:225-248) on world-space rays mapped by Ray::worldToIndexF.
+int ref_iso_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, uint8_t* hit,
+                     int32_t* ijk, float* t, float* v, int nthreads)
+{
+    return dispatch(buf, [&](auto* grid) {
+        parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
+            auto acc = grid->getAccessor();
+            for (uint64_t i = b; i < e; ++i) {
+                math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
+                                      Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
+                math::Ray<float> iRay = wRay.worldToIndexF(*grid);
+                Coord c(0);
+                float vv = 0.0f, tt = 0.0f;
+                const bool h = math::isoCrossing(iRay, acc, c, vv, tt, iso);
+                if (hit) hit[i] = h ? 1 : 0;
+                if (ijk) {
+                    ijk[3 * i] = h ? c[0] : 0;
+                    ijk[3 * i + 1] = h ? c[1] : 0;
+                    ijk[3 * i + 2] = h ? c[2] : 0;
+                }
+                if (t) t[i] = h ? tt : 0.0f;
+                if (v) v[i] = h ? vv : 0.0f;
+            }
+        });
+        return 0;
+    });
+}
+
 // Emission-absorption fog march (contract described at fogMarch above).  FP32
 // outputs L,T use FP32 arithmetic (the CPU baseline); L64,T64 use FP64
 // arithmetic over the identical FP32 step structure (the parity oracle).
