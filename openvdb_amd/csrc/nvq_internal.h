@@ -83,6 +83,39 @@ struct MultiArgs {
 int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
                int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
 
+// ------------------------------------------------------------------ persistent work queue
+// Tiles [0, total) are split into 8 contiguous bands, one per XCD (workgroup b runs on XCD b % 8),
+// each with its own counter 256 bytes away from the next.  A wavefront drains its XCD's band in
+// chunks of kQueueChunk tiles, then steals from the other bands in turn; it returns once every
+// band's counter has passed its end, so every wavefront exits.  One counter per band keeps the
+// atomics off a single address, which serialises them.
+constexpr uint32_t kQueueBands = 8, kQueueStride = 64, kQueueChunk = 2; // stride in uint32 (256 B)
+constexpr size_t kQueueBytes = size_t(kQueueBands) * kQueueStride * sizeof(uint32_t);
+
+template<class F>
+__device__ __forceinline__ void forEachQueuedTile(uint32_t* __restrict__ counters, uint32_t total, F&& f)
+{
+    const uint32_t lane = threadIdx.x & 63u, home = blockIdx.x & (kQueueBands - 1);
+    const uint32_t per = (total + kQueueBands - 1) / kQueueBands;
+    for (uint32_t k = 0; k < kQueueBands; ++k) {
+        const uint32_t r = (home + k) & (kQueueBands - 1);
+        const uint32_t lo = r * per, hi = lo + per < total ? lo + per : total;
+        if (lo >= hi) continue;
+        for (;;) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(counters + r * kQueueStride, kQueueChunk);
+            b = __shfl(b, 0, 64);
+            if (b >= hi - lo) break; // band r is drained
+            const uint32_t t0 = lo + b, t1 = t0 + kQueueChunk < hi ? t0 + kQueueChunk : hi;
+            for (uint32_t t = t0; t < t1; ++t) f(t);
+        }
+    }
+}
+
+namespace {
+__global__ void k_queue_reset(uint32_t* counters) { counters[threadIdx.x * kQueueStride] = 0u; }
+} // namespace
+
 // Iso-surface renderer camera (nvq_iso_camera of nvq.h, field for field).
 struct IsoCamera {
     int32_t width, height;

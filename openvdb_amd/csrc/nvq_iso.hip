@@ -65,23 +65,17 @@ __global__ __launch_bounds__(kBlock) void k_iso_render(GridView g, IsoCamera c, 
 
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_iso_render_persistent(GridView g, IsoCamera c, uint32_t tilesX,
-                                                                  uint32_t tiles, uint32_t* __restrict__ counter,
+                                                                  uint32_t tiles, uint32_t* __restrict__ counters,
                                                                   float* __restrict__ image, uint8_t* __restrict__ hit,
                                                                   float* __restrict__ tout)
 {
     const uint32_t lane = threadIdx.x & 63u;
     Accessor<GT> acc(g); // kept across the pixels this lane renders
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1u);
-        t = __shfl(t, 0, 64);
-        if (t >= tiles) break; // every wavefront reaches this once the counter passes the last tile
+    forEachQueuedTile(counters, tiles, [&](uint32_t t) {
         const uint32_t x = (t % tilesX) * 8 + (lane & 7u), y = (t / tilesX) * 8 + (lane >> 3);
         if (x < uint32_t(c.width) && y < uint32_t(c.height)) renderPixel<GT>(acc, g, c, x, y, image, hit, tout);
-    }
+    });
 }
-
-__global__ void k_zero_counter(uint32_t* p) { *p = 0u; }
 
 template<uint32_t GT>
 int launchIso(const GridView& g, const IsoCamera& c, int persistent, float* image, uint8_t* hit, float* t,
@@ -99,8 +93,8 @@ int launchIso(const GridView& g, const IsoCamera& c, int persistent, float* imag
         return -1;
     const uint32_t tilesX = (uint32_t(c.width) + 7) / 8, tiles = tilesX * ((uint32_t(c.height) + 7) / 8);
     uint32_t* counter = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&counter), 256, s) != hipSuccess) return -2;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
+    if (hipMallocAsync(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
     uint32_t blocks = uint32_t(cus) * 8u;
     const uint32_t wavesNeeded = (tiles + (kBlock / 64) - 1) / (kBlock / 64);
     if (blocks > wavesNeeded) blocks = wavesNeeded;

@@ -1,8 +1,9 @@
 // Persistent multi-grid ray query: one launch answers the rays of up to kMaxGrids grids of
-// mixed leaf codecs and classes.  Wavefronts pull 8x8 ray tiles from one device work counter
-// until every grid's tiles are taken (the persistent-warp pattern of the reference's
-// iso-surface renderer, examples/ex_raytrace_iso_surface/common.h:79-110, 166-189), so small
-// and large grids share the machine without per-grid launches or tail effects.  Level-set
+// mixed leaf codecs and classes.  Wavefronts pull 8x8 ray tiles from a device work queue
+// (forEachQueuedTile: one counter per XCD band, with stealing) until every grid's tiles are
+// taken (the persistent-warp pattern of the reference's iso-surface renderer,
+// examples/ex_raytrace_iso_surface/common.h:79-110, 166-189), so small and large grids share
+// the machine without per-grid launches or tail effects.  Level-set
 // grids run zeroCrossing, fog volumes run the emission-absorption march.
 #include "nvq_device.h"
 #include "nvq_internal.h"
@@ -25,16 +26,12 @@ __device__ __forceinline__ void runRay(const GridView& g, uint32_t op, const flo
 
 __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float* __restrict__ eye,
                                                         const float* __restrict__ dir, float dt, float sigma,
-                                                        uint32_t* __restrict__ counter, uint8_t* __restrict__ hit,
+                                                        uint32_t* __restrict__ counters, uint8_t* __restrict__ hit,
                                                         int32_t* __restrict__ ijk, float* __restrict__ a,
                                                         float* __restrict__ b, float* __restrict__ normal)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1u);
-        t = __shfl(t, 0, 64);
-        if (t >= A.totalTiles) break; // every wave reaches this once the counter passes the last tile
+    forEachQueuedTile(counters, A.totalTiles, [&](uint32_t t) {
         uint32_t j = 0;
         while (j + 1 < A.k && A.slot[j + 1].tileBase <= t) ++j;
         const MultiSlot& S = A.slot[j];
@@ -53,10 +50,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float
             default: runRay<GT_FPN>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
             }
         }
-    }
+    });
 }
-
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
 } // namespace
 
@@ -67,8 +62,8 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return -1;
     uint32_t* counter = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&counter), 256, s) != hipSuccess) return -2;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
+    if (hipMallocAsync(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
     const uint64_t waves = A.totalTiles; // one wave per tile is the most that can be busy
     uint32_t blocks = uint32_t(cus) * 8u;
     if (uint64_t(blocks) * (kBlock / 64) > waves) blocks = uint32_t((waves + (kBlock / 64) - 1) / (kBlock / 64));
