@@ -84,11 +84,13 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
                int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
 
 // ------------------------------------------------------------------ persistent work queue
-// Tiles [0, total) are split into 8 contiguous bands, one per XCD (workgroup b runs on XCD b % 8),
-// each with its own counter 256 bytes away from the next.  A wavefront drains its XCD's band in
-// chunks of kQueueChunk tiles, then steals from the other bands in turn; it returns once every
-// band's counter has passed its end, so every wavefront exits.  One counter per band keeps the
-// atomics off a single address, which serialises them.
+// Tiles [0, total) are handed out in chunks of kQueueChunk through 8 counters, one per XCD
+// (workgroup b runs on XCD b % 8), each 256 bytes from the next: the i-th grab on counter r
+// yields chunk 8*i + r.  All XCDs therefore advance through the tile order together (the
+// wavefronts in flight share one region of the grids) while the atomics are spread over 8
+// addresses instead of serialising on one.  A wavefront starts on its XCD's counter and moves to
+// the next once that counter's chunks pass the end; it returns after all 8, so every wavefront
+// exits.
 constexpr uint32_t kQueueBands = 8, kQueueStride = 64, kQueueChunk = 2; // stride in uint32 (256 B)
 constexpr size_t kQueueBytes = size_t(kQueueBands) * kQueueStride * sizeof(uint32_t);
 
@@ -96,17 +98,16 @@ template<class F>
 __device__ __forceinline__ void forEachQueuedTile(uint32_t* __restrict__ counters, uint32_t total, F&& f)
 {
     const uint32_t lane = threadIdx.x & 63u, home = blockIdx.x & (kQueueBands - 1);
-    const uint32_t per = (total + kQueueBands - 1) / kQueueBands;
+    const uint32_t chunks = (total + kQueueChunk - 1) / kQueueChunk;
     for (uint32_t k = 0; k < kQueueBands; ++k) {
         const uint32_t r = (home + k) & (kQueueBands - 1);
-        const uint32_t lo = r * per, hi = lo + per < total ? lo + per : total;
-        if (lo >= hi) continue;
         for (;;) {
             uint32_t b = 0;
-            if (lane == 0) b = atomicAdd(counters + r * kQueueStride, kQueueChunk);
+            if (lane == 0) b = atomicAdd(counters + r * kQueueStride, 1u);
             b = __shfl(b, 0, 64);
-            if (b >= hi - lo) break; // band r is drained
-            const uint32_t t0 = lo + b, t1 = t0 + kQueueChunk < hi ? t0 + kQueueChunk : hi;
+            const uint64_t q = uint64_t(b) * kQueueBands + r;
+            if (q >= chunks) break; // counter r has handed out all its chunks
+            const uint32_t t0 = uint32_t(q) * kQueueChunk, t1 = t0 + kQueueChunk < total ? t0 + kQueueChunk : total;
             for (uint32_t t = t0; t < t1; ++t) f(t);
         }
     }

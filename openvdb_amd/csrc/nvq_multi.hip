@@ -1,6 +1,6 @@
 // Persistent multi-grid ray query: one launch answers the rays of up to kMaxGrids grids of
 // mixed leaf codecs and classes.  Wavefronts pull 8x8 ray tiles from a device work queue
-// (forEachQueuedTile: one counter per XCD band, with stealing) until every grid's tiles are
+// (forEachQueuedTile: chunks interleaved over one counter per XCD) until every grid's tiles are
 // taken (the persistent-warp pattern of the reference's iso-surface renderer,
 // examples/ex_raytrace_iso_surface/common.h:79-110, 166-189), so small and large grids share
 // the machine without per-grid launches or tail effects.  Level-set
