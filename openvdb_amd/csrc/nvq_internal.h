@@ -91,7 +91,7 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
 // addresses instead of serialising on one.  A wavefront starts on its XCD's counter and moves to
 // the next once that counter's chunks pass the end; it returns after all 8, so every wavefront
 // exits.
-constexpr uint32_t kQueueBands = 8, kQueueStride = 64, kQueueChunk = 2; // stride in uint32 (256 B)
+constexpr uint32_t kQueueBands = 8, kQueueStride = 64, kQueueChunk = 1; // stride in uint32 (256 B)
 constexpr size_t kQueueBytes = size_t(kQueueBands) * kQueueStride * sizeof(uint32_t);
 
 template<class F>
