@@ -70,6 +70,21 @@ void nvqb_free(void* p);
  * worldBBox.dim() * 0.5) + (0, 0, wbbox_dim_z), tan_half_fov = tanf(45 / 2 degrees). */
 int nvqb_iso_camera(const void* grid_buf, uint64_t bytes, int32_t width, int32_t height, nvq_iso_camera* out);
 
+/* Read grid number `index` (name == NULL) or the first grid called `name` from a file: either a
+ * NanoVDB file of segments (io::readGrid, nanovdb/nanovdb/io/IO.h:489-600; codec NONE only, a
+ * ZIP or BLOSC segment gives NVQ_ERR_UNSUPPORTED_TYPE) or a raw file of grid buffers
+ * (GridHandle::write).  As io::readGrid does, the returned grid is numbered 0 of 1
+ * (tools::updateGridCount, tools/GridChecksum.h:412-420, header CRC refreshed when the grid
+ * carries a checksum).  Missing grid: NVQ_ERR_INVALID_ARG. */
+int nvqb_read_nvdb(const char* path, int32_t index, const char* name, void** out, uint64_t* bytes);
+
+/* Write one grid buffer as a single-segment NanoVDB file, byte for byte what
+ * io::writeGrid(path, handle, io::Codec::NONE) writes for a one-grid handle (io/IO.h:444-467). */
+int nvqb_write_nvdb(const char* path, const void* grid_buf, uint64_t bytes);
+
+/* CRC-32 (reflected, polynomial 0xEDB88320) of size bytes, as NanoVDB's checksums use it. */
+uint32_t nvqb_crc32(const void* data, uint64_t size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,9 @@ SIGNATURES = {
     "nvqb_quantize": ([_vp, _u64, _u32, _f, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
     "nvqb_camera_rays": ([_u32, _u32, _vp, _vp, _vp, _f, _vp, _vp], _i),
     "nvqb_iso_camera": ([_vp, _u64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(IsoCamera)], _i),
+    "nvqb_read_nvdb": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(_u64)], _i),
+    "nvqb_write_nvdb": ([ctypes.c_char_p, _vp, _u64], _i),
+    "nvqb_crc32": ([_vp, _u64], _u32),
     "nvqb_free": ([_vp], None),
 }
 

@@ -103,3 +103,20 @@ def iso_camera(grid, width, height):
                                 ctypes.byref(cam)), "nvqb_iso_camera")
     return {"width": cam.width, "height": cam.height, "iso": cam.iso, "voxel_size": cam.voxel_size,
             "wbbox_dim_z": cam.wbbox_dim_z, "tan_half_fov": cam.tan_half_fov, "origin": list(cam.origin)}
+
+
+def read_nvdb(path, index=0, name=None):
+    """Grid `index` (or the first grid called `name`) of a NanoVDB file or raw grid file, as a
+    uint8 host buffer ready for ``DeviceGrid`` (io::readGrid semantics; uncompressed files)."""
+    ptr = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    check(lib().nvqb_read_nvdb(str(path).encode(), int(index), None if name is None else name.encode(),
+                               ctypes.byref(ptr), ctypes.byref(n)), "nvqb_read_nvdb")
+    return _take(ptr, n)
+
+
+def write_nvdb(path, grid):
+    """Write one grid buffer as an uncompressed NanoVDB file (io::writeGrid with Codec::NONE)."""
+    buf = np.ascontiguousarray(grid)
+    check(lib().nvqb_write_nvdb(str(path).encode(), buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes),
+          "nvqb_write_nvdb")

@@ -199,6 +199,9 @@ class Reference:
         L.ref_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
         L.ref_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _i]
         L.ref_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _i]
+        L.ref_write_nvdb.argtypes = [ctypes.c_char_p, _vp, _vp, _i, _i, _i]
+        L.ref_read_nvdb.argtypes = [ctypes.c_char_p, _i, ctypes.c_char_p, ctypes.POINTER(_u64)]
+        L.ref_read_nvdb.restype = _vp
         L.ref_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp, _vp, _vp, _i]
         L.ref_ray_gen.argtypes = [_i, _i, _f, _f, _f, _f, _vp, _vp]
         L.ref_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
@@ -301,6 +304,21 @@ class Reference:
         v = np.empty(n, np.float32)
         self.L.ref_iso_crossing(_p(buf), _p(eye), _p(dir), n, iso, _p(hit), _p(ijk), _p(t), _p(v), nthreads)
         return hit, ijk, t, v
+
+    def write_nvdb(self, path, bufs, merge=False, raw=False):
+        bufs = [np.ascontiguousarray(b) for b in bufs]
+        ptrs = (ctypes.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+        sizes = (ctypes.c_uint64 * len(bufs))(*[b.nbytes for b in bufs])
+        st = self.L.ref_write_nvdb(str(path).encode(), ptrs, sizes, len(bufs), int(merge), int(raw))
+        if st != 0:
+            raise RuntimeError(f"ref_write_nvdb failed ({st})")
+
+    def read_nvdb(self, path, index=0, name=None):
+        n = ctypes.c_uint64()
+        p = self.L.ref_read_nvdb(str(path).encode(), int(index), None if name is None else name.encode(), ctypes.byref(n))
+        if not p:
+            raise RuntimeError("ref_read_nvdb failed")
+        return self._take(p, n.value)
 
     def fog_march(self, buf, eye, dir, dt, sigma, nthreads=1, fp32=True, fp64=True):
         eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)

@@ -26,6 +26,7 @@
 #include <nanovdb/tools/CreatePrimitives.h>
 #include <nanovdb/tools/CreateNanoGrid.h>
 #include <nanovdb/tools/GridBuilder.h>
+#include <nanovdb/io/IO.h>
 #include "common.h" // examples/ex_raytrace_level_set/common.h (RayGenOp)
 
 #include <atomic>
@@ -393,6 +394,49 @@ int ref_iso_crossing(const void* buf, const float* eye, const float* dir, uint64
         });
         return 0;
     });
+}
+
+// .nvdb files through the reference's io:: functions (codec NONE).  count grid buffers are copied
+// into handles; merge != 0 first merges them into one multi-grid handle (mergeGrids,
+// GridHandle.h:507-544); raw != 0 writes the handle's raw buffer (GridHandle::write) instead of a
+// file with segments (io::writeGrids, io/IO.h:475-486).
+int ref_write_nvdb(const char* path, const void* const* bufs, const uint64_t* sizes, int count, int merge, int raw)
+{
+    try {
+        std::vector<GridHandle<HostBuffer>> hs;
+        for (int i = 0; i < count; ++i) {
+            HostBuffer b = HostBuffer::create(sizes[i]);
+            std::memcpy(b.data(), bufs[i], sizes[i]);
+            hs.emplace_back(std::move(b));
+        }
+        if (merge) {
+            GridHandle<HostBuffer> m = mergeGrids(hs);
+            hs.clear();
+            hs.push_back(std::move(m));
+        }
+        if (raw) {
+            if (hs.size() != 1) return -2;
+            hs[0].write(std::string(path));
+        } else {
+            io::writeGrids(std::string(path), hs, io::Codec::NONE);
+        }
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// io::readGrid(path, n) or io::readGrid(path, name) into a malloc'ed copy
+void* ref_read_nvdb(const char* path, int index, const char* name, uint64_t* bytes)
+{
+    try {
+        GridHandle<HostBuffer> h = name ? io::readGrid<HostBuffer>(std::string(path), std::string(name))
+                                        : io::readGrid<HostBuffer>(std::string(path), index);
+        return copyOut(h, bytes);
+    } catch (...) {
+        *bytes = 0;
+        return nullptr;
+    }
 }
 
 // Emission-absorption fog march (contract described at fogMarch above).  FP32
