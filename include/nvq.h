@@ -34,15 +34,17 @@ enum nvq_status {
     NVQ_ERR_INVALID_ARG = -1,     /* NULL pointer, n too large, bad mode */
     NVQ_ERR_BAD_MAGIC = -2,       /* GridData::isValid magic check, NanoVDB.h:2002-2013 */
     NVQ_ERR_BAD_VERSION = -3,     /* major version != 32, NanoVDB.h:729-769 */
-    NVQ_ERR_UNSUPPORTED_TYPE = -4, /* grid type not Float/Fp4/Fp8/Fp16/FpN, or op needs Float */
+    NVQ_ERR_UNSUPPORTED_TYPE = -4, /* grid type not Float/Fp4/Fp8/Fp16/FpN/OnIndex, or op needs Float */
     NVQ_ERR_MISALIGNED = -5,      /* buffer not 32-byte aligned, NanoVDB.h:69-74 */
     NVQ_ERR_TRUNCATED = -6,       /* buffer shorter than GridData::mGridSize / node offsets out of range */
     NVQ_ERR_HIP = -7,             /* a HIP runtime call failed */
     NVQ_ERR_NO_DEVICE = -8        /* no usable gfx950 device */
 };
 
-/* NanoVDB GridType values the kernels decode (NanoVDB.h:219-247) */
-enum nvq_grid_type { NVQ_FLOAT = 1, NVQ_FP4 = 13, NVQ_FP8 = 14, NVQ_FP16 = 15, NVQ_FPN = 16 };
+/* NanoVDB GridType values the kernels decode (NanoVDB.h:219-247).  NVQ_ONINDEX grids (ValueOnIndex,
+ * NanoVDB.h:4147-4171) hold indices; their values come from a float channel, the grid's own blind
+ * data #0 or an array given with nvq_grid_set_channel (ChannelAccessor, NanoVDB.h:5724-5800). */
+enum nvq_grid_type { NVQ_FLOAT = 1, NVQ_FP4 = 13, NVQ_FP8 = 14, NVQ_FP16 = 15, NVQ_FPN = 16, NVQ_ONINDEX = 20 };
 
 typedef struct nvq_grid nvq_grid; /* opaque device-resident grid */
 
@@ -72,6 +74,12 @@ int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid
  * `bytes`, 32-byte aligned) instead of allocating; nvq_grid_free then leaves that memory alone.
  * Lets a framework own the buffer, e.g. to expose the leaf values as a trainable tensor. */
 int nvq_grid_upload_into(const void* host_buf, uint64_t bytes, void* device_dst, void* stream, nvq_grid** out);
+/* Index grids: read values from `channel` (a DEVICE array of at least `count` >= the grid's value
+ * count floats, e.g. a trainable tensor) instead of the grid's own channel; NULL restores the
+ * grid's own channel.  ChannelAccessor(grid, channelPtr), NanoVDB.h:5750-5757.  Waits for the
+ * device before switching; every query on an index grid without a channel returns
+ * NVQ_ERR_INVALID_ARG. */
+int nvq_grid_set_channel(nvq_grid* grid, const float* channel, uint64_t count);
 /* Free the grid handle and, for nvq_grid_upload, the device copy (waits for the device). */
 int nvq_grid_free(nvq_grid* grid);
 int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info);

@@ -51,6 +51,7 @@ SIGNATURES = {
     "nvq_grid_upload": ([_vp, _u64, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_grid_upload_into": ([_vp, _u64, _vp, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_grid_free": ([_vp], _i),
+    "nvq_grid_set_channel": ([_vp, _vp, _u64], _i),
     "nvq_grid_get_info": ([_vp, ctypes.POINTER(GridInfo)], _i),
     "nvq_get_value": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp], _i),
     "nvq_sample": ([_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _vp], _i),

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
         const int32_t gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
         const int slot = (((gx >> 3) - s0[0]) * 4 + ((gy >> 3) - s0[1])) * 4 + ((gz >> 3) - s0[2]);
         const uint8_t* leaf = slotLeaf[slot];
-        brick[v] = leaf ? Codec<GT>::get(leaf, Accessor<GT>::leafOffset(gx, gy, gz)) : slotValue[slot];
+        brick[v] = leaf ? Codec<GT>::get(leaf, Accessor<GT>::leafOffset(gx, gy, gz), g.channel) : slotValue[slot];
     }
     __syncthreads();
     for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBinThreads) {
@@ -355,6 +355,7 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     case GT_FP8: launchSample<GT_FP8>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
     case GT_FP16: launchSample<GT_FP16>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
     case GT_FPN: launchSample<GT_FPN>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
+    case GT_ONINDEX: launchSample<GT_ONINDEX>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;
     default: st = -3;
     }
     if (st == 0)

@@ -22,7 +22,13 @@ constexpr uint32_t kRootBBox = 0, kRootTableSize = 24, kRootBackground = 28, kRo
 constexpr uint32_t kUpFlags = 24, kUpVMask = 32, kUpCMask = 4128, kUpTable = 8256, kUpSize = 270400;
 constexpr uint32_t kLoFlags = 24, kLoVMask = 32, kLoCMask = 544, kLoTable = 1088, kLoSize = 33856;
 constexpr uint32_t kLeafFlags = 15, kLeafVMask = 16, kLeafMin = 80, kLeafQuantum = 84, kLeafData = 96;
-enum GridTypeId : uint32_t { GT_FLOAT = 1, GT_FP4 = 13, GT_FP8 = 14, GT_FP16 = 15, GT_FPN = 16 };
+enum GridTypeId : uint32_t { GT_FLOAT = 1, GT_FP4 = 13, GT_FP8 = 14, GT_FP16 = 15, GT_FPN = 16, GT_ONINDEX = 20 };
+// ValueOnIndex grids (NanoVDB.h:4147-4171): the root is 96 B (uint64 background index at 32),
+// root tiles carry a uint64 index at 24, node tables hold uint64 indices, and a leaf holds an
+// offset (80) and a prefix sum of 9 bits per mask word (88).  Values live in a float channel.
+constexpr uint32_t kIdxRootSize = 96, kIdxRootBackground = 32, kIdxTileValue = 24, kIdxLeafOffset = 80,
+                   kIdxLeafPrefix = 88;
+__host__ __device__ constexpr uint32_t rootHeaderSize(uint32_t gt) { return gt == GT_ONINDEX ? kIdxRootSize : kRootSize; }
 
 // Host FLT_MAX: the reference's Maximum<float> on the host (math/Math.h:118-124);
 // the reference's own HIP branch would use 1e38 (:112-115), which would change
@@ -33,6 +39,7 @@ constexpr float kFltMax = 3.40282346638528859812e+38F;
 struct GridView {
     const uint8_t* root;  // RootData
     const uint8_t* leaf0; // first leaf node
+    const float* channel; // index grids: values by index (blind channel 0 or a caller array)
     uint32_t tableSize;
     uint32_t type;
     float background;
@@ -61,32 +68,59 @@ __device__ __forceinline__ bool maskOn(const uint8_t* mask, uint32_t n)
 // Quantised values are float(code)*quantum + minimum with no fused multiply-add.
 template<uint32_t GT> struct Codec;
 template<> struct Codec<GT_FLOAT> {
-    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n) { return ld_f32(leaf + kLeafData + 4 * n); }
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n, const float*)
+    {
+        return ld_f32(leaf + kLeafData + 4 * n);
+    }
 };
 __device__ __forceinline__ float dequant(const uint8_t* leaf, uint32_t code)
 {
     return __fadd_rn(__fmul_rn(float(code), ld_f32(leaf + kLeafQuantum)), ld_f32(leaf + kLeafMin));
 }
 template<> struct Codec<GT_FP4> {
-    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n)
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n, const float*)
     {
         return dequant(leaf, (ld_u8(leaf + kLeafData + (n >> 1)) >> ((n & 1) << 2)) & 15u);
     }
 };
 template<> struct Codec<GT_FP8> {
-    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n) { return dequant(leaf, ld_u8(leaf + kLeafData + n)); }
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n, const float*)
+    {
+        return dequant(leaf, ld_u8(leaf + kLeafData + n));
+    }
 };
 template<> struct Codec<GT_FP16> {
-    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n) { return dequant(leaf, ld_u16(leaf + kLeafData + 2 * n)); }
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n, const float*)
+    {
+        return dequant(leaf, ld_u16(leaf + kLeafData + 2 * n));
+    }
 };
 template<> struct Codec<GT_FPN> {
-    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n)
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n, const float*)
     {
         const uint32_t b = ld_u8(leaf + kLeafFlags) >> 5; // log2 of the bit width
         uint32_t code = ld_u32(leaf + kLeafData + 4 * (n >> (5 - b)));
         code >>= (n & ((32u >> b) - 1u)) << b;
         code &= (1u << (1u << b)) - 1u;
         return dequant(leaf, code);
+    }
+};
+// LeafData<ValueOnIndex>::getValue (NanoVDB.h:4160-4170): index 0 (the background) for an inactive
+// voxel, else offset + the active voxels before n (popcount within its mask word + the 9-bit
+// prefix sum of the words before); the value is channel[index] (ChannelAccessor, :5790-5792).
+template<> struct Codec<GT_ONINDEX> {
+    static __device__ __forceinline__ uint64_t index(const uint8_t* leaf, uint32_t n)
+    {
+        const uint32_t w = n >> 6;
+        const uint64_t word = ld_u64(leaf + kLeafVMask + 8 * w), bit = uint64_t(1) << (n & 63u);
+        if (!(word & bit)) return 0;
+        uint64_t sum = ld_u64(leaf + kIdxLeafOffset) + uint64_t(__popcll(word & (bit - 1u)));
+        if (w) sum += (ld_u64(leaf + kIdxLeafPrefix) >> (9u * (w - 1u))) & 511u;
+        return sum;
+    }
+    static __device__ __forceinline__ float get(const uint8_t* leaf, uint32_t n, const float* ch)
+    {
+        return ch[index(leaf, n)];
     }
 };
 
@@ -106,6 +140,7 @@ struct Probe {
 template<uint32_t GT>
 struct Accessor {
     const uint8_t* root;
+    const float* channel;
     uint32_t tableSize;
     float background;
     int32_t kx, ky, kz;
@@ -114,8 +149,16 @@ struct Accessor {
     const uint8_t* leaf;
 
     __device__ __forceinline__ explicit Accessor(const GridView& grid)
-        : root(grid.root), tableSize(grid.tableSize), background(grid.background), kx(0x7fffffff), ky(0x7fffffff),
-          kz(0x7fffffff), upper(nullptr), lower(nullptr), leaf(nullptr) {}
+        : root(grid.root), channel(grid.channel), tableSize(grid.tableSize),
+          background(GT == GT_ONINDEX ? grid.channel[ld_u64(grid.root + kIdxRootBackground)] : grid.background),
+          kx(0x7fffffff), ky(0x7fffffff), kz(0x7fffffff), upper(nullptr), lower(nullptr), leaf(nullptr) {}
+
+    // value of a node-table entry or root tile: a float in the low word, or an index into the channel
+    __device__ __forceinline__ float entryValue(uint64_t entry) const
+    {
+        if constexpr (GT == GT_ONINDEX) return channel[entry];
+        else return __uint_as_float(uint32_t(entry));
+    }
 
     __device__ __forceinline__ static uint32_t leafOffset(int32_t x, int32_t y, int32_t z)
     {
@@ -127,7 +170,7 @@ struct Accessor {
         const uint32_t n = leafOffset(x, y, z);
         r.leaf = leaf;
         r.n = n;
-        r.value = Codec<GT>::get(leaf, n);
+        r.value = Codec<GT>::get(leaf, n, channel);
         r.active = maskOn(leaf + kLeafVMask, n);
         r.dim = (ld_u8(leaf + kLeafFlags) & 1u) ? 8u : 1u;
     }
@@ -151,7 +194,7 @@ struct Accessor {
             leaf = nullptr;
             r.leaf = nullptr;
             r.n = 0;
-            r.value = __uint_as_float(uint32_t(entry));
+            r.value = entryValue(entry);
             r.active = maskOn(lower + kLoVMask, n);
             r.dim = (ld_u64(lower + kLoFlags) & 1u) ? 128u : 8u;
         }
@@ -171,7 +214,7 @@ struct Accessor {
             leaf = nullptr;
             r.leaf = nullptr;
             r.n = 0;
-            r.value = __uint_as_float(uint32_t(entry));
+            r.value = entryValue(entry);
             r.active = maskOn(upper + kUpVMask, n);
             r.dim = 128u;
         }
@@ -189,7 +232,7 @@ struct Accessor {
                              (uint64_t(uint32_t(x) >> 12) << 42);
         const uint8_t* tile = nullptr;
         for (uint32_t i = 0; i < tableSize; ++i) {
-            const uint8_t* t = root + kRootSize + kTileSize * i;
+            const uint8_t* t = root + rootHeaderSize(GT) + kTileSize * i;
             if (ld_u64(t) == key) { tile = t; break; }
         }
         upper = nullptr;
@@ -205,7 +248,7 @@ struct Accessor {
         }
         const int64_t child = ld_i64(tile + 8);
         if (child == 0) {
-            r.value = ld_f32(tile + 20);
+            r.value = GT == GT_ONINDEX ? channel[ld_u64(tile + kIdxTileValue)] : ld_f32(tile + 20);
             r.active = ld_u32(tile + 16) > 0u;
             return;
         }
@@ -228,7 +271,7 @@ struct Accessor {
     __device__ __forceinline__ float getValue(int32_t x, int32_t y, int32_t z)
     {
         const uint32_t dirty = uint32_t((x ^ kx) | (y ^ ky) | (z ^ kz));
-        if (leaf && !(dirty & ~7u)) return Codec<GT>::get(leaf, leafOffset(x, y, z));
+        if (leaf && !(dirty & ~7u)) return Codec<GT>::get(leaf, leafOffset(x, y, z), channel);
         return probe(x, y, z).value;
     }
 };

@@ -120,6 +120,7 @@ int renderIso(const GridView& g, const IsoCamera& c, int persistent, float* imag
     case GT_FP8: return launchIso<GT_FP8>(g, c, persistent, image, hit, t, s);
     case GT_FP16: return launchIso<GT_FP16>(g, c, persistent, image, hit, t, s);
     case GT_FPN: return launchIso<GT_FPN>(g, c, persistent, image, hit, t, s);
+    case GT_ONINDEX: return launchIso<GT_ONINDEX>(g, c, persistent, image, hit, t, s);
     default: return -5;
     }
 }

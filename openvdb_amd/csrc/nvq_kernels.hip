@@ -29,7 +29,13 @@ struct nvq_grid {
     bool owns; // false when the device memory belongs to the caller (nvq_grid_upload_into)
     GridView view;
     nvq_grid_info info;
+    const float* ownChannel; // index grids: the buffer's own float channel (blind data #0) or null
+    uint64_t valueCount;     // index grids: GridData::mData1
 };
+
+// A grid can be queried when its values are reachable: value grids always, index grids once they
+// have a float channel.
+static bool queryable(const nvq_grid* g) { return g->view.type != GT_ONINDEX || g->view.channel != nullptr; }
 
 namespace {
 
@@ -151,6 +157,7 @@ int launchTyped(uint32_t type, Args&&... args)
     case GT_FP8: K<GT_FP8>::run(args...); break;
     case GT_FP16: K<GT_FP16>::run(args...); break;
     case GT_FPN: K<GT_FPN>::run(args...); break;
+    case GT_ONINDEX: K<GT_ONINDEX>::run(args...); break;
     default: return NVQ_ERR_UNSUPPORTED_TYPE;
     }
     return hipGetLastError() == hipSuccess ? NVQ_OK : NVQ_ERR_HIP;
@@ -220,21 +227,38 @@ uint32_t leafSizeOf(uint32_t type)
     case GT_FP4: return 352u;
     case GT_FP8: return 608u;
     case GT_FP16: return 1120u;
+    case GT_ONINDEX: return 96u;
     default: return 0u;
     }
 }
 
 // Host-side validation of a NanoVDB buffer before upload (GridData::isValid, NanoVDB.h:2002-2013,
 // plus bounds checks of every node offset the kernels follow).
+// Byte offset of an index grid's float channel: blind data #0 when its type is Float and it holds
+// at least GridData::mData1 (the value count) floats inside the buffer, as ChannelAccessor(grid, 0)
+// takes it (NanoVDB.h:5737-5790, GridBlindMetaData :1591-1600); -1 when there is none.
+int64_t indexChannelOffset(const uint8_t* b, uint64_t bytes)
+{
+    const int64_t metaOff = rd<int64_t>(b + 640);
+    const uint32_t metaCount = rd<uint32_t>(b + 648);
+    if (metaCount == 0 || metaOff < 0 || uint64_t(metaOff) + 288u > bytes) return -1;
+    const uint8_t* m = b + metaOff;
+    if (rd<uint32_t>(m + 28) != GT_FLOAT || rd<uint32_t>(m + 16) != 4u) return -1;
+    const int64_t data = metaOff + rd<int64_t>(m);
+    const uint64_t count = rd<uint64_t>(m + 8), need = rd<uint64_t>(b + 656);
+    if (data < 0 || count < need || uint64_t(data) + 4u * count > bytes || (data & 3)) return -1;
+    return data;
+}
+
 int validate(const uint8_t* b, uint64_t bytes)
 {
-    if (!b || bytes < kGridSize + 64 + kRootSize) return b ? NVQ_ERR_TRUNCATED : NVQ_ERR_INVALID_ARG;
+    if (!b || bytes < kGridSize + 64 + kIdxRootSize) return b ? NVQ_ERR_TRUNCATED : NVQ_ERR_INVALID_ARG;
     if (reinterpret_cast<uintptr_t>(b) % 32u) return NVQ_ERR_MISALIGNED;
     const uint64_t magic = rd<uint64_t>(b) & 0x00FFFFFFFFFFFFFFull;
     if (magic != (0x314244566f6e614eull & 0x00FFFFFFFFFFFFFFull)) return NVQ_ERR_BAD_MAGIC;
     if ((rd<uint32_t>(b + 16) >> 21) != 32u) return NVQ_ERR_BAD_VERSION;
     const uint32_t type = rd<uint32_t>(b + kGridType);
-    if (type != GT_FLOAT && type != GT_FP4 && type != GT_FP8 && type != GT_FP16 && type != GT_FPN)
+    if (type != GT_FLOAT && type != GT_FP4 && type != GT_FP8 && type != GT_FP16 && type != GT_FPN && type != GT_ONINDEX)
         return NVQ_ERR_UNSUPPORTED_TYPE;
     const uint64_t gridSize = rd<uint64_t>(b + 32);
     if (gridSize > bytes) return NVQ_ERR_TRUNCATED;
@@ -246,9 +270,10 @@ int validate(const uint8_t* b, uint64_t bytes)
     auto inside = [&](int64_t off, uint64_t len) {
         return off >= 0 && base + uint64_t(off) + len <= bytes;
     };
-    if (!inside(offRoot, kRootSize)) return NVQ_ERR_TRUNCATED;
+    const uint32_t rootSize = rootHeaderSize(type);
+    if (!inside(offRoot, rootSize)) return NVQ_ERR_TRUNCATED;
     const uint32_t tableSize = rd<uint32_t>(tree + offRoot + kRootTableSize);
-    if (!inside(offRoot, kRootSize + uint64_t(kTileSize) * tableSize)) return NVQ_ERR_TRUNCATED;
+    if (!inside(offRoot, rootSize + uint64_t(kTileSize) * tableSize)) return NVQ_ERR_TRUNCATED;
     if (nUpper && !inside(offUpper, uint64_t(kUpSize) * nUpper)) return NVQ_ERR_TRUNCATED;
     if (nLower && !inside(offLower, uint64_t(kLoSize) * nLower)) return NVQ_ERR_TRUNCATED;
     const uint32_t ls = leafSizeOf(type);
@@ -326,6 +351,13 @@ static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, vo
     }
     v.leafCount = rd<uint32_t>(tree + 32);
     v.leafSize = leafSizeOf(v.type);
+    v.channel = nullptr;
+    if (v.type == GT_ONINDEX) {
+        const int64_t ch = indexChannelOffset(b, bytes);
+        if (ch >= 0) v.channel = reinterpret_cast<const float*>(dbase + ch);
+    }
+    g->ownChannel = v.channel;
+    g->valueCount = v.type == GT_ONINDEX ? rd<uint64_t>(b + 656) : 0;
     nvq_grid_info& in = g->info;
     std::memset(&in, 0, sizeof(in));
     in.grid_type = v.type;
@@ -369,6 +401,17 @@ int nvq_grid_free(nvq_grid* grid)
     return (e1 == hipSuccess && e2 == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
 }
 
+int nvq_grid_set_channel(nvq_grid* grid, const float* channel, uint64_t count)
+{
+    if (!grid || grid->view.type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
+    if (channel && count < grid->valueCount) return NVQ_ERR_INVALID_ARG;
+    // queued work may still read the current view: wait for the device, then swap
+    if (hipDeviceSynchronize() != hipSuccess) return NVQ_ERR_HIP;
+    grid->view.channel = channel ? channel : grid->ownChannel;
+    if (hipMemcpy(grid->dview, &grid->view, sizeof(GridView), hipMemcpyHostToDevice) != hipSuccess) return NVQ_ERR_HIP;
+    return NVQ_OK;
+}
+
 int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info)
 {
     if (!grid || !info) return NVQ_ERR_INVALID_ARG;
@@ -379,7 +422,7 @@ int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info)
 int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* val, uint8_t* active, uint32_t* dim,
                   void* stream)
 {
-    if (!grid || (n && !ijk)) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || (n && !ijk)) return NVQ_ERR_INVALID_ARG;
     if (n == 0) return NVQ_OK;
     return launchTyped<RunGetValue>(grid->view.type, grid->view, ijk, n, val, active, dim, S(stream));
 }
@@ -387,7 +430,7 @@ int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* v
 int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                int grad_mode, void* stream)
 {
-    if (!grid || (n && !xyz) || (grad && (grad_mode < 1 || grad_mode > 3))) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || (n && !xyz) || (grad && (grad_mode < 1 || grad_mode > 3))) return NVQ_ERR_INVALID_ARG;
     if (n == 0) return NVQ_OK;
     // Large batches go through spatial binning + LDS bricks (bit-identical results);
     // NVQ_SAMPLE_DIRECT=1 forces the lane-per-query kernel.
@@ -404,7 +447,7 @@ int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, fl
 int nvq_grad_stencil(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* grad, float* norm_sq, float* lap,
                      uint8_t* zero_crossing, void* stream)
 {
-    if (!grid || (n && !ijk)) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || (n && !ijk)) return NVQ_ERR_INVALID_ARG;
     if (grid->view.type != GT_FLOAT) return NVQ_ERR_UNSUPPORTED_TYPE;
     if (n == 0) return NVQ_OK;
     return launchTyped<RunGradStencil>(grid->view.type, grid->view, ijk, n, grad, norm_sq, lap, zero_crossing, S(stream));
@@ -413,7 +456,7 @@ int nvq_grad_stencil(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float
 int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                       uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream)
 {
-    if (!grid || (n && (!eye || !dir))) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || (n && (!eye || !dir))) return NVQ_ERR_INVALID_ARG;
     if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
     if (n == 0) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
@@ -425,7 +468,7 @@ int nvq_zero_crossing(const nvq_grid* grid, const float* eye, const float* dir, 
 int nvq_iso_crossing(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                      float iso, uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal, void* stream)
 {
-    if (!grid || (n && (!eye || !dir))) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || (n && (!eye || !dir))) return NVQ_ERR_INVALID_ARG;
     if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
     if (n == 0) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
@@ -437,7 +480,7 @@ int nvq_iso_crossing(const nvq_grid* grid, const float* eye, const float* dir, u
 int nvq_render_iso_surface(const nvq_grid* grid, const nvq_iso_camera* camera, int persistent, float* image,
                            uint8_t* hit, float* t, void* stream)
 {
-    if (!grid || !camera || !image || camera->width <= 0 || camera->height <= 0) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || !camera || !image || camera->width <= 0 || camera->height <= 0) return NVQ_ERR_INVALID_ARG;
     if (uint64_t(camera->width) * uint64_t(camera->height) >= (uint64_t(1) << 31)) return NVQ_ERR_INVALID_ARG;
     static_assert(sizeof(nvq_iso_camera) == sizeof(IsoCamera), "camera layouts differ");
     IsoCamera c;
@@ -448,7 +491,7 @@ int nvq_render_iso_surface(const nvq_grid* grid, const nvq_iso_camera* camera, i
 int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width, float dt,
                   float sigma_t, float* radiance, float* transmittance, void* stream)
 {
-    if (!grid || (n && (!eye || !dir)) || !(dt > 0.0f)) return NVQ_ERR_INVALID_ARG;
+    if (!grid || !queryable(grid) || (n && (!eye || !dir)) || !(dt > 0.0f)) return NVQ_ERR_INVALID_ARG;
     if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
     if (n == 0) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
@@ -494,7 +537,7 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
     for (uint32_t j = 0; j < k; ++j) {
         const nvq_grid* g = host_grids[j];
         const uint64_t n = host_ray_offsets[j + 1] - host_ray_offsets[j];
-        if (!g || host_ray_offsets[j + 1] < host_ray_offsets[j]) return NVQ_ERR_INVALID_ARG;
+        if (!g || !queryable(g) || host_ray_offsets[j + 1] < host_ray_offsets[j]) return NVQ_ERR_INVALID_ARG;
         const uint32_t given = host_image_width ? host_image_width[j] : 0u;
         if (given && n % given) return NVQ_ERR_INVALID_ARG;
         const uint32_t w = given ? given : 64u; // linear batches: rows of 64 rays

@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float
             case GT_FP4: runRay<GT_FP4>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
             case GT_FP8: runRay<GT_FP8>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
             case GT_FP16: runRay<GT_FP16>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
-            default: runRay<GT_FPN>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            case GT_FPN: runRay<GT_FPN>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
+            default: runRay<GT_ONINDEX>(g, S.op, eye, dir, i, dt, sigma, hit, ijk, a, b, normal); break;
             }
         }
     });

@@ -11,8 +11,8 @@ import numpy as np
 
 from ._lib import GridInfo, check, lib
 
-GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN"}
-GRID_CLASSES = {0: "Unknown", 1: "LevelSet", 2: "FogVolume"}
+GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN", 20: "OnIndex"}
+GRID_CLASSES = {0: "Unknown", 1: "LevelSet", 2: "FogVolume", 8: "IndexGrid"}
 
 
 def aligned_buffer(data):
@@ -65,6 +65,25 @@ class DeviceGrid:
         leaf_off = 672 + int(np.frombuffer(self._host[672:680].tobytes(), np.int64)[0])
         f32 = self.device_bytes.view(torch.float32)
         return torch.as_strided(f32, (self.leaf_count, 512), (2144 // 4, 1), (leaf_off + 96) // 4)
+
+    @property
+    def value_count(self):
+        """Index grids: number of values the channel holds (GridData::mData1)."""
+        return int(np.frombuffer(self._host[656:664].tobytes(), np.uint64)[0])
+
+    def set_channel(self, channel):
+        """Index grids: read values from ``channel`` (a float32 CUDA tensor of at least
+        ``value_count`` elements, e.g. a trainable sidecar) instead of the grid's own channel;
+        None restores the grid's own channel.  The tensor is kept alive while in use."""
+        if channel is None:
+            check(lib().nvq_grid_set_channel(self.handle, None, 0), "nvq_grid_set_channel")
+            self._channel = None
+            return
+        if not channel.is_cuda or channel.dtype.itemsize != 4 or not channel.is_contiguous():
+            raise ValueError("channel must be a contiguous float32 CUDA tensor")
+        check(lib().nvq_grid_set_channel(self.handle, ctypes.c_void_p(channel.data_ptr()), channel.numel()),
+              "nvq_grid_set_channel")
+        self._channel = channel
 
     @property
     def grid_type(self):

@@ -43,7 +43,12 @@ enum {
     LO_FLAGS = 24, LO_VMASK = 32, LO_CMASK = 544, LO_TABLE = 1088,
     LEAF_FLAGS = 15, LEAF_VMASK = 16, LEAF_MIN = 80, LEAF_QUANTUM = 84, LEAF_DATA = 96
 };
-enum { GT_FLOAT = 1, GT_FP4 = 13, GT_FP8 = 14, GT_FP16 = 15, GT_FPN = 16 };
+enum { GT_FLOAT = 1, GT_FP4 = 13, GT_FP8 = 14, GT_FP16 = 15, GT_FPN = 16, GT_ONINDEX = 20 };
+/* ValueOnIndex grids (NanoVDB.h:4147-4171): RootData is 96 B with a uint64 background index and
+ * tiles {key, child, state, uint64 index}; node tables hold uint64 indices; leaves hold an offset
+ * (80) and a 9-bit-per-word prefix sum (88). */
+enum { IDX_ROOT_SIZE = 96, IDX_ROOT_BACKGROUND = 32, IDX_TILE_VALUE = 24, IDX_LEAF_OFFSET = 80, IDX_LEAF_PREFIX = 88,
+       GRID_BLIND_OFFSET = 640, GRID_BLIND_COUNT = 648, BLIND_DATATYPE = 28 };
 static const uint64_t MAGIC_GRID = 0x314244566f6e614eULL; /* "NanoVDB1", NanoVDB.h:140 */
 
 static inline uint64_t rd_u64(const uint8_t* p) { uint64_t v; memcpy(&v, p, 8); return v; }
@@ -59,9 +64,10 @@ typedef struct {
     const uint8_t* tree;
     const uint8_t* root;
     const uint8_t* leaf0;
-    uint32_t type, leafCount, tableSize, leafSize;
+    uint32_t type, leafCount, tableSize, leafSize, rootSize;
     float background;
     int32_t bbox[6];
+    const float* channel; /* index grids: values by index (blind channel 0) */
 } G;
 
 static uint32_t leaf_size_of(uint32_t type)
@@ -71,6 +77,7 @@ static uint32_t leaf_size_of(uint32_t type)
     case GT_FP4: return 352u;
     case GT_FP8: return 608u;
     case GT_FP16: return 1120u;
+    case GT_ONINDEX: return 96u;
     default: return 0u; /* FpN: variable */
     }
 }
@@ -85,8 +92,21 @@ static G open_grid(const void* p)
     g.leaf0 = g.tree + rd_i64(g.tree + TREE_NODEOFF);
     g.leafCount = rd_u32(g.tree + TREE_NODECNT);
     g.tableSize = rd_u32(g.root + ROOT_TABLESIZE);
-    g.background = rd_f32(g.root + ROOT_BACKGROUND);
     g.leafSize = leaf_size_of(g.type);
+    g.channel = NULL;
+    g.rootSize = ROOT_SIZE;
+    if (g.type == GT_ONINDEX) {
+        /* ChannelAccessor(grid, 0): blind data #0 when it holds floats (NanoVDB.h:5737-5790,
+         * GridBlindMetaData :1591-1600, offsets relative to the metadata entry) */
+        if (rd_u32(g.buf + GRID_BLIND_COUNT) > 0) {
+            const uint8_t* m = g.buf + rd_i64(g.buf + GRID_BLIND_OFFSET);
+            if (rd_u32(m + BLIND_DATATYPE) == GT_FLOAT) g.channel = (const float*)(m + rd_i64(m));
+        }
+        g.rootSize = IDX_ROOT_SIZE;
+        g.background = g.channel ? g.channel[rd_u64(g.root + IDX_ROOT_BACKGROUND)] : 0.0f;
+    } else {
+        g.background = rd_f32(g.root + ROOT_BACKGROUND);
+    }
     for (int i = 0; i < 6; ++i) g.bbox[i] = rd_i32(g.root + ROOT_BBOX + 4 * i);
     return g;
 }
@@ -98,7 +118,7 @@ int or_validate(const void* buf, uint64_t bytes)
     if ((rd_u64(b + GRID_MAGIC) & 0x00FFFFFFFFFFFFFFULL) != (MAGIC_GRID & 0x00FFFFFFFFFFFFFFULL)) return -2;
     if ((rd_u32(b + GRID_VERSION) >> 21) != 32u) return -3; /* Version: major<<21 | minor<<10 | patch */
     const uint32_t t = rd_u32(b + GRID_TYPE);
-    if (t != GT_FLOAT && t != GT_FP4 && t != GT_FP8 && t != GT_FP16 && t != GT_FPN) return -4;
+    if (t != GT_FLOAT && t != GT_FP4 && t != GT_FP8 && t != GT_FP16 && t != GT_FPN && t != GT_ONINDEX) return -4;
     return 0;
 }
 uint32_t or_grid_type(const void* buf) { return rd_u32((const uint8_t*)buf + GRID_TYPE); }
@@ -123,9 +143,29 @@ static uint32_t leaf_code(const G* g, const uint8_t* leaf, uint32_t n)
     }
 }
 
+/* LeafData<ValueOnIndex>::getValue, NanoVDB.h:4160-4170: 0 (the background) for an inactive voxel,
+ * else offset + active voxels before n (popcount of the word below n + 9-bit prefix of the words
+ * before it) */
+static uint64_t leaf_index(const uint8_t* leaf, uint32_t n)
+{
+    const uint32_t w = n >> 6;
+    const uint64_t word = rd_u64(leaf + LEAF_VMASK + 8 * w), bit = (uint64_t)1 << (n & 63u);
+    if (!(word & bit)) return 0;
+    uint64_t sum = rd_u64(leaf + IDX_LEAF_OFFSET) + (uint64_t)__builtin_popcountll(word & (bit - 1u));
+    if (w) sum += (rd_u64(leaf + IDX_LEAF_PREFIX) >> (9u * (w - 1u))) & 511u;
+    return sum;
+}
+
+/* value of an internal-node table entry or a root tile: a float, or an index into the channel */
+static float entry_value(const G* g, const uint8_t* entry)
+{
+    return g->type == GT_ONINDEX ? g->channel[rd_u64(entry)] : rd_f32(entry);
+}
+
 static float leaf_value(const G* g, const uint8_t* leaf, uint32_t n)
 {
     if (g->type == GT_FLOAT) return rd_f32(leaf + LEAF_DATA + 4 * n);
+    if (g->type == GT_ONINDEX) return g->channel[leaf_index(leaf, n)];
     return (float)leaf_code(g, leaf, n) * rd_f32(leaf + LEAF_QUANTUM) + rd_f32(leaf + LEAF_MIN);
 }
 
@@ -133,6 +173,7 @@ static float leaf_value(const G* g, const uint8_t* leaf, uint32_t n)
 static double leaf_value_f64(const G* g, const uint8_t* leaf, uint32_t n)
 {
     if (g->type == GT_FLOAT) return (double)rd_f32(leaf + LEAF_DATA + 4 * n);
+    if (g->type == GT_ONINDEX) return (double)g->channel[leaf_index(leaf, n)];
     return (double)leaf_code(g, leaf, n) * (double)rd_f32(leaf + LEAF_QUANTUM) + (double)rd_f32(leaf + LEAF_MIN);
 }
 
@@ -160,17 +201,22 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
                          ((uint64_t)((uint32_t)x >> 12) << 42);
     const uint8_t* tile = NULL;
     for (uint32_t i = 0; i < g->tableSize; ++i) {
-        const uint8_t* t = g->root + ROOT_SIZE + (uint64_t)TILE_SIZE * i;
+        const uint8_t* t = g->root + g->rootSize + (uint64_t)TILE_SIZE * i;
         if (rd_u64(t) == key) { tile = t; break; }
     }
     if (!tile) { r.value = g->background; r.active = 0; r.dim = 4096u; return r; }
     const int64_t child = rd_i64(tile + 8);
-    if (child == 0) { r.value = rd_f32(tile + 20); r.active = rd_u32(tile + 16) > 0u; r.dim = 4096u; return r; }
+    if (child == 0) {
+        r.value = entry_value(g, tile + (g->type == GT_ONINDEX ? IDX_TILE_VALUE : 20));
+        r.active = rd_u32(tile + 16) > 0u;
+        r.dim = 4096u;
+        return r;
+    }
     const uint8_t* up = g->root + child;
     const int upSkip = (int)(rd_u64(up + UP_FLAGS) & 1u);
     const uint32_t nu = ((((uint32_t)x & 4095u) >> 7) << 10) | ((((uint32_t)y & 4095u) >> 7) << 5) | (((uint32_t)z & 4095u) >> 7);
     if (!mask_on(up + UP_CMASK, nu)) {
-        r.value = rd_f32(up + UP_TABLE + 8 * nu);
+        r.value = entry_value(g, up + UP_TABLE + 8 * nu);
         r.active = mask_on(up + UP_VMASK, nu);
         r.dim = upSkip ? 4096u : 128u;
         return r;
@@ -179,7 +225,7 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
     const int loSkip = (int)(rd_u64(lo + LO_FLAGS) & 1u);
     const uint32_t nl = ((((uint32_t)x & 127u) >> 3) << 8) | ((((uint32_t)y & 127u) >> 3) << 4) | (((uint32_t)z & 127u) >> 3);
     if (!mask_on(lo + LO_CMASK, nl)) {
-        r.value = rd_f32(lo + LO_TABLE + 8 * nl);
+        r.value = entry_value(g, lo + LO_TABLE + 8 * nl);
         r.active = mask_on(lo + LO_VMASK, nl);
         r.dim = upSkip ? 4096u : (loSkip ? 128u : 8u);
         return r;

@@ -200,6 +200,8 @@ class Reference:
         L.ref_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _i]
         L.ref_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _i]
         L.ref_write_nvdb.argtypes = [ctypes.c_char_p, _vp, _vp, _i, _i, _i]
+        L.ref_index_grid.argtypes = [_vp, ctypes.c_uint32, _i, _i, ctypes.POINTER(_u64)]
+        L.ref_index_grid.restype = _vp
         L.ref_read_nvdb.argtypes = [ctypes.c_char_p, _i, ctypes.c_char_p, ctypes.POINTER(_u64)]
         L.ref_read_nvdb.restype = _vp
         L.ref_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp, _vp, _vp, _i]
@@ -304,6 +306,11 @@ class Reference:
         v = np.empty(n, np.float32)
         self.L.ref_iso_crossing(_p(buf), _p(eye), _p(dir), n, iso, _p(hit), _p(ijk), _p(t), _p(v), nthreads)
         return hit, ijk, t, v
+
+    def index_grid(self, float_buf, channels=1, include_stats=True, include_tiles=True):
+        n = ctypes.c_uint64()
+        p = self.L.ref_index_grid(_p(float_buf), channels, int(include_stats), int(include_tiles), ctypes.byref(n))
+        return self._take(p, n.value)
 
     def write_nvdb(self, path, bufs, merge=False, raw=False):
         bufs = [np.ascontiguousarray(b) for b in bufs]

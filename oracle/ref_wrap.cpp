@@ -78,6 +78,14 @@ void parallelFor(uint64_t n, int nthreads, F&& f)
 template<typename BuildT>
 const NanoGrid<BuildT>* asGrid(const void* buf) { return reinterpret_cast<const NanoGrid<BuildT>*>(buf); }
 
+// Float-valued accessor of any grid: ReadAccessor for value grids, ChannelAccessor over blind
+// channel 0 for index grids (getAccessor<GridT, float>, NanoVDB.h:5814-5826).
+template<typename GridT>
+auto makeAcc(const GridT* grid)
+{
+    return nanovdb::getAccessor<GridT, float>(*grid);
+}
+
 // Dispatch a functor templated on the build type of the buffer.
 template<typename F>
 int dispatch(const void* buf, F&& f)
@@ -88,6 +96,7 @@ int dispatch(const void* buf, F&& f)
     case GridType::Fp8: return f(asGrid<Fp8>(buf));
     case GridType::Fp16: return f(asGrid<Fp16>(buf));
     case GridType::FpN: return f(asGrid<FpN>(buf));
+    case GridType::OnIndex: return f(asGrid<ValueOnIndex>(buf));
     default: return -4;
     }
 }
@@ -103,7 +112,7 @@ template<typename GridT, typename RealT>
 void fogMarch(const GridT* grid, const math::Ray<float>& wRay, float dt, float sigma, RealT& L, RealT& T, uint32_t& nsteps)
 {
     using RayT = math::Ray<float>;
-    auto acc = grid->getAccessor();
+    auto acc = makeAcc(grid);
     L = RealT(0);
     T = RealT(1);
     nsteps = 0;
@@ -251,7 +260,7 @@ int ref_get_value(const void* buf, const int32_t* ijk, uint64_t n, float* val, u
 {
     return dispatch(buf, [&](auto* grid) {
         parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
-            auto acc = grid->getAccessor();
+            auto acc = makeAcc(grid);
             const math::Ray<float> ray;
             for (uint64_t i = b; i < e; ++i) {
                 const Coord c(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
@@ -274,7 +283,7 @@ int ref_sample(const void* buf, const float* xyz, uint64_t n, int world, float* 
     return dispatch(buf, [&](auto* grid) {
         using GridT = std::remove_cv_t<std::remove_pointer_t<decltype(grid)>>;
         parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
-            auto acc = grid->getAccessor();
+            auto acc = makeAcc(grid);
             auto s0 = math::createSampler<0, decltype(acc), false>(acc);
             auto s1 = math::createSampler<1, decltype(acc), false>(acc);
             math::GradStencil<GridT> st(*grid);
@@ -334,7 +343,7 @@ int ref_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
 {
     return dispatch(buf, [&](auto* grid) {
         parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
-            auto acc = grid->getAccessor();
+            auto acc = makeAcc(grid);
             auto s1 = math::createSampler<1, decltype(acc), false>(acc);
             for (uint64_t i = b; i < e; ++i) {
                 math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
@@ -368,13 +377,28 @@ int ref_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
     });
 }
 
+// createNanoGrid<NanoGrid<float>, ValueOnIndex>(grid, channels, includeStats, includeTiles)
+// (tools/CreateNanoGrid.h:183-207): an index grid of the active values, with `channels` float
+// channels of the source values as blind data.
+void* ref_index_grid(const void* float_buf, uint32_t channels, int include_stats, int include_tiles, uint64_t* bytes)
+{
+    try {
+        auto h = tools::createNanoGrid<NanoGrid<float>, ValueOnIndex, HostBuffer>(
+            *asGrid<float>(float_buf), channels, include_stats != 0, include_tiles != 0, 0);
+        return copyOut(h, bytes);
+    } catch (...) {
+        *bytes = 0;
+        return nullptr;
+    }
+}
+
 // math::isoCrossing (math/HDDA.h:225-248) on world-space rays mapped by Ray::worldToIndexF.
 int ref_iso_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, uint8_t* hit,
                      int32_t* ijk, float* t, float* v, int nthreads)
 {
     return dispatch(buf, [&](auto* grid) {
         parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
-            auto acc = grid->getAccessor();
+            auto acc = makeAcc(grid);
             for (uint64_t i = b; i < e; ++i) {
                 math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
                                       Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
@@ -445,29 +469,30 @@ void* ref_read_nvdb(const char* path, int index, const char* name, uint64_t* byt
 int ref_fog_march(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma, float* L,
                   float* T, double* L64, double* T64, uint32_t* steps, int nthreads)
 {
-    if (reinterpret_cast<const GridData*>(buf)->mGridType != GridType::Float) return -4;
-    const auto* grid = asGrid<float>(buf);
-    parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
-        for (uint64_t i = b; i < e; ++i) {
-            math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
-                                  Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
-            uint32_t ns = 0;
-            if (L || T) {
-                float l, tr;
-                fogMarch<NanoGrid<float>, float>(grid, wRay, dt, sigma, l, tr, ns);
-                if (L) L[i] = l;
-                if (T) T[i] = tr;
+    return dispatch(buf, [&](auto* grid) {
+        using GridT = std::remove_cv_t<std::remove_pointer_t<decltype(grid)>>;
+        parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; ++i) {
+                math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
+                                      Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
+                uint32_t ns = 0;
+                if (L || T) {
+                    float l, tr;
+                    fogMarch<GridT, float>(grid, wRay, dt, sigma, l, tr, ns);
+                    if (L) L[i] = l;
+                    if (T) T[i] = tr;
+                }
+                if (L64 || T64) {
+                    double l, tr;
+                    fogMarch<GridT, double>(grid, wRay, dt, sigma, l, tr, ns);
+                    if (L64) L64[i] = l;
+                    if (T64) T64[i] = tr;
+                }
+                if (steps) steps[i] = ns;
             }
-            if (L64 || T64) {
-                double l, tr;
-                fogMarch<NanoGrid<float>, double>(grid, wRay, dt, sigma, l, tr, ns);
-                if (L64) L64[i] = l;
-                if (T64) T64[i] = tr;
-            }
-            if (steps) steps[i] = ns;
-        }
+        });
+        return 0;
     });
-    return 0;
 }
 
 // RayGenOp of the level-set example: eye = centre + (0,0,wBBoxDimZ), pinhole FOV 45

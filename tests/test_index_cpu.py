@@ -1,0 +1,97 @@
+"""Index grids (ValueOnIndex + float channel): the oracle against the reference's ChannelAccessor
+(getAccessor<GridT, float>) on the same grids, bit-exact (CPU only)."""
+import numpy as np
+import pytest
+
+from conftest import points_near, rng
+from pyoracle import have_reference
+
+pytestmark = pytest.mark.skipif(not have_reference(), reason="reference sources not built")
+
+
+def _bits(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+
+
+def _sources(ref):
+    import openvdb_amd as nvq
+    return {
+        "sphere": ref.level_set_sphere(40.0, (1.5, -2.0, 3.0), 0.5, 3.0),
+        "torus": nvq.tools.create_level_set_torus(60.0, 15.0, (0.0, 0.0, 0.0), 1.0, 3.0),
+        "fog": nvq.tools.create_fog_fbm([0, 0, 0, 63, 63, 63], seed=3),
+    }
+
+
+@pytest.fixture(scope="module")
+def index_grids(ref):
+    out = {}
+    for name, f in _sources(ref).items():
+        for stats, tiles in ((True, True), (False, False)):
+            out[(name, stats, tiles)] = (f, ref.index_grid(f, 1, stats, tiles))
+    return out
+
+
+def test_index_grid_header(index_grids):
+    for (name, _, _), (f, ix) in index_grids.items():
+        assert np.frombuffer(ix[636:640].tobytes(), np.uint32)[0] == 20  # GridType::OnIndex
+        assert np.frombuffer(ix[632:636].tobytes(), np.uint32)[0] == 8   # GridClass::IndexGrid
+
+
+@pytest.mark.parametrize("key", [("sphere", True, True), ("torus", False, False), ("fog", True, True)])
+def test_get_value_and_sample(oracle, ref, index_grids, key):
+    f, ix = index_grids[key]
+    ijk = rng(1).integers(-110, 110, (40000, 3)).astype(np.int32)
+    rv, ra, rd = ref.get_value(ix, ijk)
+    ov, oa, od = oracle.get_value(ix, ijk)
+    assert _bits(rv, ov) and _bits(ra, oa) and _bits(rd, od)
+    # active voxels keep the source values; inactive ones read the background (index 0)
+    fv, fa, _ = ref.get_value(f, ijk)
+    assert _bits(fa, ra) and _bits(fv[fa == 1], rv[ra == 1])
+    xyz = points_near(oracle, f, 40000, seed=2)
+    for mode in (1, 3):
+        r = ref.sample(ix, xyz, grad_mode=mode)
+        o = oracle.sample(ix, xyz, grad_mode=mode)
+        assert all(_bits(a, b) for a, b in zip(r, o)), mode
+    # GradStencil: the reference's stencil reads indices through its own ReadAccessor
+    # (math/Stencils.h BaseStencil), not the channel, so it is pinned against the source grid's
+    # GradStencil wherever the six neighbours are active (their index-grid values are the source's)
+    og = oracle.sample(ix, xyz, grad_mode=2)[2]
+    fg = ref.sample(f, xyz, grad_mode=2)[2]
+    c = np.floor(xyz).astype(np.int32)
+    ok = np.ones(len(xyz), bool)
+    for axis in range(3):
+        for s in (-1, 1):
+            nb = c.copy()
+            nb[:, axis] += s
+            ok &= ref.get_value(f, nb)[1] == 1
+    assert int(ok.sum()) > 1000
+    assert _bits(og[ok], fg[ok])
+    rw = ref.sample(ix, xyz * np.float32(0.5), world=True, grad_mode=1)
+    ow = oracle.sample(ix, xyz * np.float32(0.5), world=True, grad_mode=1)
+    assert all(_bits(a, b) for a, b in zip(rw, ow))
+
+
+@pytest.mark.parametrize("key", [("sphere", True, True), ("torus", False, False)])
+def test_zero_and_iso_crossing(oracle, ref, index_grids, key):
+    import openvdb_amd as nvq
+    _, ix = index_grids[key]
+    e, d = nvq.tools.camera_rays(64, 64, (20.0, 35.0, 250.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0)
+    rz = ref.zero_crossing(ix, e, d)
+    oz = oracle.zero_crossing(ix, e, d)
+    assert int(rz[0].sum()) > 300
+    assert all(_bits(a, b) for a, b in zip(rz, oz))
+    for iso in (0.5, -0.25):
+        ri = ref.iso_crossing(ix, e, d, iso)
+        oi = oracle.iso_crossing(ix, e, d, iso)[:4]
+        assert all(_bits(a, b) for a, b in zip(ri, oi))
+
+
+def test_fog_march_on_an_index_grid(oracle, ref, index_grids):
+    import openvdb_amd as nvq
+    _, ix = index_grids[("fog", True, True)]
+    e, d = nvq.tools.camera_rays(32, 32, (32.0, 32.0, 200.0), (32.0, 32.0, 32.0), (0.0, 1.0, 0.0), 40.0)
+    rL, rT, rL64, rT64, rsteps = ref.fog_march(ix, e, d, 0.5, 0.05)
+    oL, oT, osteps = oracle.fog_march(ix, e, d, 0.5, 0.05)
+    assert osteps.max() > 50 and _bits(rsteps, osteps)
+    np.testing.assert_allclose(oL, rL64, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(oT, rT64, rtol=1e-12, atol=1e-15)
