@@ -66,7 +66,7 @@ def test_rays_on_index_grids(oracle, sphere_index, fog_index):
     for iso in (0.0, 0.5):
         got = nvq.ops.iso_crossing(d, _t(e), _t(dr), iso, image_width=96)
         exp = oracle.iso_crossing(sphere_index, e, dr, iso)
-        assert int(exp[0].sum()) > 1000
+        assert int(exp[0].sum()) > 300
         for name, x, y in zip(("hit", "ijk", "t", "v", "normal"), got, exp):
             _bits(f"iso {iso} {name}", x, y)
     f = nvq.DeviceGrid(fog_index)
