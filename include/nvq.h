@@ -151,17 +151,22 @@ int nvq_render_iso_surface(const nvq_grid* grid, const nvq_iso_camera* camera, i
 int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                   float dt, float sigma_t, float* radiance, float* transmittance, void* stream);
 
+/* Per-grid query of nvq_multi_query */
+enum nvq_multi_op { NVQ_OP_ZERO_CROSSING = 0, NVQ_OP_FOG_MARCH = 1 };
+
 /* One persistent launch over k <= 32 grids of any leaf codecs and classes.  host_grids[k] and
- * host_ray_offsets[k+1] / host_image_width[k] are HOST arrays; the rays of grid j are
+ * host_ray_offsets[k+1] / host_image_width[k] / host_ops[k] are HOST arrays; the rays of grid j are
  * eye/dir[host_ray_offsets[j] .. host_ray_offsets[j+1]), a row-major image of width
  * host_image_width[j] (its ray count must be a multiple of the width; width 0 = rows of 64).
- * Level-set grids run zeroCrossing (hit, ijk, t -> out_a, v -> out_b, normal); fog-volume grids
- * run the emission-absorption march (radiance -> out_a, transmittance -> out_b).  Wavefronts
- * pull 8x8 ray tiles from one device work counter (the persistent-warp launch of
+ * host_ops[j] chooses the query: NVQ_OP_ZERO_CROSSING (hit, ijk, t -> out_a, v -> out_b, normal) or
+ * NVQ_OP_FOG_MARCH (radiance -> out_a, transmittance -> out_b).  With host_ops NULL the class
+ * decides (level set: zeroCrossing, fog volume: march) and other classes, e.g. index grids, are
+ * rejected.  Wavefronts pull 8x8 ray tiles from a device work queue (the persistent-warp launch of
  * examples/ex_raytrace_iso_surface/common.h:166-189).  hit / ijk / normal may be NULL. */
 int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
-                    const uint32_t* host_image_width, const float* eye, const float* dir, float dt, float sigma_t,
-                    uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal, void* stream);
+                    const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
+                    float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal,
+                    void* stream);
 
 /* Adjoint of nvq_fog_march (no counterpart in the reference): dLoss/dvalue for every leaf value
  * of a Float grid, given the forward outputs radiance[n] (and transmittance[n] when

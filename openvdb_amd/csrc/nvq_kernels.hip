@@ -527,8 +527,9 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
 }
 
 int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
-                    const uint32_t* host_image_width, const float* eye, const float* dir, float dt, float sigma_t,
-                    uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal, void* stream)
+                    const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
+                    float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal,
+                    void* stream)
 {
     if (!host_grids || !host_ray_offsets || k == 0 || k > kMaxGrids || !(dt > 0.0f)) return NVQ_ERR_INVALID_ARG;
     MultiArgs A{};
@@ -549,7 +550,14 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
         S.height = uint32_t((n + w - 1) / w);
         S.tilesX = (w + 7) / 8;
         S.tileBase = uint32_t(tiles);
-        S.op = g->info.grid_class == 2u ? kOpFogMarch : kOpZeroCrossing;
+        if (host_ops) {
+            if (host_ops[j] != NVQ_OP_ZERO_CROSSING && host_ops[j] != NVQ_OP_FOG_MARCH) return NVQ_ERR_INVALID_ARG;
+            S.op = host_ops[j] == NVQ_OP_FOG_MARCH ? kOpFogMarch : kOpZeroCrossing;
+        } else if (g->info.grid_class == 1u || g->info.grid_class == 2u) {
+            S.op = g->info.grid_class == 2u ? kOpFogMarch : kOpZeroCrossing;
+        } else {
+            return NVQ_ERR_INVALID_ARG; // e.g. index grids: the class does not say which query
+        }
         if (n && (!eye || !dir)) return NVQ_ERR_INVALID_ARG;
         tiles += uint64_t(S.tilesX) * ((S.height + 7) / 8);
     }

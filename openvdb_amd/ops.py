@@ -147,11 +147,16 @@ def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):
     return L, T
 
 
-def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.05):
+OP_ZERO_CROSSING, OP_FOG_MARCH = 0, 1
+
+
+def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.05, ops=None):
     """One persistent launch over ``grids`` (<= 32, any codecs).  The rays of grid j are
     ``eye/dir[offsets[j]:offsets[j+1]]``; ``image_widths[j]`` (0 = linear batch) makes them a
-    row-major image.  Returns (hit, ijk, a, b, normal): level sets give hit/ijk/t/v/normal,
-    fog volumes give radiance in ``a`` and transmittance in ``b``."""
+    row-major image.  ``ops[j]`` is OP_ZERO_CROSSING or OP_FOG_MARCH (None: level sets run
+    zeroCrossing, fog volumes the march; index grids need ``ops``).  Returns
+    (hit, ijk, a, b, normal): zeroCrossing gives hit/ijk/t/v/normal, the march gives radiance in
+    ``a`` and transmittance in ``b``."""
     eye = _points(eye, torch.float32, "eye")
     dir = _points(dir, torch.float32, "dir")
     k = len(grids)
@@ -162,12 +167,13 @@ def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.0
     hs = (ctypes.c_void_p * max(k, 1))(*[g.handle.value for g in grids])
     offs = (ctypes.c_uint64 * (k + 1))(*[int(o) for o in offsets])
     widths = None if image_widths is None else (ctypes.c_uint32 * max(k, 1))(*[int(w) for w in image_widths])
+    opa = None if ops is None else (ctypes.c_uint32 * max(k, 1))(*[int(x) for x in ops])
     hit = torch.zeros(n, dtype=torch.uint8, device=dev)
     ijk = torch.zeros((n, 3), dtype=torch.int32, device=dev)
     a = torch.zeros(n, dtype=torch.float32, device=dev)
     b = torch.zeros(n, dtype=torch.float32, device=dev)
     nrm = torch.zeros((n, 3), dtype=torch.float32, device=dev)
-    check(lib().nvq_multi_query(hs, k, offs, widths, _ptr(eye), _ptr(dir), dt, sigma_t, _ptr(hit), _ptr(ijk),
+    check(lib().nvq_multi_query(hs, k, offs, widths, opa, _ptr(eye), _ptr(dir), dt, sigma_t, _ptr(hit), _ptr(ijk),
                                 _ptr(a), _ptr(b), _ptr(nrm), _stream()), "nvq_multi_query")
     return hit, ijk, a, b, nrm
 

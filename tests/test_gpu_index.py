@@ -79,12 +79,16 @@ def test_rays_on_index_grids(oracle, sphere_index, fog_index):
     # both index grids in one persistent launch: identical to the single-grid kernels
     n1 = len(e)
     hit, ijk, a, b, nrm = nvq.ops.multi_query([d, f], _t(np.concatenate([e, e2])), _t(np.concatenate([dr, d2])),
-                                              [0, n1, n1 + len(e2)], [96, 48])
+                                              [0, n1, n1 + len(e2)], [96, 48],
+                                              ops=[nvq.ops.OP_ZERO_CROSSING, nvq.ops.OP_FOG_MARCH])
     z = nvq.ops.zero_crossing(d, _t(e), _t(dr), image_width=96)
     _bits("multi hit", hit[:n1], z[0].cpu().numpy())
     _bits("multi t", a[:n1], z[2].cpu().numpy())
     _bits("multi fog L", a[n1:], L.cpu().numpy())
     _bits("multi fog T", b[n1:], T.cpu().numpy())
+    with pytest.raises(nvq.NvqError):  # an index grid's class does not choose the query
+        nvq.ops.multi_query([d, f], _t(np.concatenate([e, e2])), _t(np.concatenate([dr, d2])),
+                            [0, n1, n1 + len(e2)], [96, 48])
 
 
 def test_iso_render_on_an_index_grid(oracle, ref):
