@@ -169,10 +169,11 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
                     void* stream);
 
 /* Adjoint of nvq_fog_march (no counterpart in the reference): dLoss/dvalue for every leaf value
- * of a Float grid, given the forward outputs radiance[n] (and transmittance[n] when
- * dL_dtransmittance is given) and the loss gradients per ray.  Contributions are ADDED to
- * grad_accum[leafCount*512] (leaf order of the buffer, voxel offset n = x<<6|y<<3|z within the
- * leaf) as int64 fixed point with 2^-fixed_shift resolution (range +-2^(63-fixed_shift)); the
+ * of a Float grid, or every channel value of an index grid (NVQ_ONINDEX), given the forward
+ * outputs radiance[n] (and transmittance[n] when dL_dtransmittance is given) and the loss
+ * gradients per ray.  Contributions are ADDED to grad_accum — Float grids: [leafCount*512] (leaf
+ * order of the buffer, voxel offset n = x<<6|y<<3|z within the leaf); index grids: [value count]
+ * (the channel's own order, a dense scatter) — as int64 fixed point with 2^-fixed_shift resolution (range +-2^(63-fixed_shift)); the
  * sum is deterministic for any schedule.  The caller zeroes grad_accum before the first call
  * and may accumulate several calls (e.g. cameras).  Transmittance is recomputed by re-marching
  * each ray; no per-step state is stored. */

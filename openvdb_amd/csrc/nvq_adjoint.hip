@@ -1,5 +1,6 @@
-// Adjoint of the emission-absorption fog march with respect to the leaf values of a
-// Float grid (no counterpart in the reference, which has no differentiable path).
+// Adjoint of the emission-absorption fog march with respect to the leaf values of a Float grid,
+// or the channel values of an index grid (no counterpart in the reference, which has no
+// differentiable path).
 //
 // Forward (per ray, steps k = 0..N-1 from the shared marchSteps structure):
 //   rho_k = trilinear(x_k), a_k = exp(-s*rho_k) with s = sigma*dt,
@@ -27,6 +28,7 @@
 namespace nvq {
 namespace {
 
+template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const float* __restrict__ eye,
                                                           const float* __restrict__ dir, Sched s, float dt, float sigma,
                                                           const float* __restrict__ Lfwd, const float* __restrict__ Tfwd,
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
     if (!schedule(s, i)) return;
     const float gL = gLin ? gLin[i] : 0.0f, gT = gTin ? gTin[i] : 0.0f;
     if (gL == 0.0f && gT == 0.0f) return;
-    Accessor<GT_FLOAT> acc(g);
+    Accessor<GT> acc(g);
     const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
     const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
     const float Ltot = Lfwd[i];
@@ -86,7 +88,8 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
     const Sched sc = makeSched(n, width);
     const float scale = ldexpf(1.0f, shift);
     ProfScope prof("k_fog_march_bwd", s);
-    hipLaunchKernelGGL(k_fog_march_bwd, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
+    auto kernel = g.type == GT_ONINDEX ? k_fog_march_bwd<GT_ONINDEX> : k_fog_march_bwd<GT_FLOAT>;
+    hipLaunchKernelGGL(kernel, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
                        gT, reinterpret_cast<unsigned long long*>(accum), scale);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -135,10 +135,12 @@ struct Probe {
     const uint8_t* leaf; // leaf holding the voxel or nullptr
     uint32_t n;         // voxel offset in the leaf
     bool active;
+    uint64_t index;     // index grids: the channel index the value comes from
 };
 
 template<uint32_t GT>
 struct Accessor {
+    static constexpr uint32_t kType = GT;
     const uint8_t* root;
     const float* channel;
     uint32_t tableSize;
@@ -170,7 +172,12 @@ struct Accessor {
         const uint32_t n = leafOffset(x, y, z);
         r.leaf = leaf;
         r.n = n;
-        r.value = Codec<GT>::get(leaf, n, channel);
+        if constexpr (GT == GT_ONINDEX) {
+            r.index = Codec<GT_ONINDEX>::index(leaf, n);
+            r.value = channel[r.index];
+        } else {
+            r.value = Codec<GT>::get(leaf, n, channel);
+        }
         r.active = maskOn(leaf + kLeafVMask, n);
         r.dim = (ld_u8(leaf + kLeafFlags) & 1u) ? 8u : 1u;
     }
@@ -195,6 +202,7 @@ struct Accessor {
             r.leaf = nullptr;
             r.n = 0;
             r.value = entryValue(entry);
+            if constexpr (GT == GT_ONINDEX) r.index = entry;
             r.active = maskOn(lower + kLoVMask, n);
             r.dim = (ld_u64(lower + kLoFlags) & 1u) ? 128u : 8u;
         }
@@ -215,6 +223,7 @@ struct Accessor {
             r.leaf = nullptr;
             r.n = 0;
             r.value = entryValue(entry);
+            if constexpr (GT == GT_ONINDEX) r.index = entry;
             r.active = maskOn(upper + kUpVMask, n);
             r.dim = 128u;
         }
@@ -244,11 +253,17 @@ struct Accessor {
         if (!tile) {
             r.value = background;
             r.active = false;
+            if constexpr (GT == GT_ONINDEX) r.index = ld_u64(root + kIdxRootBackground);
             return;
         }
         const int64_t child = ld_i64(tile + 8);
         if (child == 0) {
-            r.value = GT == GT_ONINDEX ? channel[ld_u64(tile + kIdxTileValue)] : ld_f32(tile + 20);
+            if constexpr (GT == GT_ONINDEX) {
+                r.index = ld_u64(tile + kIdxTileValue);
+                r.value = channel[r.index];
+            } else {
+                r.value = ld_f32(tile + 20);
+            }
             r.active = ld_u32(tile + 16) > 0u;
             return;
         }
@@ -434,9 +449,13 @@ struct CellWindow {
             if constexpr (KEYS) {
                 const Probe p = acc.probe(ni + a, nj + b, nk + e);
                 c.v[a][b][e] = p.value;
-                // leaf index = byte offset / 2144 (Float leaf size) = (offset / 32) / 67, exact and in
-                // 32 bits for the <= 2^22 leaves the adjoint accepts
-                key[s] = p.leaf ? int32_t((uint32_t(uint64_t(p.leaf - g.leaf0) >> 5) / 67u) * 512u + p.n) : -1;
+                if constexpr (AccT::kType == GT_ONINDEX) {
+                    key[s] = int32_t(p.index); // the channel entry the corner reads (< 2^31 values)
+                } else {
+                    // leaf index = byte offset / 2144 (Float leaf size) = (offset / 32) / 67, exact and
+                    // in 32 bits for the <= 2^22 leaves the adjoint accepts
+                    key[s] = p.leaf ? int32_t((uint32_t(uint64_t(p.leaf - g.leaf0) >> 5) / 67u) * 512u + p.n) : -1;
+                }
                 sum[s] = 0.0f;
             } else {
                 c.v[a][b][e] = acc.getValue(ni + a, nj + b, nk + e);

@@ -508,8 +508,11 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
     if (!grid || !grad_accum || (n && (!eye || !dir || !radiance)) || !(dt > 0.0f) || fixed_shift < 0 ||
         fixed_shift > 62 || (dL_dtransmittance && !transmittance))
         return NVQ_ERR_INVALID_ARG;
-    if (grid->view.type != GT_FLOAT) return NVQ_ERR_UNSUPPORTED_TYPE;
-    if (grid->info.node_count[0] > (1u << 22)) return NVQ_ERR_INVALID_ARG; // leaf*512+n must fit the int32 key
+    if (grid->view.type != GT_FLOAT && grid->view.type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (!queryable(grid)) return NVQ_ERR_INVALID_ARG;
+    // keys are int32: leaf*512+n for Float grids, the channel index for index grids
+    if (grid->view.type == GT_FLOAT && grid->info.node_count[0] > (1u << 22)) return NVQ_ERR_INVALID_ARG;
+    if (grid->view.type == GT_ONINDEX && grid->valueCount >= (uint64_t(1) << 31)) return NVQ_ERR_INVALID_ARG;
     if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
     if (n == 0 || (!dL_dradiance && !dL_dtransmittance)) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;

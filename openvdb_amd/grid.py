@@ -66,6 +66,26 @@ class DeviceGrid:
         f32 = self.device_bytes.view(torch.float32)
         return torch.as_strided(f32, (self.leaf_count, 512), (2144 // 4, 1), (leaf_off + 96) // 4)
 
+    def channel_values(self):
+        """Index grids: [value_count] float32 view of the grid's own channel inside the device buffer
+        (the dense sidecar; writing to it changes the values the grid returns).  It is the parameter
+        tensor of the differentiable march on an index grid."""
+        if self.info.grid_type != 20:
+            raise ValueError("channel_values() needs an index grid")
+        import torch
+        meta = int(np.frombuffer(self._host[640:648].tobytes(), np.int64)[0])
+        count = int(np.frombuffer(self._host[648:652].tobytes(), np.uint32)[0])
+        if count == 0 or int(np.frombuffer(self._host[meta + 28:meta + 32].tobytes(), np.uint32)[0]) != 1:
+            raise ValueError("the grid has no float channel")
+        data = meta + int(np.frombuffer(self._host[meta:meta + 8].tobytes(), np.int64)[0])
+        f32 = self.device_bytes.view(torch.float32)
+        return f32[data // 4:data // 4 + self.value_count]
+
+    @property
+    def grad_size(self):
+        """Length of the adjoint's accumulator: leafCount*512 (Float grids) or value_count (index grids)."""
+        return self.value_count if self.info.grid_type == 20 else self.leaf_count * 512
+
     @property
     def value_count(self):
         """Index grids: number of values the channel holds (GridData::mData1)."""

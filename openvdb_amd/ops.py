@@ -186,7 +186,7 @@ def fog_march_backward(grid, eye, dir, radiance, grad_radiance, transmittance=No
     dir = _points(dir, torch.float32, "dir")
     n = eye.shape[0]
     if accum is None:
-        accum = torch.zeros(grid.leaf_count * 512, dtype=torch.int64, device=eye.device)
+        accum = torch.zeros(grid.grad_size, dtype=torch.int64, device=eye.device)
     gr = None if grad_radiance is None else grad_radiance.contiguous().float()
     gt = None if grad_transmittance is None else grad_transmittance.contiguous().float()
     check(lib().nvq_fog_march_bwd(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t,
@@ -208,8 +208,10 @@ def fixed_to_float(accum, fixed_shift=40, out=None):
 class FogMarch(torch.autograd.Function):
     """Differentiable emission-absorption render: (radiance, transmittance) = march(values).
 
-    ``values`` must be ``grid.leaf_values()`` (the grid reads it in place); the backward pass
-    returns dLoss/dvalues of the same shape, reduced deterministically.
+    ``values`` must be what the grid reads in place: ``grid.leaf_values()`` for a Float grid, or
+    for an index grid its channel (``grid.channel_values()``, or a tensor given to
+    ``grid.set_channel``); the backward pass returns dLoss/dvalues of the same shape, reduced
+    deterministically.
     """
 
     @staticmethod
@@ -230,5 +232,7 @@ class FogMarch(torch.autograd.Function):
 
 
 def fog_render(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0, fixed_shift=40):
-    """Differentiable fog march over ``grid.leaf_values()``; returns (radiance, transmittance)."""
-    return FogMarch.apply(grid.leaf_values(), grid, eye, dir, dt, sigma_t, image_width, fixed_shift)
+    """Differentiable fog march over ``grid.leaf_values()`` (Float grids) or the channel
+    ``grid.channel_values()`` (index grids); returns (radiance, transmittance)."""
+    values = grid.channel_values() if grid.grid_type == "OnIndex" else grid.leaf_values()
+    return FogMarch.apply(values, grid, eye, dir, dt, sigma_t, image_width, fixed_shift)

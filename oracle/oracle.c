@@ -123,6 +123,8 @@ int or_validate(const void* buf, uint64_t bytes)
 }
 uint32_t or_grid_type(const void* buf) { return rd_u32((const uint8_t*)buf + GRID_TYPE); }
 uint32_t or_leaf_count(const void* buf) { return open_grid(buf).leafCount; }
+/* index grids: GridData::mData1, the number of channel values */
+uint64_t or_value_count(const void* buf) { return rd_u64((const uint8_t*)buf + 656); }
 
 /* ------------------------------------------------------------------ leaf decode
  * float NanoVDB.h:3703; Fp4 :3839-3847; Fp8 :3875-3878; Fp16 :3906-3909;
@@ -190,6 +192,7 @@ typedef struct {
     uint32_t dim;
     const uint8_t* leaf; /* leaf node holding the voxel, or NULL */
     uint32_t n;          /* voxel offset in that leaf */
+    uint64_t index;      /* index grids: the channel index the value comes from */
 } Look;
 
 static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
@@ -197,6 +200,8 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
     Look r;
     r.leaf = NULL;
     r.n = 0;
+    r.index = 0;
+    const int idx = g->type == GT_ONINDEX;
     const uint64_t key = ((uint64_t)((uint32_t)z >> 12)) | ((uint64_t)((uint32_t)y >> 12) << 21) |
                          ((uint64_t)((uint32_t)x >> 12) << 42);
     const uint8_t* tile = NULL;
@@ -204,10 +209,17 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
         const uint8_t* t = g->root + g->rootSize + (uint64_t)TILE_SIZE * i;
         if (rd_u64(t) == key) { tile = t; break; }
     }
-    if (!tile) { r.value = g->background; r.active = 0; r.dim = 4096u; return r; }
+    if (!tile) {
+        r.value = g->background;
+        r.active = 0;
+        r.dim = 4096u;
+        if (idx) r.index = rd_u64(g->root + IDX_ROOT_BACKGROUND);
+        return r;
+    }
     const int64_t child = rd_i64(tile + 8);
     if (child == 0) {
-        r.value = entry_value(g, tile + (g->type == GT_ONINDEX ? IDX_TILE_VALUE : 20));
+        r.value = entry_value(g, tile + (idx ? IDX_TILE_VALUE : 20));
+        if (idx) r.index = rd_u64(tile + IDX_TILE_VALUE);
         r.active = rd_u32(tile + 16) > 0u;
         r.dim = 4096u;
         return r;
@@ -217,6 +229,7 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
     const uint32_t nu = ((((uint32_t)x & 4095u) >> 7) << 10) | ((((uint32_t)y & 4095u) >> 7) << 5) | (((uint32_t)z & 4095u) >> 7);
     if (!mask_on(up + UP_CMASK, nu)) {
         r.value = entry_value(g, up + UP_TABLE + 8 * nu);
+        if (idx) r.index = rd_u64(up + UP_TABLE + 8 * nu);
         r.active = mask_on(up + UP_VMASK, nu);
         r.dim = upSkip ? 4096u : 128u;
         return r;
@@ -226,6 +239,7 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
     const uint32_t nl = ((((uint32_t)x & 127u) >> 3) << 8) | ((((uint32_t)y & 127u) >> 3) << 4) | (((uint32_t)z & 127u) >> 3);
     if (!mask_on(lo + LO_CMASK, nl)) {
         r.value = entry_value(g, lo + LO_TABLE + 8 * nl);
+        if (idx) r.index = rd_u64(lo + LO_TABLE + 8 * nl);
         r.active = mask_on(lo + LO_VMASK, nl);
         r.dim = upSkip ? 4096u : (loSkip ? 128u : 8u);
         return r;
@@ -235,6 +249,7 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
     r.leaf = lf;
     r.n = n;
     r.value = leaf_value(g, lf, n);
+    if (idx) r.index = leaf_index(lf, n);
     r.active = mask_on(lf + LEAF_VMASK, n);
     r.dim = upSkip ? 4096u : (loSkip ? 128u : ((lf[LEAF_FLAGS] & 1u) ? 8u : 1u));
     return r;
@@ -825,12 +840,14 @@ void or_fog_march(const void* buf, const float* eye, const float* dir, uint64_t 
 
 /* dLoss/drho_k = dL*(T_k[(1-a_k) + rho_k*s*a_k] - s*R_{>k}) - dT*s*T_N, with s = sigma*dt and
  * R_{>k} = sum_{j>k} T_j (1-a_j) rho_j; scattered with the trilinear weights of step k onto
- * the corner voxels that live in leaf nodes (grad index = leafIndex*512 + n). */
+ * the corner voxels that live in leaf nodes (grad index = leafIndex*512 + n), or, for index grids,
+ * onto the channel entry every corner reads (grad index = channel index). */
 void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma,
                           const double* dL, const double* dT, double* grad)
 {
     const G g = open_grid(buf);
     if (g.leafSize == 0) return;
+    const int idx = g.type == GT_ONINDEX;
     MarchCtx m;
     memset(&m, 0, sizeof(m));
     m.record = 1;
@@ -853,6 +870,10 @@ void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, u
                 for (int b0 = 0; b0 < 2; ++b0)
                     for (int c0 = 0; c0 < 2; ++c0) {
                         const Look l = lookup(&g, st.c[0] + a0, st.c[1] + b0, st.c[2] + c0);
+                        if (idx) {
+                            grad[l.index] += gk * wx[a0] * wy[b0] * wz[c0];
+                            continue;
+                        }
                         if (!l.leaf) continue;
                         const uint64_t li = (uint64_t)(l.leaf - g.leaf0) / g.leafSize;
                         grad[li * 512u + l.n] += gk * wx[a0] * wy[b0] * wz[c0];

@@ -39,6 +39,7 @@ void or_grad_stencil(const void* buf, const int32_t* ijk, uint64_t n, float* gra
                      uint8_t* zc);
 
 /* world rays -> worldToIndexF -> zeroCrossing (+ normalised world-space trilinear-gradient normal) */
+uint64_t or_value_count(const void* buf);
 void or_iso_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, uint8_t* hit,
                      int32_t* ijk, float* t, float* v, float* normal);
 void or_iso_render(const void* buf, int32_t width, int32_t height, float iso, float dx, float wbbox_dim_z,

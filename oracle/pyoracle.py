@@ -49,6 +49,8 @@ class Oracle:
         L.or_sample_f64.argtypes = [_vp, _vp, _u64, _vp, _vp]
         L.or_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
         L.or_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]
+        L.or_value_count.argtypes = [_vp]
+        L.or_value_count.restype = _u64
         L.or_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _vp]
         L.or_iso_render.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f, _f, _f, _f, _vp, _vp, _vp, _vp]
         L.or_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
@@ -149,7 +151,8 @@ class Oracle:
         n = eye.shape[0]
         dL = np.ascontiguousarray(dL, np.float64)
         dT = None if dT is None else np.ascontiguousarray(dT, np.float64)
-        grad = np.zeros(self.leaf_count(buf) * 512, np.float64)
+        is_index = int(np.frombuffer(buf[636:640].tobytes(), np.uint32)[0]) == 20
+        grad = np.zeros(int(self.L.or_value_count(_p(buf))) if is_index else self.leaf_count(buf) * 512, np.float64)
         self.L.or_fog_march_adjoint(_p(buf), _p(eye), _p(dir), n, dt, sigma, _p(dL), _p(dT), _p(grad))
         return grad
 

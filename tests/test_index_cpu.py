@@ -95,3 +95,41 @@ def test_fog_march_on_an_index_grid(oracle, ref, index_grids):
     assert osteps.max() > 50 and _bits(rsteps, osteps)
     np.testing.assert_allclose(oL, rL64, rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(oT, rT64, rtol=1e-12, atol=1e-15)
+
+
+def test_index_fog_adjoint_matches_central_differences(oracle, ref):
+    """The FP64 oracle adjoint on an index grid scatters onto channel entries (the dense sidecar);
+    pinned against central differences of the FP64 oracle march with the channel entry perturbed,
+    h = 1e-3, including tile and background entries that the rays read."""
+    src = ref.fog_volume_sphere(9.0, (0.5, -0.25, 0.75), 1.0, 3.0)
+    grid = ref.index_grid(src, 1, True, True)
+    meta = int(np.frombuffer(grid[640:648].tobytes(), np.int64)[0])
+    data = meta + int(np.frombuffer(grid[meta:meta + 8].tobytes(), np.int64)[0])
+    r = rng(5)
+    n = 64
+    eye = np.tile(np.array([[0.0, 0.0, 40.0]], np.float32), (n, 1)) + r.normal(0, 1.5, (n, 3)).astype(np.float32)
+    tgt = r.uniform(-6, 6, (n, 3))
+    d = tgt - eye
+    dir = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    wL = r.normal(0, 1, n)
+    wT = r.normal(0, 1, n)
+    g = oracle.fog_march_adjoint(grid, eye, dir, 0.5, 0.05, wL, wT)
+    assert len(g) == int(np.frombuffer(grid[656:664].tobytes(), np.uint64)[0])
+    cand = np.nonzero(np.abs(g) > 1e-3 * np.abs(g).max())[0]
+    pick = np.union1d(r.choice(cand, size=min(30, len(cand)), replace=False), [0] if abs(g[0]) > 0 else [])
+    h = 1e-3
+
+    def loss(buf):
+        L, T, _ = oracle.fog_march(buf, eye, dir, 0.5, 0.05)
+        return float(np.dot(wL, L) + np.dot(wT, T))
+
+    for k in pick.astype(np.int64):
+        b = grid.copy()
+        o = data + 4 * int(k)
+        v = np.frombuffer(grid[o:o + 4].tobytes(), np.float32)[0]
+        b[o:o + 4] = np.frombuffer(np.float32(v + h).tobytes(), np.uint8)
+        lp = loss(b)
+        b[o:o + 4] = np.frombuffer(np.float32(v - h).tobytes(), np.uint8)
+        lm = loss(b)
+        fd = (lp - lm) / (float(np.float32(v + h)) - float(np.float32(v - h)))
+        assert abs(fd - g[k]) <= 2e-5 * max(1.0, abs(g[k])) + 1e-6 * np.abs(g).max(), (k, fd, g[k])
