@@ -184,6 +184,23 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
 /* out[i] = float(double(accum[i]) * 2^-fixed_shift) */
 int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream);
 
+/* Device-side construction: tools::cuda::voxelsToGrid<float | ValueOnIndex>(d_ijk, n, voxelSize)
+ * (tools/cuda/PointsToGrid.cuh:119-125, :1417-1424).  ijk is a DEVICE array of n integer voxels
+ * (nanovdb::Coord, int32 x3; duplicates allowed; 1 <= n < 2^31).  grid_type NVQ_FLOAT gives a
+ * grid whose active voxels hold 1 and inactive 0; NVQ_ONINDEX an IndexGrid whose active voxels
+ * hold indices 1..count in breadth-first order.  The buffer equals the reference's byte for
+ * byte.  `name` may be NULL.  `index` (index grids only; may be NULL) is a DEVICE array of n that
+ * receives each input voxel's value index, so that channel[index[i]] = value[i] fills a float
+ * channel for nvq_grid_set_channel.  The new grid owns its device buffer.  The input may span at
+ * most 2^28 upper-node tiles (4096^3 voxels each) in its bbox, else NVQ_ERR_INVALID_ARG.
+ * Synchronises `stream` twice (to size the buffer) and allocates scratch of about 40 B per voxel. */
+int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
+                       uint64_t* index, void* stream, nvq_grid** out);
+/* Copy the first `bytes` bytes (at most nvq_grid_info.bytes; pass that for the whole grid) of a
+ * grid's device buffer to host memory and wait for it: GridHandle::deviceDownload
+ * (GridHandle.h:186-194). */
+int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void* stream);
+
 /* Opt-in kernel timing for measurement tools (no counterpart in the reference).  While enabled,
  * each call records HIP events on its stream around its dominant kernel: "k_sample" (direct
  * point queries), "k_bin_sample" (binned point queries, n >= 2^20), "k_zero_crossing",

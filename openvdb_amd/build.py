@@ -4,6 +4,7 @@ Flags: -ffp-contract=off keeps every float operation of the reference
 unfused (the kernels write fmaf where the reference does); packed FP32 ops are
 disabled.
 """
+import concurrent.futures
 import os
 import subprocess
 
@@ -13,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnvq.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["nvq_kernels.hip", "nvq_binned.hip", "nvq_adjoint.hip", "nvq_multi.hip", "nvq_profile.hip", "nvq_iso.hip"]
+HIP_SOURCES = ["nvq_kernels.hip", "nvq_binned.hip", "nvq_adjoint.hip", "nvq_multi.hip", "nvq_profile.hip", "nvq_iso.hip",
+               "nvq_voxels.hip"]
 HOST_SOURCES = ["nvq_build.cpp", "nvq_io.cpp"]  # host-only C++ (grid builder), compiled by g++
 OBJ_DIR = os.path.join(HERE, "_build")
 HOST_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-pthread", "-Wall", "-Wextra",
@@ -40,19 +42,19 @@ def _stale(obj, src):
 def build(verbose=False, incremental=False):
     """Compile every source (incremental=True skips objects newer than their sources and headers)."""
     os.makedirs(OBJ_DIR, exist_ok=True)
-    objs = []
-    for src in HOST_SOURCES:
+    objs, jobs = [], []
+    for src in HOST_SOURCES + HIP_SOURCES:
         obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         if incremental and not _stale(obj, os.path.join(CSRC, src)):
             continue
-        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
-    for src in HIP_SOURCES:
-        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-        objs.append(obj)
-        if incremental and not _stale(obj, os.path.join(CSRC, src)):
-            continue
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        cc = [HIPCC] + HIP_FLAGS if src.endswith(".hip") else ["g++"] + HOST_FLAGS
+        jobs.append(cc + ["-c", os.path.join(CSRC, src), "-o", obj])
+    # one compiler per source, at most MAX_JOBS (or the CPU count, capped at 8) at a time
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", 0)) or min(8, os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd, verbose) for cmd in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs + ["-pthread"], verbose)
     os.replace(tmp, LIB)

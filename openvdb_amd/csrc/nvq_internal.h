@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "nvq_device.h"
+#include "../../include/nvq.h"
 
 namespace nvq {
 
@@ -131,5 +132,9 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
                      float sigma, const float* L, const float* T, const float* gL, const float* gT, int64_t* accum,
                      int shift, hipStream_t s);
 int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hipStream_t s);
+
+// Handle for a grid buffer of `bytes` at device address d, its headers read from the host copy b
+// (nvq_kernels.hip).
+int makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hipStream_t s, nvq_grid** out);
 
 } // namespace nvq

@@ -303,27 +303,18 @@ const char* nvq_status_string(int status)
     }
 }
 
-static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, void* stream, nvq_grid** out)
+} // extern "C"
+
+// Build the handle of a grid whose `bytes`-long buffer already is (or is being copied) at device
+// address d.  `b` is a host copy of at least its grid, tree and root headers, or of the whole
+// buffer when an index grid's blind channel must be located.  On failure nothing is freed.
+int nvq::makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hipStream_t stream, nvq_grid** out)
 {
-    if (!out) return NVQ_ERR_INVALID_ARG;
-    *out = nullptr;
-    const uint8_t* b = static_cast<const uint8_t*>(host_buf);
-    const int st = validate(b, bytes);
-    if (st != NVQ_OK) return st;
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
-    void* d = device_dst;
-    if (!d && hipMalloc(&d, bytes) != hipSuccess) return NVQ_ERR_HIP;
-    if (hipMemcpyAsync(d, host_buf, bytes, hipMemcpyHostToDevice, S(stream)) != hipSuccess) {
-        if (!device_dst) (void)hipFree(d);
-        return NVQ_ERR_HIP;
-    }
     nvq_grid* g = new (std::nothrow) nvq_grid;
-    if (!g) {
-        if (!device_dst) (void)hipFree(d);
-        return NVQ_ERR_HIP;
-    }
-    g->owns = device_dst == nullptr;
+    if (!g) return NVQ_ERR_HIP;
+    g->owns = owns;
     g->dbuf = d;
     g->dview = nullptr;
     g->bytes = bytes;
@@ -370,14 +361,31 @@ static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, vo
     in.background = v.background;
     in.device_buffer = d;
     if (hipMalloc(reinterpret_cast<void**>(&g->dview), sizeof(GridView)) != hipSuccess ||
-        hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, S(stream)) != hipSuccess) {
+        hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, stream) != hipSuccess) {
         if (g->dview) (void)hipFree(g->dview);
-        if (g->owns) (void)hipFree(d);
         delete g;
         return NVQ_ERR_HIP;
     }
     *out = g;
     return NVQ_OK;
+}
+
+extern "C" {
+
+static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, void* stream, nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    const uint8_t* b = static_cast<const uint8_t*>(host_buf);
+    const int st = validate(b, bytes);
+    if (st != NVQ_OK) return st;
+    void* d = device_dst;
+    if (!d && hipMalloc(&d, bytes) != hipSuccess) return NVQ_ERR_HIP;
+    int rc = NVQ_ERR_HIP;
+    if (hipMemcpyAsync(d, host_buf, bytes, hipMemcpyHostToDevice, S(stream)) == hipSuccess)
+        rc = makeGridHandle(b, bytes, d, device_dst == nullptr, S(stream), out);
+    if (rc != NVQ_OK && !device_dst) (void)hipFree(d);
+    return rc;
 }
 
 int nvq_grid_upload(const void* host_buf, uint64_t bytes, void* stream, nvq_grid** out)

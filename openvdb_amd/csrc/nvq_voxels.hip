@@ -1,0 +1,514 @@
+// Device-side grid construction from integer voxels: voxelsToGrid<float | ValueOnIndex>
+// (tools/cuda/PointsToGrid.cuh:119-125, :1417-1424).  The buffer written is the reference's,
+// byte for byte (tile order, node order, masks, child offsets, bboxes, index offsets).
+//
+// The reference sorts 63-bit tile keys, run-length encodes them and then sorts the 36-bit
+// in-tile keys segment by segment (countNodes, :619-826).  Here the voxels' upper-node tiles are
+// numbered densely over the bbox of the input (x-major, which is the order of the reference's
+// tile keys), so one radix sort of (tile << 36 | in-tile key) over 36 + log2(tiles) bits gives
+// the final order.  One scan of the run heads at the four levels (voxel, leaf, lower, upper)
+// then numbers every node, and each node is written by one thread (headers, child tables) or
+// one 64-wide wavefront (leaf masks, bboxes): no per-voxel atomics, no host round trip
+// between the levels.
+#include "nvq_device.h"
+#include "nvq_internal.h"
+#include "../../include/nvq.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+using namespace nvq;
+
+namespace {
+
+constexpr int kB = 256;
+constexpr uint32_t kWaves = kB / 64;
+constexpr uint64_t kTreeOff = 672, kRootOff = 736;
+constexpr uint64_t kUpperBytes = 270400, kLowerBytes = 33856;
+
+// Inclusive counts of run heads at the voxel, leaf, lower and upper level.
+struct Cnt {
+    uint32_t v, l, m, u;
+};
+struct CntSum {
+    __host__ __device__ Cnt operator()(const Cnt& a, const Cnt& b) const
+    {
+        return Cnt{a.v + b.v, a.l + b.l, a.m + b.m, a.u + b.u};
+    }
+};
+// Position p of the sorted keys starts a new voxel / leaf / lower / upper when the key differs
+// from its predecessor in the bits above 0 / 9 / 21 / 36.
+struct HeadOp {
+    const uint64_t* keys;
+    __host__ __device__ Cnt operator()(uint32_t p) const
+    {
+        if (p == 0) return Cnt{1u, 1u, 1u, 1u};
+        const uint64_t k = keys[p], q = keys[p - 1];
+        return Cnt{uint32_t(k != q), uint32_t((k >> 9) != (q >> 9)), uint32_t((k >> 21) != (q >> 21)),
+                   uint32_t((k >> 36) != (q >> 36))};
+    }
+};
+
+struct Layout {
+    uint64_t offUpper, offLower, offLeaf, total;
+    uint32_t rootHdr, leafSize, isIndex;
+    int32_t tlo[3];   // smallest tile coordinate (x >> 12) per axis
+    uint32_t dim[3];  // tiles per axis in the bbox of the input
+};
+
+__device__ __forceinline__ void tileOf(const Layout& L, uint64_t tile, int32_t t[3])
+{
+    t[2] = int32_t(tile % L.dim[2]) + L.tlo[2];
+    const uint64_t r = tile / L.dim[2];
+    t[1] = int32_t(r % L.dim[1]) + L.tlo[1];
+    t[0] = int32_t(r / L.dim[1]) + L.tlo[0];
+}
+
+__global__ void k_v2g_bounds_init(int32_t* bnd)
+{
+    if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
+}
+
+// Exact bbox of the input voxels (= RootData::mBBox of the result, processBBox :1386-1414).
+__global__ __launch_bounds__(kB) void k_v2g_bounds(const int32_t* __restrict__ ijk, uint64_t n, int32_t* __restrict__ bnd)
+{
+    int32_t lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    const uint64_t stride = uint64_t(gridDim.x) * kB;
+    for (uint64_t i = uint64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int32_t v = ijk[3 * i + a];
+            lo[a] = min(lo[a], v);
+            hi[a] = max(hi[a], v);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+    if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(bnd + a, lo[a]);
+            atomicMax(bnd + 3 + a, hi[a]);
+        }
+    }
+}
+
+// Sort key: dense tile number << 36 | upper offset << 21 | lower offset << 9 | leaf offset
+// (InternalNode::CoordToOffset NanoVDB.h:3560-3566, LeafNode::CoordToOffset :4516-4519).
+__global__ __launch_bounds__(kB) void k_v2g_keys(const int32_t* __restrict__ ijk, uint32_t n, Layout L,
+                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ ids)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const int32_t x = ijk[3ull * i], y = ijk[3ull * i + 1], z = ijk[3ull * i + 2];
+    const uint64_t tile = (uint64_t(uint32_t((x >> 12) - L.tlo[0])) * L.dim[1] + uint32_t((y >> 12) - L.tlo[1])) * L.dim[2] +
+                          uint32_t((z >> 12) - L.tlo[2]);
+    const uint64_t up = (uint64_t((x & 4095) >> 7) << 10) | (uint64_t((y & 4095) >> 7) << 5) | uint64_t((z & 4095) >> 7);
+    const uint64_t lo = (uint64_t((x & 127) >> 3) << 8) | (uint64_t((y & 127) >> 3) << 4) | uint64_t((z & 127) >> 3);
+    const uint64_t lf = (uint64_t(x & 7) << 6) | (uint64_t(y & 7) << 3) | uint64_t(z & 7);
+    keys[i] = (tile << 36) | (up << 21) | (lo << 9) | lf;
+    ids[i] = i;
+}
+
+// One thread per sorted position: the node that starts there writes its root tile / parent
+// child-table entry / child-mask bit (BuildUpperNodesFunctor :1034-1061, BuildLowerNodesFunctor
+// :1085-1104, ProcessLeafMetaDataFunctor :1126-1154) and records where its children start.
+__global__ __launch_bounds__(kB) void k_v2g_heads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
+                                                  const Cnt* __restrict__ cnt, uint32_t n, Layout L,
+                                                  uint8_t* __restrict__ buf, uint32_t* __restrict__ leafFirst,
+                                                  uint32_t* __restrict__ lowerFirst, uint32_t* __restrict__ upperFirst,
+                                                  uint64_t* __restrict__ index)
+{
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= n) return;
+    const Cnt c = cnt[p];
+    const uint64_t k = keys[p], q = p ? keys[p - 1] : 0;
+    const bool first = p == 0;
+    const bool hU = first || (k >> 36) != (q >> 36), hM = first || (k >> 21) != (q >> 21), hL = first || (k >> 9) != (q >> 9);
+    const uint32_t u = c.u - 1u, m = c.m - 1u, l = c.l - 1u;
+    const uint32_t n2 = uint32_t(k >> 21) & 32767u, n1 = uint32_t(k >> 9) & 4095u;
+    const uint64_t oU = L.offUpper + kUpperBytes * u, oL = L.offLower + kLowerBytes * m, oF = L.offLeaf + uint64_t(L.leafSize) * l;
+    if (hU) {
+        int32_t t[3];
+        tileOf(L, k >> 36, t);
+        uint8_t* tile = buf + kRootOff + L.rootHdr + 32ull * u;
+        // RootData::CoordToKey (NanoVDB.h:2633-2640) of the tile origin, Tile::setChild
+        const uint64_t key = uint64_t(uint32_t(t[2] * 4096) >> 12) | (uint64_t(uint32_t(t[1] * 4096) >> 12) << 21) |
+                             (uint64_t(uint32_t(t[0] * 4096) >> 12) << 42);
+        *reinterpret_cast<uint64_t*>(tile) = key;
+        *reinterpret_cast<int64_t*>(tile + 8) = int64_t(oU - kRootOff);
+        *reinterpret_cast<uint64_t*>(buf + oU + 24) = 2u; // GridFlags::HasBBox
+        upperFirst[u] = m;
+    }
+    if (hM) {
+        atomicOr(reinterpret_cast<unsigned long long*>(buf + oU + 4128 + 8u * (n2 >> 6)), 1ull << (n2 & 63u));
+        *reinterpret_cast<int64_t*>(buf + oU + 8256 + 8ull * n2) = int64_t(oL - oU);
+        *reinterpret_cast<uint64_t*>(buf + oL + 24) = 2u;
+        lowerFirst[m] = l;
+    }
+    if (hL) {
+        atomicOr(reinterpret_cast<unsigned long long*>(buf + oL + 544 + 8u * (n1 >> 6)), 1ull << (n1 & 63u));
+        *reinterpret_cast<int64_t*>(buf + oL + 1088 + 8ull * n1) = int64_t(oF - oL);
+        leafFirst[l] = p;
+    }
+    if (index) index[ids[p]] = c.v; // 1 + rank of the voxel: its value index (index 0 is the background)
+    if (p == n - 1) {
+        leafFirst[c.l] = n;
+        lowerFirst[c.m] = c.l;
+        upperFirst[c.u] = c.m;
+    }
+}
+
+template<typename T>
+__device__ __forceinline__ T laneSelect8(const T (&w)[8], uint32_t j)
+{
+    T r = w[0];
+#pragma unroll
+    for (uint32_t i = 1; i < 8; ++i) r = j == i ? w[i] : r;
+    return r;
+}
+
+// One wavefront per leaf: value mask from its voxels, LeafNode::updateBBox (NanoVDB.h:4596-4624),
+// flags, and the index offset + prefix sums (leafPrefixSumKernel :459-479) or the float values
+// (1 active, 0 inactive; SetLeafActiveVoxelStateAndValuesFunctor :1156-1174).
+__global__ __launch_bounds__(kB) void k_v2g_leaves(const uint64_t* __restrict__ keys, const Cnt* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ leafFirst, uint32_t nLeaf, Layout L,
+                                                   uint8_t* __restrict__ buf)
+{
+    const uint32_t lane = threadIdx.x & 63u, l = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (l >= nLeaf) return;
+    const uint32_t a = leafFirst[l], b = leafFirst[l + 1];
+    uint64_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t q = a + lane; q < b; q += 64) {
+        const uint32_t n0 = uint32_t(keys[q]) & 511u;
+        const uint64_t bit = 1ull << (n0 & 63u);
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) w[j] |= (n0 >> 6) == j ? bit : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) w[j] |= __shfl_xor(w[j], off, 64);
+
+    uint8_t* F = buf + L.offLeaf + uint64_t(L.leafSize) * l;
+    if (lane == 0) {
+        const uint64_t k0 = keys[a];
+        int32_t t[3];
+        tileOf(L, k0 >> 36, t);
+        const uint32_t n2 = uint32_t(k0 >> 21) & 32767u, n1 = uint32_t(k0 >> 9) & 4095u;
+        const int32_t origin[3] = {t[0] * 4096 + int32_t((n2 >> 10) & 31u) * 128 + int32_t((n1 >> 8) & 15u) * 8,
+                                   t[1] * 4096 + int32_t((n2 >> 5) & 31u) * 128 + int32_t((n1 >> 4) & 15u) * 8,
+                                   t[2] * 4096 + int32_t(n2 & 31u) * 128 + int32_t(n1 & 15u) * 8};
+        uint64_t word = w[0];
+        uint32_t xmin = w[0] ? 0u : 8u, xmax = xmin;
+#pragma unroll
+        for (uint32_t j = 1; j < 8; ++j)
+            if (w[j]) {
+                word |= w[j];
+                if (xmin == 8u) xmin = j;
+                xmax = j;
+            }
+        const uint32_t ymin = uint32_t(__builtin_ctzll(word)) >> 3, ymax = (63u - uint32_t(__builtin_clzll(word))) >> 3;
+        const uint32_t w32 = uint32_t(word) | uint32_t(word >> 32);
+        const uint32_t w16 = (w32 & 0xffffu) | (w32 >> 16);
+        const uint32_t byte = (w16 & 0xffu) | (w16 >> 8);
+        const uint32_t zmin = uint32_t(__builtin_ctz(byte)), zmax = 31u - uint32_t(__builtin_clz(byte));
+        int4 h;
+        h.x = origin[0] + int32_t(xmin);
+        h.y = origin[1] + int32_t(ymin);
+        h.z = origin[2] + int32_t(zmin);
+        h.w = int32_t((xmax - xmin) | ((ymax - ymin) << 8) | ((zmax - zmin) << 16) | (2u << 24)); // dif[3], flags HasBBox
+        *reinterpret_cast<int4*>(F) = h;
+        if (L.isIndex) {
+            uint64_t sum = uint64_t(__popcll(w[0])), prefix = sum;
+#pragma unroll
+            for (int j = 1; j < 7; ++j) {
+                sum += uint64_t(__popcll(w[j]));
+                prefix |= sum << (9 * j);
+            }
+            *reinterpret_cast<uint64_t*>(F + 80) = uint64_t(cnt[a].v); // 1 + rank of the leaf's first voxel
+            *reinterpret_cast<uint64_t*>(F + 88) = prefix;
+        }
+    }
+    if (lane < 8) *reinterpret_cast<uint64_t*>(F + 16 + 8 * lane) = laneSelect8(w, lane);
+    if (!L.isIndex) {
+        // values 8*lane .. 8*lane+7 live in byte (lane & 7) of mask word lane >> 3
+        const uint32_t bits = uint32_t(laneSelect8(w, lane >> 3) >> (8u * (lane & 7u))) & 255u;
+        float4 v0, v1;
+        v0.x = (bits & 1u) ? 1.0f : 0.0f;
+        v0.y = (bits & 2u) ? 1.0f : 0.0f;
+        v0.z = (bits & 4u) ? 1.0f : 0.0f;
+        v0.w = (bits & 8u) ? 1.0f : 0.0f;
+        v1.x = (bits & 16u) ? 1.0f : 0.0f;
+        v1.y = (bits & 32u) ? 1.0f : 0.0f;
+        v1.z = (bits & 64u) ? 1.0f : 0.0f;
+        v1.w = (bits & 128u) ? 1.0f : 0.0f;
+        float4* dst = reinterpret_cast<float4*>(F + 96 + 32u * lane);
+        dst[0] = v0;
+        dst[1] = v1;
+    }
+}
+
+__device__ __forceinline__ void waveBoxReduce(int32_t lo[3], int32_t hi[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+}
+
+__device__ __forceinline__ void storeBox(uint8_t* node, const int32_t lo[3], const int32_t hi[3])
+{
+    *reinterpret_cast<int4*>(node) = make_int4(lo[0], lo[1], lo[2], hi[0]);
+    *reinterpret_cast<int2*>(node + 16) = make_int2(hi[1], hi[2]);
+}
+
+// One wavefront per lower node: its bbox is the union of its leaves' (processBBox).
+__global__ __launch_bounds__(kB) void k_v2g_lowers(const uint32_t* __restrict__ lowerFirst, uint32_t nLower, Layout L,
+                                                   uint8_t* __restrict__ buf)
+{
+    const uint32_t lane = threadIdx.x & 63u, m = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (m >= nLower) return;
+    int32_t lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (uint32_t l = lowerFirst[m] + lane; l < lowerFirst[m + 1]; l += 64) {
+        const int4 h = *reinterpret_cast<const int4*>(buf + L.offLeaf + uint64_t(L.leafSize) * l);
+        const uint32_t d = uint32_t(h.w);
+        lo[0] = min(lo[0], h.x);
+        lo[1] = min(lo[1], h.y);
+        lo[2] = min(lo[2], h.z);
+        hi[0] = max(hi[0], h.x + int32_t(d & 255u));
+        hi[1] = max(hi[1], h.y + int32_t((d >> 8) & 255u));
+        hi[2] = max(hi[2], h.z + int32_t((d >> 16) & 255u));
+    }
+    waveBoxReduce(lo, hi);
+    if (lane == 0) storeBox(buf + L.offLower + kLowerBytes * m, lo, hi);
+}
+
+// One wavefront per upper node: union of its lower nodes' bboxes.
+__global__ __launch_bounds__(kB) void k_v2g_uppers(const uint32_t* __restrict__ upperFirst, uint32_t nUpper, Layout L,
+                                                   uint8_t* __restrict__ buf)
+{
+    const uint32_t lane = threadIdx.x & 63u, u = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (u >= nUpper) return;
+    int32_t lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (uint32_t m = upperFirst[u] + lane; m < upperFirst[u + 1]; m += 64) {
+        const int32_t* b = reinterpret_cast<const int32_t*>(buf + L.offLower + kLowerBytes * m);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], b[a]);
+            hi[a] = max(hi[a], b[3 + a]);
+        }
+    }
+    waveBoxReduce(lo, hi);
+    if (lane == 0) storeBox(buf + L.offUpper + kUpperBytes * u, lo, hi);
+}
+
+template<typename T>
+void put(uint8_t* b, uint64_t off, const T& v)
+{
+    static_assert(!std::is_pointer<T>::value, "put() copies values, not what a pointer points to");
+    std::memcpy(b + off, &v, sizeof(T));
+}
+
+// Grid, tree and root headers (BuildGridTreeRootFunctor :877-1008 with GridData::init
+// NanoVDB.h:1968-2000, Map(double) :1441-1451, UpdateRootWorldBBoxFunctor :1374-1384).
+void writeHeaders(uint8_t* h, const Layout& L, const uint32_t counts[4], const int32_t box[6], double voxelSize,
+                  const char* name, uint32_t gridType)
+{
+    const uint64_t offTree = kTreeOff, offRoot = kRootOff;
+    for (int a = 0; a < 6; ++a) put(h, offRoot + 4 * a, box[a]);
+    put(h, offRoot + 24, counts[3]);
+    put(h, offTree + 0, int64_t(L.offLeaf - offTree));
+    put(h, offTree + 8, int64_t(L.offLower - offTree));
+    put(h, offTree + 16, int64_t(L.offUpper - offTree));
+    put(h, offTree + 24, int64_t(offRoot - offTree));
+    put(h, offTree + 32, counts[1]);
+    put(h, offTree + 36, counts[2]);
+    put(h, offTree + 40, counts[3]);
+    put(h, offTree + 56, uint64_t(counts[0]));
+
+    put(h, 0, uint64_t(0x314244566f6e614eull)); // "NanoVDB1"
+    put(h, 8, ~uint64_t(0));                    // checksum disabled
+    put(h, 16, uint32_t((32u << 21) | (9u << 10) | 2u));
+    put(h, 20, uint32_t(2u | 32u)); // HasBBox | IsBreadthFirst
+    put(h, 28, uint32_t(1));
+    put(h, 32, L.total);
+    if (name) std::memcpy(h + 40, name, std::min<size_t>(std::strlen(name), 255));
+    const float sF = float(voxelSize), invF = 1.0f / sF;
+    for (int i = 0; i < 9; ++i) {
+        const bool d = i % 4 == 0;
+        put(h, 296 + 4 * i, d ? sF : 0.0f);
+        put(h, 296 + 36 + 4 * i, d ? invF : 0.0f);
+        put(h, 296 + 88 + 8 * i, d ? voxelSize : 0.0);
+        put(h, 296 + 160 + 8 * i, d ? 1.0 / voxelSize : 0.0);
+    }
+    put(h, 296 + 84, 1.0f);
+    put(h, 296 + 256, 1.0);
+    // Map::applyMap (matMult, math/Math.h:1748-1753) of the 8 corners of (min, max + 1)
+    auto apply = [&](const double x[3], double y[3]) {
+        for (int r = 0; r < 3; ++r)
+            y[r] = std::fma(x[0], r == 0 ? voxelSize : 0.0, std::fma(x[1], r == 1 ? voxelSize : 0.0, std::fma(x[2], r == 2 ? voxelSize : 0.0, 0.0)));
+    };
+    double wb[6];
+    for (int c = 0; c < 8; ++c) {
+        const double x[3] = {double(c & 1 ? box[3] + 1 : box[0]), double(c & 2 ? box[4] + 1 : box[1]),
+                             double(c & 4 ? box[5] + 1 : box[2])};
+        double y[3];
+        apply(x, y);
+        for (int a = 0; a < 3; ++a) {
+            wb[a] = c == 0 ? y[a] : std::min(wb[a], y[a]);
+            wb[3 + a] = c == 0 ? y[a] : std::max(wb[3 + a], y[a]);
+        }
+    }
+    put(h, 560, wb);
+    const double one[3] = {1.0, 1.0, 1.0}, zero[3] = {0.0, 0.0, 0.0};
+    double p1[3], p0[3];
+    apply(one, p1);
+    apply(zero, p0);
+    const double vs[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    put(h, 608, vs);
+    put(h, 632, uint32_t(L.isIndex ? 8u : 0u)); // GridClass IndexGrid / Unknown
+    put(h, 636, gridType);
+    put(h, 640, int64_t(L.total)); // no blind metadata
+    put(h, 656, uint64_t(L.isIndex ? uint64_t(counts[0]) + 1u : 0u));
+}
+
+// Device allocations of one build, released when the build returns.
+struct Scratch {
+    std::vector<void*> ptrs;
+    template<typename T>
+    T* get(size_t count)
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~Scratch()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+uint32_t blocksFor(uint64_t items, uint32_t perBlock) { return uint32_t((items + perBlock - 1) / perBlock); }
+
+} // namespace
+
+extern "C" {
+
+int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
+                       uint64_t* index, void* stream, nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!ijk || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size))
+        return NVQ_ERR_INVALID_ARG;
+    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t n32 = uint32_t(n);
+    Scratch S;
+
+    // 1. bbox of the input, read back to size the dense tile numbering
+    int32_t* bnd = S.get<int32_t>(6);
+    if (!bnd) return NVQ_ERR_HIP;
+    k_v2g_bounds_init<<<1, 64, 0, s>>>(bnd);
+    k_v2g_bounds<<<std::min<uint32_t>(blocksFor(n, kB), 2048u), kB, 0, s>>>(ijk, n, bnd);
+    int32_t box[6];
+    if (hipMemcpyAsync(box, bnd, sizeof(box), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    Layout L{};
+    uint64_t tiles = 1;
+    for (int a = 0; a < 3; ++a) {
+        L.tlo[a] = box[a] >> 12;
+        L.dim[a] = uint32_t((box[3 + a] >> 12) - L.tlo[a] + 1);
+        tiles *= L.dim[a];
+    }
+    const int tileBits = tiles > 1 ? 64 - __builtin_clzll(tiles - 1) : 0;
+    if (36 + tileBits > 64) return NVQ_ERR_INVALID_ARG; // input spans more than 2^28 upper-node tiles
+
+    // 2. keys, one radix sort, run-head scan
+    uint64_t* keys = S.get<uint64_t>(n);
+    uint64_t* keys2 = S.get<uint64_t>(n);
+    uint32_t* ids = S.get<uint32_t>(n);
+    uint32_t* ids2 = S.get<uint32_t>(n);
+    Cnt* cnt = S.get<Cnt>(n);
+    if (!keys || !keys2 || !ids || !ids2 || !cnt) return NVQ_ERR_HIP;
+    k_v2g_keys<<<blocksFor(n, kB), kB, 0, s>>>(ijk, n32, L, keys, ids);
+    hipcub::TransformInputIterator<Cnt, HeadOp, hipcub::CountingInputIterator<uint32_t>> heads(
+        hipcub::CountingInputIterator<uint32_t>(0u), HeadOp{keys2});
+    size_t sortBytes = 0, scanBytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keys2, ids, ids2, int(n32), 0, 36 + tileBits, s) !=
+            hipSuccess ||
+        hipcub::DeviceScan::InclusiveScan(nullptr, scanBytes, heads, cnt, CntSum(), int(n32), s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    void* temp = S.get<uint8_t>(std::max(sortBytes, scanBytes));
+    if (!temp) return NVQ_ERR_HIP;
+    if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, ids, ids2, int(n32), 0, 36 + tileBits, s) !=
+            hipSuccess ||
+        hipcub::DeviceScan::InclusiveScan(temp, scanBytes, heads, cnt, CntSum(), int(n32), s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    Cnt tot;
+    if (hipMemcpyAsync(&tot, cnt + (n32 - 1), sizeof(Cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return NVQ_ERR_HIP;
+
+    // 3. layout (GridT::memUsage sums of PointsToGrid::getBuffer :848-856), zeroed buffer
+    L.isIndex = grid_type == GT_ONINDEX;
+    L.rootHdr = L.isIndex ? 96u : 64u;
+    L.leafSize = L.isIndex ? 96u : 2144u;
+    L.offUpper = kRootOff + L.rootHdr + 32ull * tot.u;
+    L.offLower = L.offUpper + kUpperBytes * tot.u;
+    L.offLeaf = L.offLower + kLowerBytes * tot.m;
+    L.total = L.offLeaf + uint64_t(L.leafSize) * tot.l;
+    void* dbuf = nullptr;
+    if (hipMalloc(&dbuf, L.total) != hipSuccess) return NVQ_ERR_HIP;
+    uint8_t* buf = static_cast<uint8_t*>(dbuf);
+    std::vector<uint8_t> hdr(kRootOff + L.rootHdr, 0);
+    const uint32_t counts[4] = {tot.v, tot.l, tot.m, tot.u};
+    writeHeaders(hdr.data(), L, counts, box, voxel_size, name, grid_type);
+    uint32_t* leafFirst = S.get<uint32_t>(uint64_t(tot.l) + 1);
+    uint32_t* lowerFirst = S.get<uint32_t>(uint64_t(tot.m) + 1);
+    uint32_t* upperFirst = S.get<uint32_t>(uint64_t(tot.u) + 1);
+    int rc = NVQ_ERR_HIP;
+    if (leafFirst && lowerFirst && upperFirst && hipMemsetAsync(buf, 0, L.total, s) == hipSuccess &&
+        hipMemcpyAsync(buf, hdr.data(), hdr.size(), hipMemcpyHostToDevice, s) == hipSuccess) {
+        // 4. nodes, then leaves, then bboxes bottom-up
+        k_v2g_heads<<<blocksFor(n, kB), kB, 0, s>>>(keys2, ids2, cnt, n32, L, buf, leafFirst, lowerFirst, upperFirst, index);
+        k_v2g_leaves<<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, cnt, leafFirst, tot.l, L, buf);
+        k_v2g_lowers<<<blocksFor(tot.m, kWaves), kB, 0, s>>>(lowerFirst, tot.m, L, buf);
+        k_v2g_uppers<<<blocksFor(tot.u, kWaves), kB, 0, s>>>(upperFirst, tot.u, L, buf);
+        if (hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess)
+            rc = makeGridHandle(hdr.data(), L.total, dbuf, true, s, out);
+    }
+    if (rc != NVQ_OK) (void)hipFree(dbuf);
+    return rc;
+}
+
+int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void* stream)
+{
+    nvq_grid_info info;
+    if (!grid || !host_dst || nvq_grid_get_info(grid, &info) != NVQ_OK || bytes > info.bytes) return NVQ_ERR_INVALID_ARG;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(host_dst, info.device_buffer, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    return NVQ_OK;
+}
+
+} // extern "C"

@@ -56,11 +56,54 @@ class DeviceGrid:
         check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
         self.info = info
 
+    @classmethod
+    def from_voxels(cls, ijk, grid_type="OnIndex", voxel_size=1.0, name=None, return_index=False, stream=None):
+        """Build a grid on the device from integer voxels: tools::cuda::voxelsToGrid<BuildT>
+        (tools/cuda/PointsToGrid.cuh:119-125) for BuildT = ValueOnIndex ("OnIndex") or float.
+
+        ``ijk`` is an [n, 3] int32 CUDA tensor (duplicates allowed).  With ``return_index`` (index
+        grids) also returns an int64 CUDA tensor holding each input voxel's value index, so that
+        ``channel[index] = values`` fills a channel for ``set_channel``.  The library owns the new
+        grid's device buffer (``leaf_values`` is not available on it)."""
+        import torch
+        codes = {"OnIndex": 20, "float": 1}
+        if grid_type not in codes:
+            raise ValueError("grid_type must be 'OnIndex' or 'float'")
+        if not (ijk.is_cuda and ijk.dtype == torch.int32 and ijk.dim() == 2 and ijk.shape[1] == 3):
+            raise ValueError("ijk must be an [n, 3] int32 CUDA tensor")
+        ijk = ijk.contiguous()
+        n = ijk.shape[0]
+        index = torch.empty(n, dtype=torch.int64, device=ijk.device) if return_index else None
+        handle = ctypes.c_void_p()
+        check(lib().nvq_voxels_to_grid(ctypes.c_void_p(ijk.data_ptr()), n, codes[grid_type], float(voxel_size),
+                                       None if name is None else name.encode(),
+                                       None if index is None else ctypes.c_void_p(index.data_ptr()),
+                                       _stream_ptr(stream), ctypes.byref(handle)), "nvq_voxels_to_grid")
+        g = cls.__new__(cls)
+        g.handle = handle
+        g.device_bytes = None
+        info = GridInfo()
+        check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
+        g.info = info
+        g._host = g.download(stream, limit=1024)  # grid, tree and root headers
+        return (g, index) if return_index else g
+
+    def download(self, stream=None, limit=None):
+        """The grid buffer (or its first ``limit`` bytes) as a 32-byte aligned uint8 numpy array
+        (GridHandle::deviceDownload)."""
+        n = self.nbytes if limit is None else min(self.nbytes, int(limit))
+        out = aligned_buffer(np.zeros(n, np.uint8))
+        check(lib().nvq_grid_download(self.handle, out.ctypes.data_as(ctypes.c_void_p), n, _stream_ptr(stream)),
+              "nvq_grid_download")
+        return out
+
     def leaf_values(self):
         """[leafCount, 512] float32 view of the leaf values inside the device buffer (Float grids).
         Writing to it changes the grid; it is the parameter tensor of the differentiable march."""
         if self.info.grid_type != 1:
             raise ValueError("leaf_values() needs a Float grid")
+        if self.device_bytes is None:
+            raise ValueError("leaf_values() needs a grid uploaded from host memory")
         import torch
         leaf_off = 672 + int(np.frombuffer(self._host[672:680].tobytes(), np.int64)[0])
         f32 = self.device_bytes.view(torch.float32)

@@ -55,6 +55,14 @@ void or_fog_march(const void* buf, const float* eye, const float* dir, uint64_t 
 void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, uint64_t n, float dt, float sigma,
                           const double* dL, const double* dT, double* grad);
 
+/* tools/cuda/PointsToGrid.cuh voxelsToGrid<float | ValueOnIndex> restated on the CPU: the
+ * buffer (malloc'd, free with or_free) for n integer voxels ijk[n][3], duplicates allowed.
+ * grid_type 1 (float: active voxels 1, inactive 0) or 20 (ValueOnIndex).  out_index (index
+ * grids only, may be NULL) receives each input voxel's value index.  0 ok, -1 bad argument. */
+int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
+                      void** out, uint64_t* bytes_out, uint64_t* out_index);
+void or_free(void* p);
+
 /* Known-answer helpers */
 void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,
                    float* nexts, int32_t* voxels, uint8_t* ok);

@@ -57,6 +57,9 @@ class Oracle:
         L.or_fog_march_adjoint.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
         L.or_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
         L.or_ray_clip.argtypes = [_vp, _vp, _f, _f, _vp, _vp]
+        L.or_voxels_to_grid.argtypes = [_vp, _u64, ctypes.c_uint32, _d, ctypes.c_char_p, ctypes.POINTER(_vp),
+                                        ctypes.POINTER(_u64), _vp]
+        L.or_free.argtypes = [_vp]
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -156,6 +159,21 @@ class Oracle:
         self.L.or_fog_march_adjoint(_p(buf), _p(eye), _p(dir), n, dt, sigma, _p(dL), _p(dT), _p(grad))
         return grad
 
+    def voxels_to_grid(self, ijk, grid_type=20, voxel_size=1.0, name=None, want_index=False):
+        """voxelsToGrid<float | ValueOnIndex> restated: (buffer, per-input value index or None)."""
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        n = len(ijk)
+        index = np.zeros(n, np.uint64) if want_index else None
+        p, nb = _vp(), _u64()
+        st = self.L.or_voxels_to_grid(_p(ijk), n, grid_type, voxel_size, None if name is None else name.encode(),
+                                      ctypes.byref(p), ctypes.byref(nb), _p(index))
+        if st != 0:
+            raise ValueError("or_voxels_to_grid failed: %d" % st)
+        buf = aligned_empty(nb.value)
+        ctypes.memmove(buf.ctypes.data, p.value, nb.value)
+        self.L.or_free(p)
+        return buf, index
+
     def hdda_trace(self, eye, dir, t0, t1, dim, nsteps):
         eye = np.asarray(eye, np.float32)
         dir = np.asarray(dir, np.float32)
@@ -211,6 +229,10 @@ class Reference:
         L.ref_ray_gen.argtypes = [_i, _i, _f, _f, _f, _f, _vp, _vp]
         L.ref_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
         L.ref_ray_clip.argtypes = [_vp, _vp, _f, _f, _vp, _vp]
+        L.ref_validate.argtypes = [_vp, _i]
+        L.ref_grid_from_voxels.argtypes = [_vp, _u64, _i, _d, ctypes.POINTER(_u64)]
+        L.ref_grid_from_voxels.restype = _vp
+        L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp]
 
     def hw_threads(self):
         return int(self.L.ref_hw_threads())
@@ -313,6 +335,28 @@ class Reference:
     def index_grid(self, float_buf, channels=1, include_stats=True, include_tiles=True):
         n = ctypes.c_uint64()
         p = self.L.ref_index_grid(_p(float_buf), channels, int(include_stats), int(include_tiles), ctypes.byref(n))
+        return self._take(p, n.value)
+
+    def validate(self, buf, full=True):
+        """tools::isValid(grid, CheckMode::Full | Half): True when the reference accepts the buffer."""
+        st = self.L.ref_validate(_p(buf), int(full))
+        if st < 0:
+            raise ValueError("grid type not covered by the reference dispatch")
+        return st == 1
+
+    def index_values(self, buf, ijk):
+        """ReadAccessor<ValueOnIndex>::getValue per voxel (uint64 value indices)."""
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros(len(ijk), np.uint64)
+        if self.L.ref_index_values(_p(buf), _p(ijk), len(ijk), _p(out)) != 0:
+            raise ValueError("not an index grid")
+        return out
+
+    def grid_from_voxels(self, ijk, grid_type=1, voxel_size=1.0):
+        """build::Grid<float> with value 1 at each voxel, converted by createNanoGrid."""
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        n = ctypes.c_uint64()
+        p = self.L.ref_grid_from_voxels(_p(ijk), len(ijk), grid_type, voxel_size, ctypes.byref(n))
         return self._take(p, n.value)
 
     def write_nvdb(self, path, bufs, merge=False, raw=False):

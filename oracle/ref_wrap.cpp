@@ -26,6 +26,7 @@
 #include <nanovdb/tools/CreatePrimitives.h>
 #include <nanovdb/tools/CreateNanoGrid.h>
 #include <nanovdb/tools/GridBuilder.h>
+#include <nanovdb/tools/GridValidator.h>
 #include <nanovdb/io/IO.h>
 #include "common.h" // examples/ex_raytrace_level_set/common.h (RayGenOp)
 
@@ -390,6 +391,48 @@ void* ref_index_grid(const void* float_buf, uint32_t channels, int include_stats
         *bytes = 0;
         return nullptr;
     }
+}
+
+// tools::isValid(grid, CheckMode::Full | Half) (tools/GridValidator.h:57-169): 1 valid, 0 not,
+// -4 for a grid type the dispatch does not cover.
+int ref_validate(const void* buf, int full)
+{
+    return dispatch(buf, [&](auto* grid) {
+        return tools::isValid(grid, full ? CheckMode::Full : CheckMode::Half, false) ? 1 : 0;
+    });
+}
+
+// The host path for a voxel list: build::Grid<float>(0) with value 1 at every voxel, then
+// createNanoGrid to float (default statistics) or ValueOnIndex (no stats, no tiles), as
+// compareNodeOrdering in unittest/TestNanoVDB.cu builds the CPU grid it compares with
+// voxelsToGrid.
+void* ref_grid_from_voxels(const int32_t* ijk, uint64_t n, int grid_type, double voxel_size, uint64_t* bytes)
+{
+    try {
+        tools::build::Grid<float> src(0.0f);
+        src.setTransform(voxel_size);
+        auto acc = src.getAccessor();
+        for (uint64_t i = 0; i < n; ++i) acc.setValue(Coord(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]), 1.0f);
+        if (grid_type == 20) {
+            auto h = tools::createNanoGrid<tools::build::Grid<float>, ValueOnIndex, HostBuffer>(src, 0u, false, false, 0);
+            return copyOut(h, bytes);
+        }
+        auto h = tools::createNanoGrid<tools::build::Grid<float>, float, HostBuffer>(src);
+        return copyOut(h, bytes);
+    } catch (...) {
+        *bytes = 0;
+        return nullptr;
+    }
+}
+
+// ReadAccessor<ValueOnIndex>::getValue: the value index of each voxel of an index grid
+// (LeafData<ValueOnIndex>::getValue, NanoVDB.h:4147-4171).
+int ref_index_values(const void* buf, const int32_t* ijk, uint64_t n, uint64_t* out)
+{
+    if (reinterpret_cast<const GridData*>(buf)->mGridType != GridType::OnIndex) return -4;
+    auto acc = asGrid<ValueOnIndex>(buf)->getAccessor();
+    for (uint64_t i = 0; i < n; ++i) out[i] = acc.getValue(Coord(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]));
+    return 0;
 }
 
 // math::isoCrossing (math/HDDA.h:225-248) on world-space rays mapped by Ray::worldToIndexF.

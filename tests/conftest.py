@@ -73,3 +73,26 @@ def grid_bbox(buf):
     root = tree + off_root
     bb = np.frombuffer(buf[root:root + 24].tobytes(), np.int32)
     return bb[:3].astype(np.int64), bb[3:].astype(np.int64)
+
+
+def voxel_cases():
+    """Voxel lists for the device builder (voxelsToGrid): the reference's own test inputs
+    (unittest/TestNanoVDB.cu: one voxel (1,2,3); three voxels; the 125..145 cube; random voxels in
+    [-512, 512)) plus duplicates, negative coordinates and inputs spanning several upper tiles."""
+    r = rng(11)
+    cube = np.stack(np.meshgrid(*[np.arange(125, 146)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    rand = r.integers(-512, 512, (20000, 3))
+    dups = np.concatenate([rand[:3000], rand[:3000][::-1], rand[:500]])
+    wide = r.integers(-9000, 9000, (6000, 3))
+    shell = r.normal(size=(30000, 3))
+    shell = np.rint(shell / np.linalg.norm(shell, axis=1, keepdims=True) * r.uniform(198, 202, (30000, 1)) +
+                    np.array([-37.0, 11.0, 5.0]))
+    return {
+        "one": np.array([[1, 2, 3]]),
+        "three": np.array([[1, 2, 3], [1, 2, 4], [8, 2, 3]]),
+        "cube": cube,
+        "random": rand,
+        "duplicates": dups,
+        "wide": wide,
+        "shell": shell,
+    }

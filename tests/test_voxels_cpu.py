@@ -1,0 +1,117 @@
+"""Device-side construction (voxelsToGrid<float | ValueOnIndex>, tools/cuda/PointsToGrid.cuh):
+the oracle's restatement against the reference's host path and validator (CPU only).
+
+The reference's CUDA builder cannot be compiled here.  Its own test compareNodeOrdering
+(unittest/TestNanoVDB.cu) requires it to produce the node order, origins and masks of the host
+path build::Grid<float> + createNanoGrid; the oracle is pinned to that host path byte for byte,
+outside the fields the two builders fill differently (statistics, grid flags, checksum), and to
+tools::isValid(Full)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import voxel_cases
+from pyoracle import have_reference
+
+pytestmark = pytest.mark.skipif(not have_reference(), reason="reference sources not built")
+
+CASES = voxel_cases()
+
+
+def _u(b, off, dt, n=1):
+    return np.frombuffer(b[off:off + np.dtype(dt).itemsize * n].tobytes(), dt)
+
+
+def _host_only_fields(buf, grid_type):
+    """Mask of the bytes both builders must agree on: all but the checksum, the grid flags
+    (the host path also sets HasMinMax/HasAverage/HasStdDeviation) and, for float grids, the
+    node statistics the device builder leaves zero (BuildUpperNodesFunctor and friends)."""
+    keep = np.ones(len(buf), bool)
+    keep[8:16] = False
+    if grid_type == 20:
+        return keep
+    keep[20:24] = False
+    offs, cnt = _u(buf, 672, np.int64, 4), _u(buf, 704, np.uint32, 3)
+    root = 672 + offs[3]
+    keep[root + 32:root + 48] = False
+    for base, size, lo, hi, c in ((672 + offs[2], 270400, 8224, 8240, cnt[2]), (672 + offs[1], 33856, 1056, 1072, cnt[1]),
+                                  (672 + offs[0], 2144, 80, 96, cnt[0])):
+        for i in range(int(c)):
+            keep[base + size * i + lo:base + size * i + hi] = False
+    return keep
+
+
+@pytest.mark.parametrize("grid_type", [20, 1])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_oracle_matches_reference_host_build(oracle, ref, case, grid_type):
+    ijk = CASES[case]
+    ob, _ = oracle.voxels_to_grid(ijk, grid_type, 0.5)
+    rb = ref.grid_from_voxels(ijk, grid_type, 0.5)
+    assert len(ob) == len(rb)
+    keep = _host_only_fields(rb, grid_type)
+    bad = np.nonzero((ob != rb) & keep)[0]
+    assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
+    assert ref.validate(ob, full=True)
+    assert _u(ob, 8, np.uint64)[0] == np.uint64(2**64 - 1)  # checksum disabled, as the device builder writes
+
+
+def test_reference_kat_single_voxel(oracle, ref):
+    """Basic_CudaPointsToGrid_float (unittest/TestNanoVDB.cu): one voxel at (1,2,3)."""
+    ob, _ = oracle.voxels_to_grid([[1, 2, 3]], 1, 1.0)
+    assert len(ob) == 672 + 64 + 64 + 32 + 270400 + 33856 + 2144  # grid+tree+root(1)+upper+lower+leaf
+    val, act, _ = ref.get_value(ob, np.array([[0, 2, 3], [1, 2, 3], [1, 2, 4], [2, 2, 3]]))
+    assert list(act) == [0, 1, 0, 0]
+    assert val[1] == 1.0 and val[0] == 0.0
+    leaf = 672 + int(_u(ob, 672, np.int64)[0])
+    assert list(_u(ob, leaf, np.int32, 3)) == [1, 2, 3]  # bbox min
+    assert list(ob[leaf + 12:leaf + 15]) == [0, 0, 0]   # bbox max = min
+    assert _u(ob, 736 + 24, np.uint32)[0] == 1
+
+
+def test_index_of_each_input_voxel(oracle, ref):
+    """out_index[i] equals ReadAccessor<ValueOnIndex>::getValue(ijk[i]) of the built grid; indices
+    are 1..count, shared by duplicates, distinct otherwise."""
+    ijk = CASES["duplicates"]
+    ob, idx = oracle.voxels_to_grid(ijk, 20, 1.0, want_index=True)
+    np.testing.assert_array_equal(idx, ref.index_values(ob, ijk))
+    uniq, inv = np.unique(ijk, axis=0, return_inverse=True)
+    assert idx.min() == 1 and idx.max() == len(uniq) == _u(ob, 656, np.uint64)[0] - 1
+    first = np.full(len(uniq), -1, np.int64)
+    first[inv[::-1]] = np.arange(len(ijk))[::-1]
+    np.testing.assert_array_equal(idx, idx[first[inv.reshape(-1)]])
+
+
+def test_extreme_coordinates(oracle, ref):
+    """Voxels at the int32 limits: root keys of negative tiles and a 2^20-tile span."""
+    ijk = np.array([[-2**31, -2**31, -2**31], [2**31 - 1, 2**31 - 1, 2**31 - 1], [0, -1, 5], [-4097, 4096, 0]])
+    for gt in (20, 1):
+        ob, _ = oracle.voxels_to_grid(ijk, gt, 1.0)
+        rb = ref.grid_from_voxels(ijk, gt, 1.0)
+        assert not np.any((ob != rb) & _host_only_fields(rb, gt))
+        assert ref.validate(ob, full=True)
+
+
+def test_name_and_voxel_size(oracle, ref):
+    ob, _ = oracle.voxels_to_grid(CASES["three"], 20, 0.1, name="points")
+    assert bytes(ob[40:46]) == b"points"
+    rb = ref.grid_from_voxels(CASES["three"], 20, 0.1)
+    keep = _host_only_fields(rb, 20)
+    keep[40:296] = False  # the host build has no name here
+    assert not np.any((ob != rb) & keep)
+
+
+def test_c_abi_argument_errors():
+    """nvq_voxels_to_grid rejects bad arguments before it touches a device."""
+    import openvdb_amd._lib as L
+    lib = L.lib()
+    h = ctypes.c_void_p()
+    dummy = ctypes.c_void_p(64)
+    assert lib.nvq_voxels_to_grid(None, 4, 20, 1.0, None, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_voxels_to_grid(dummy, 0, 20, 1.0, None, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_voxels_to_grid(dummy, 2**31, 20, 1.0, None, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_voxels_to_grid(dummy, 4, 20, 0.0, None, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_voxels_to_grid(dummy, 4, 15, 1.0, None, None, None, ctypes.byref(h)) == -4
+    assert lib.nvq_voxels_to_grid(dummy, 4, 1, 1.0, None, dummy, None, ctypes.byref(h)) == -1
+    assert lib.nvq_voxels_to_grid(dummy, 4, 20, 1.0, None, None, None, None) == -1
+    assert h.value is None
