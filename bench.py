@@ -49,6 +49,7 @@ def parse():
     p.add_argument("--cpu-log2-points", type=int, default=21)
     p.add_argument("--no-diff", action="store_true", help="skip the differentiable-march workload")
     p.add_argument("--no-multi", action="store_true", help="skip the quantised multi-grid workload")
+    p.add_argument("--no-build", action="store_true", help="skip the device-side grid construction workload")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="also run one elementwise copy of MIB MiB (reads and writes MIB MiB) to scale HBM counters")
     return p.parse_args()
@@ -264,6 +265,70 @@ def multi_query_bench(torch, nvq, steps, total_log2=24):
     return out
 
 
+def fog_voxels(torch, nvq, fog, size, gen):
+    """Active voxels of the fog grid, found on the GPU by getValue over its dense lattice, shuffled."""
+    ax = torch.arange(size, dtype=torch.int32, device="cuda")
+    parts = []
+    for x0 in range(0, size, 32):
+        xs = ax[x0:x0 + 32]
+        lattice = torch.stack(torch.meshgrid(xs, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+        _, act = nvq.ops.get_value(fog, lattice)
+        parts.append(lattice[act.bool()])
+    ijk = torch.cat(parts)
+    return ijk[torch.randperm(len(ijk), device="cuda", generator=gen)].contiguous()
+
+
+def build_bench(torch, nvq, ijk, steps, warmup):
+    """Device-side construction (nvq_voxels_to_grid, index grid + per-voxel index) of the given
+    voxels: wall time per build including its allocations and two host reads, plus the stages."""
+    idx = torch.empty(len(ijk), dtype=torch.int64, device="cuda")
+    import ctypes
+    lib = nvq.lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def build():
+        h = ctypes.c_void_p()
+        st = lib.nvq_voxels_to_grid(ctypes.c_void_p(ijk.data_ptr()), len(ijk), 20, 1.0, None,
+                                    ctypes.c_void_p(idx.data_ptr()), stream, ctypes.byref(h))
+        if st != 0:
+            raise RuntimeError(f"nvq_voxels_to_grid failed: {st}")
+        build.last = h
+
+    def build_and_free():
+        build()
+        lib.nvq_grid_free(build.last)
+    wall, _ = timed_launches(torch, [build_and_free], steps, warmup)
+    stages = {k: kernel_ms(k)[0] / steps for k in ("v2g_sort", "v2g_scan", "k_v2g_heads", "k_v2g_leaves")}
+    from openvdb_amd._lib import GridInfo
+    build()
+    gi = GridInfo()
+    lib.nvq_grid_get_info(build.last, ctypes.byref(gi))
+    lib.nvq_grid_free(build.last)
+    return {"voxels": len(ijk), "ms": wall * 1e3 / steps, "voxels_per_s": len(ijk) * steps / wall,
+            "stages_ms": stages, "grid_bytes": int(gi.bytes), "leaves": int(gi.node_count[0])}
+
+
+def build_cpu_baseline(ijk_np, log2n):
+    """The reference host path for the same job (build::Grid<float>::setValue per voxel, then
+    createNanoGrid<ValueOnIndex>), on the first 2^log2n voxels, one thread, median of 3."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Reference, have_reference
+    if not have_reference():
+        return None
+    ref = Reference()
+    sample = np.ascontiguousarray(ijk_np[: 1 << log2n])
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ref.grid_from_voxels(sample, 20, 1.0)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    threads = max(1, ref.hw_threads() // 2)  # util::forEach's std::thread fallback (util/ForEach.h:47-48)
+    return {"value": len(sample) / t, "unit": "voxels/s", "cores": threads, "kind": "reference",
+            "sample": f"first 2^{log2n} of the shuffled fog voxels; build::Grid<float>::setValue (1 thread) + "
+                      f"createNanoGrid<ValueOnIndex>(no stats, no tiles; {threads} threads), g++ -O2, median of 3"}
+
+
 def cpu_baseline(grids_np, pts_np, log2n):
     """Reference host code on the host cores, bounded sample (2^log2n points per grid)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -433,6 +498,15 @@ def main():
                 f"ratio {extra['c4']['ratio']:.2f}")
         if not args.no_multi:
             extra["c5"] = multi_query_bench(torch, nvq, 3)
+        if not args.no_build:
+            vox = fog_voxels(torch, nvq, fog, args.fog_size, gen)
+            extra["voxels_to_grid"] = build_bench(torch, nvq, vox, 5, 2)
+            if not args.no_cpu:
+                extra["voxels_to_grid"]["cpu_baseline"] = build_cpu_baseline(vox.cpu().numpy(), 21)
+            b = extra["voxels_to_grid"]
+            log(f"voxelsToGrid: {b['voxels']/1e6:.1f} M voxels in {b['ms']:.1f} ms "
+                f"({b['voxels_per_s']/1e9:.2f} G voxels/s); stages {b['stages_ms']}")
+            del vox
 
     cpu = None
     if not args.no_cpu:

@@ -201,12 +201,23 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
  * (GridHandle.h:186-194). */
 int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void* stream);
 
+/* tools::cuda::updateGridStats(d_grid, StatsMode::Default) (tools/cuda/GridStats.cuh:343-362) for
+ * NVQ_FLOAT grids, in place on the device buffer: leaf bboxes (LeafNode::updateBBox), node and
+ * root bboxes, min / max / average / standard deviation of every node, render flags, the world
+ * bbox and the grid's statistics flags.  The merge order is the reference device code's (32
+ * lanes per leaf, 128 threads per internal node), so the bits are its bits; they can differ in
+ * the last place of an average from the host tools::updateGridStats.  Leaves without active
+ * voxels keep their statistics and add nothing to their parent.  Waits for the device (the
+ * grid's cached bbox is refreshed).  Other grid types: NVQ_ERR_UNSUPPORTED_TYPE. */
+int nvq_update_grid_stats(nvq_grid* grid, void* stream);
+
 /* Opt-in kernel timing for measurement tools (no counterpart in the reference).  While enabled,
  * each call records HIP events on its stream around its dominant kernel: "k_sample" (direct
  * point queries), "k_bin_sample" (binned point queries, n >= 2^20), "k_zero_crossing",
  * "k_fog_march", "k_fog_march_bwd", "k_multi_query".  nvq_profile_read waits for the recorded
  * launches and returns the summed milliseconds and launch count of one kernel name since the
- * last reset.  The iso-surface renderer records "k_iso_render" / "k_iso_render_persistent".
+ * last reset.  The iso-surface renderer records "k_iso_render" / "k_iso_render_persistent"; the
+ * device builder "v2g_sort", "v2g_scan" (library sort and scan), "k_v2g_heads", "k_v2g_leaves".
  * Process-wide; thread-safe. */
 int nvq_profile_enable(int enable);
 int nvq_profile_read(const char* kernel, double* total_ms, uint64_t* launches);

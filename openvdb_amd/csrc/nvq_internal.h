@@ -137,4 +137,15 @@ int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hi
 // (nvq_kernels.hip).
 int makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hipStream_t s, nvq_grid** out);
 
+// Where a grid's node arrays lie in its device buffer (offsets relative to the tree, i.e. to
+// base + 672), for the kernels that rewrite nodes in place.
+struct GridLayout {
+    uint8_t* base;
+    int64_t nodeOff[4]; // leaf, lower, upper, root
+    uint32_t count[3];  // leaf, lower, upper
+    uint32_t tableSize, type;
+};
+GridLayout gridLayout(const nvq_grid* g);
+int gridRefreshBBox(nvq_grid* g);
+
 } // namespace nvq

@@ -31,7 +31,33 @@ struct nvq_grid {
     nvq_grid_info info;
     const float* ownChannel; // index grids: the buffer's own float channel (blind data #0) or null
     uint64_t valueCount;     // index grids: GridData::mData1
+    int64_t nodeOff[4];      // TreeData::mNodeOffset: leaf, lower, upper, root (from the tree)
 };
+
+nvq::GridLayout nvq::gridLayout(const nvq_grid* g)
+{
+    GridLayout L;
+    L.base = static_cast<uint8_t*>(g->dbuf);
+    for (int i = 0; i < 4; ++i) L.nodeOff[i] = g->nodeOff[i];
+    for (int i = 0; i < 3; ++i) L.count[i] = g->info.node_count[i];
+    L.tableSize = g->info.root_tiles;
+    L.type = g->info.grid_type;
+    return L;
+}
+
+// The device buffer's root bbox may have changed (statistics recomputed on the device): re-read
+// it into the handle's view and info.  Waits for the device first, as queries may use the view.
+int nvq::gridRefreshBBox(nvq_grid* g)
+{
+    int32_t box[6];
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(box, static_cast<const uint8_t*>(g->dbuf) + kGridSize + g->nodeOff[3] + kRootBBox, sizeof(box),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return NVQ_ERR_HIP;
+    for (int i = 0; i < 6; ++i) g->view.bbox[i] = g->info.index_bbox[i] = box[i];
+    if (hipMemcpy(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice) != hipSuccess) return NVQ_ERR_HIP;
+    return NVQ_OK;
+}
 
 // A grid can be queried when its values are reachable: value grids always, index grids once they
 // have a float channel.
@@ -321,6 +347,7 @@ int nvq::makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hi
     g->device = device;
     const uint8_t* tree = b + kGridSize;
     const int64_t offLeaf = rd<int64_t>(tree), offRoot = rd<int64_t>(tree + 24);
+    for (int i = 0; i < 4; ++i) g->nodeOff[i] = rd<int64_t>(tree + 8 * i);
     const uint8_t* root = tree + offRoot;
     const uint8_t* dbase = static_cast<const uint8_t*>(d);
     GridView& v = g->view;

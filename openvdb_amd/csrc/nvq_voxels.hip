@@ -12,6 +12,7 @@
 // between the levels.
 #include "nvq_device.h"
 #include "nvq_internal.h"
+#include "nvq_profile.h"
 #include "../../include/nvq.h"
 
 #include <hip/hip_runtime.h>
@@ -459,10 +460,17 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
         return NVQ_ERR_HIP;
     void* temp = S.get<uint8_t>(std::max(sortBytes, scanBytes));
     if (!temp) return NVQ_ERR_HIP;
-    if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, ids, ids2, int(n32), 0, 36 + tileBits, s) !=
-            hipSuccess ||
-        hipcub::DeviceScan::InclusiveScan(temp, scanBytes, heads, cnt, CntSum(), int(n32), s) != hipSuccess)
-        return NVQ_ERR_HIP;
+    {
+        ProfScope prof("v2g_sort", s);
+        if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, ids, ids2, int(n32), 0, 36 + tileBits, s) !=
+            hipSuccess)
+            return NVQ_ERR_HIP;
+    }
+    {
+        ProfScope prof("v2g_scan", s);
+        if (hipcub::DeviceScan::InclusiveScan(temp, scanBytes, heads, cnt, CntSum(), int(n32), s) != hipSuccess)
+            return NVQ_ERR_HIP;
+    }
     Cnt tot;
     if (hipMemcpyAsync(&tot, cnt + (n32 - 1), sizeof(Cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
@@ -489,8 +497,15 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
     if (leafFirst && lowerFirst && upperFirst && hipMemsetAsync(buf, 0, L.total, s) == hipSuccess &&
         hipMemcpyAsync(buf, hdr.data(), hdr.size(), hipMemcpyHostToDevice, s) == hipSuccess) {
         // 4. nodes, then leaves, then bboxes bottom-up
-        k_v2g_heads<<<blocksFor(n, kB), kB, 0, s>>>(keys2, ids2, cnt, n32, L, buf, leafFirst, lowerFirst, upperFirst, index);
-        k_v2g_leaves<<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, cnt, leafFirst, tot.l, L, buf);
+        {
+            ProfScope prof("k_v2g_heads", s);
+            k_v2g_heads<<<blocksFor(n, kB), kB, 0, s>>>(keys2, ids2, cnt, n32, L, buf, leafFirst, lowerFirst, upperFirst,
+                                                        index);
+        }
+        {
+            ProfScope prof("k_v2g_leaves", s);
+            k_v2g_leaves<<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, cnt, leafFirst, tot.l, L, buf);
+        }
         k_v2g_lowers<<<blocksFor(tot.m, kWaves), kB, 0, s>>>(lowerFirst, tot.m, L, buf);
         k_v2g_uppers<<<blocksFor(tot.u, kWaves), kB, 0, s>>>(upperFirst, tot.u, L, buf);
         if (hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess)

@@ -97,6 +97,14 @@ class DeviceGrid:
               "nvq_grid_download")
         return out
 
+    def update_stats(self, stream=None):
+        """Recompute bboxes and statistics on the device (tools::cuda::updateGridStats, float grids),
+        e.g. after the leaf values were changed through ``leaf_values``."""
+        check(lib().nvq_update_grid_stats(self.handle, _stream_ptr(stream)), "nvq_update_grid_stats")
+        info = GridInfo()
+        check(lib().nvq_grid_get_info(self.handle, ctypes.byref(info)), "nvq_grid_get_info")
+        self.info = info
+
     def leaf_values(self):
         """[leafCount, 512] float32 view of the leaf values inside the device buffer (Float grids).
         Writing to it changes the grid; it is the parameter tensor of the differentiable march."""

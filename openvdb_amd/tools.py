@@ -120,3 +120,23 @@ def write_nvdb(path, grid):
     buf = np.ascontiguousarray(grid)
     check(lib().nvqb_write_nvdb(str(path).encode(), buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes),
           "nvqb_write_nvdb")
+
+
+def active_voxels(grid):
+    """(ijk [n, 3] int32, values [n] float32) of the active voxels inside the leaves of a float grid
+    buffer, in leaf order then bit order (LeafNode::beginValueOn, NanoVDB.h).  Active tiles of the
+    internal nodes are not expanded."""
+    buf = np.ascontiguousarray(grid).reshape(-1).view(np.uint8)
+    if int(np.frombuffer(buf[636:640].tobytes(), np.uint32)[0]) != 1:
+        raise ValueError("active_voxels needs a float grid")
+    tree = 672
+    off = int(np.frombuffer(buf[tree:tree + 8].tobytes(), np.int64)[0])
+    count = int(np.frombuffer(buf[tree + 32:tree + 36].tobytes(), np.uint32)[0])
+    leaves = buf[tree + off:tree + off + 2144 * count].reshape(count, 2144)
+    origin = leaves[:, 0:12].copy().view(np.int32) & ~7
+    mask = np.unpackbits(leaves[:, 16:80], axis=1, bitorder="little").astype(bool)
+    n = np.arange(512)
+    local = np.stack([n >> 6, (n >> 3) & 7, n & 7], 1).astype(np.int32)
+    li, vi = np.nonzero(mask)
+    values = np.ascontiguousarray(leaves[:, 96:]).view(np.float32)[li, vi]
+    return (origin[li] + local[vi]).astype(np.int32), values

@@ -62,6 +62,9 @@ void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, u
 int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
                       void** out, uint64_t* bytes_out, uint64_t* out_index);
 void or_free(void* p);
+/* tools/cuda/GridStats.cuh updateGridStats<float>(StatsMode::Default) restated, in place on a host
+ * float grid buffer (bboxes, node statistics, flags, world bbox).  0 ok, -4 not a float grid. */
+int or_update_grid_stats(void* buf);
 
 /* Known-answer helpers */
 void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,

@@ -60,6 +60,7 @@ class Oracle:
         L.or_voxels_to_grid.argtypes = [_vp, _u64, ctypes.c_uint32, _d, ctypes.c_char_p, ctypes.POINTER(_vp),
                                         ctypes.POINTER(_u64), _vp]
         L.or_free.argtypes = [_vp]
+        L.or_update_grid_stats.argtypes = [_vp]
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -174,6 +175,15 @@ class Oracle:
         self.L.or_free(p)
         return buf, index
 
+    def update_grid_stats(self, buf):
+        """cuda::updateGridStats<float> restated: a copy of the float grid buffer with its bboxes,
+        statistics and flags recomputed in the device code's merge order."""
+        out = aligned_empty(buf.nbytes)
+        out[:] = buf
+        if self.L.or_update_grid_stats(_p(out)) != 0:
+            raise ValueError("not a float grid")
+        return out
+
     def hdda_trace(self, eye, dir, t0, t1, dim, nsteps):
         eye = np.asarray(eye, np.float32)
         dir = np.asarray(dir, np.float32)
@@ -233,6 +243,7 @@ class Reference:
         L.ref_grid_from_voxels.argtypes = [_vp, _u64, _i, _d, ctypes.POINTER(_u64)]
         L.ref_grid_from_voxels.restype = _vp
         L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp]
+        L.ref_update_grid_stats.argtypes = [_vp]
 
     def hw_threads(self):
         return int(self.L.ref_hw_threads())
@@ -343,6 +354,14 @@ class Reference:
         if st < 0:
             raise ValueError("grid type not covered by the reference dispatch")
         return st == 1
+
+    def update_grid_stats(self, buf):
+        """Host tools::updateGridStats(StatsMode::Default) on a copy of a float grid buffer."""
+        out = aligned_empty(buf.nbytes)
+        out[:] = buf
+        if self.L.ref_update_grid_stats(_p(out)) != 0:
+            raise ValueError("not a float grid")
+        return out
 
     def index_values(self, buf, ijk):
         """ReadAccessor<ValueOnIndex>::getValue per voxel (uint64 value indices)."""

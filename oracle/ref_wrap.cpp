@@ -27,6 +27,7 @@
 #include <nanovdb/tools/CreateNanoGrid.h>
 #include <nanovdb/tools/GridBuilder.h>
 #include <nanovdb/tools/GridValidator.h>
+#include <nanovdb/tools/GridStats.h>
 #include <nanovdb/io/IO.h>
 #include "common.h" // examples/ex_raytrace_level_set/common.h (RayGenOp)
 
@@ -423,6 +424,15 @@ void* ref_grid_from_voxels(const int32_t* ijk, uint64_t n, int grid_type, double
         *bytes = 0;
         return nullptr;
     }
+}
+
+// Host tools::updateGridStats(grid, StatsMode::Default) in place on a float grid
+// (tools/GridStats.h): bboxes and statistics in the host's merge order.
+int ref_update_grid_stats(void* buf)
+{
+    if (reinterpret_cast<const GridData*>(buf)->mGridType != GridType::Float) return -4;
+    tools::updateGridStats(reinterpret_cast<NanoGrid<float>*>(buf), tools::StatsMode::Default);
+    return 0;
 }
 
 // ReadAccessor<ValueOnIndex>::getValue: the value index of each voxel of an index grid

@@ -96,3 +96,17 @@ def voxel_cases():
         "wide": wide,
         "shell": shell,
     }
+
+
+def stats_grids(ref, oracle):
+    """Float grids for the statistics tests: reference-built level set and fog with active tiles,
+    our builder's torus and fBm fog, and a device-builder grid (statistics zero)."""
+    import openvdb_amd as nvq
+    v2g, _ = oracle.voxels_to_grid(voxel_cases()["shell"], 1, 0.5)
+    return {
+        "sphere": ref.level_set_sphere(40.0, (1.5, -2.0, 3.0), 0.5, 3.0),
+        "fog_sphere_tiles": ref.fog_volume_sphere(30.0, (0, 0, 0), 1.0, 3.0),
+        "torus": nvq.tools.create_level_set_torus(60.0, 15.0, (0.0, 0.0, 0.0), 1.0, 3.0),
+        "fbm": nvq.tools.create_fog_fbm([0, 0, 0, 63, 63, 63], seed=3),
+        "voxels_to_grid": v2g,
+    }

@@ -1,0 +1,78 @@
+"""Device-side statistics (nvq_update_grid_stats = cuda::updateGridStats<float>) on the GPU: the
+buffer after the update equals the oracle's restatement byte for byte."""
+import numpy as np
+import pytest
+
+from conftest import rng, stats_grids, voxel_cases
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _nvq():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import openvdb_amd
+    return openvdb_amd
+
+
+def _u(b, off, dt, n=1):
+    return np.frombuffer(b[off:off + np.dtype(dt).itemsize * n].tobytes(), dt)
+
+
+def _zero_stats(g):
+    d = g.copy()
+    offs, cnt = _u(g, 672, np.int64, 4), _u(g, 704, np.uint32, 3)
+    root = 672 + offs[3]
+    d[root + 32:root + 48] = 0
+    for base, size, lo, c in ((672 + offs[2], 270400, 8224, cnt[2]), (672 + offs[1], 33856, 1056, cnt[1]),
+                              (672 + offs[0], 2144, 80, cnt[0])):
+        for i in range(int(c)):
+            d[base + size * i + lo:base + size * i + lo + 16] = 0
+    d[20] &= ~0x1e & 0xff
+    return d
+
+
+@pytest.mark.parametrize("name", ["sphere", "fog_sphere_tiles", "torus", "fbm", "voxels_to_grid"])
+def test_update_stats_matches_oracle(ref, oracle, name):
+    nvq = _nvq()
+    g = _zero_stats(stats_grids(ref, oracle)[name])
+    print(f"[update_grid_stats] grid={name} bytes={g.nbytes}", flush=True)
+    dg = nvq.DeviceGrid(g)
+    dg.update_stats()
+    got = dg.download()
+    want = oracle.update_grid_stats(g)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
+    assert tuple(dg.index_bbox) == tuple(_u(want, 672 + int(_u(want, 696, np.int64)[0]), np.int32, 6))
+
+
+def test_update_stats_after_value_edit(oracle):
+    """Change leaf values through the leaf_values view, then recompute on the device."""
+    nvq = _nvq()
+    g = nvq.tools.create_fog_fbm([0, 0, 0, 95, 95, 95], seed=5)
+    dg = nvq.DeviceGrid(g)
+    lv = dg.leaf_values()
+    noise = torch.from_numpy(rng(7).normal(size=tuple(lv.shape)).astype(np.float32)).cuda()
+    lv.mul_(1.5).add_(noise)
+    torch.cuda.synchronize()
+    before = dg.download()
+    dg.update_stats()
+    np.testing.assert_array_equal(dg.download(), oracle.update_grid_stats(before))
+
+
+def test_device_built_grid_then_stats(oracle):
+    """voxelsToGrid<float> then updateGridStats, both on the device."""
+    nvq = _nvq()
+    ijk = voxel_cases()["random"]
+    g = nvq.DeviceGrid.from_voxels(torch.from_numpy(ijk.astype(np.int32)).cuda(), "float", 0.25)
+    g.update_stats()
+    ob, _ = oracle.voxels_to_grid(ijk, 1, 0.25)
+    np.testing.assert_array_equal(g.download(), oracle.update_grid_stats(ob))
+
+
+def test_update_stats_rejects_other_types(ref):
+    nvq = _nvq()
+    q = nvq.DeviceGrid(ref.quantize(ref.level_set_sphere(20.0, (0, 0, 0), 1.0, 3.0), 3))  # Fp16
+    with pytest.raises(nvq.NvqError):
+        q.update_stats()
