@@ -211,6 +211,16 @@ int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void
  * grid's cached bbox is refreshed).  Other grid types: NVQ_ERR_UNSUPPORTED_TYPE. */
 int nvq_update_grid_stats(nvq_grid* grid, void* stream);
 
+/* tools::cuda::signedFloodFill(d_grid) (tools/cuda/SignedFloodFill.cuh:240-253) for NVQ_FLOAT level
+ * sets, in place on the device buffer: every inactive leaf voxel and every non-child node entry
+ * becomes +-background with the sign of its nearest active neighbour in index order (next, else
+ * previous), and inactive root tiles between two inside children on a z-scanline become
+ * -background.  This is the reference device algorithm, which differs from the host
+ * build::Grid fill on leaves and nodes that straddle the surface.  NVQ_ERR_INVALID_ARG when a root
+ * tile the fill must write is missing (the reference throws); NVQ_ERR_UNSUPPORTED_TYPE for other
+ * grid types.  A full checksum is not recomputed.  Waits for `stream`. */
+int nvq_signed_flood_fill(nvq_grid* grid, void* stream);
+
 /* Opt-in kernel timing for measurement tools (no counterpart in the reference).  While enabled,
  * each call records HIP events on its stream around its dominant kernel: "k_sample" (direct
  * point queries), "k_bin_sample" (binned point queries, n >= 2^20), "k_zero_crossing",

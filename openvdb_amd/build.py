@@ -15,7 +15,8 @@ LIB = os.path.join(HERE, "libnvq.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["nvq_kernels.hip", "nvq_binned.hip", "nvq_adjoint.hip", "nvq_multi.hip", "nvq_profile.hip", "nvq_iso.hip",
-               "nvq_voxels.hip", "nvq_stats.hip"]
+               "nvq_voxels.hip", "nvq_stats.hip",
+               "nvq_flood.hip"]
 HOST_SOURCES = ["nvq_build.cpp", "nvq_io.cpp"]  # host-only C++ (grid builder), compiled by g++
 OBJ_DIR = os.path.join(HERE, "_build")
 HOST_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-pthread", "-Wall", "-Wextra",

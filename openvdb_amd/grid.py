@@ -105,6 +105,11 @@ class DeviceGrid:
         check(lib().nvq_grid_get_info(self.handle, ctypes.byref(info)), "nvq_grid_get_info")
         self.info = info
 
+    def signed_flood_fill(self, stream=None):
+        """Re-sign the inactive values of a float level set on the device
+        (tools::cuda::signedFloodFill)."""
+        check(lib().nvq_signed_flood_fill(self.handle, _stream_ptr(stream)), "nvq_signed_flood_fill")
+
     def leaf_values(self):
         """[leafCount, 512] float32 view of the leaf values inside the device buffer (Float grids).
         Writing to it changes the grid; it is the parameter tensor of the differentiable march."""

@@ -65,6 +65,9 @@ void or_free(void* p);
 /* tools/cuda/GridStats.cuh updateGridStats<float>(StatsMode::Default) restated, in place on a host
  * float grid buffer (bboxes, node statistics, flags, world bbox).  0 ok, -4 not a float grid. */
 int or_update_grid_stats(void* buf);
+/* tools/cuda/SignedFloodFill.cuh signedFloodFill<float> restated, in place on a host float grid
+ * buffer.  0 ok, -4 not a float grid, -3 a root tile the fill must write is missing. */
+int or_signed_flood_fill(void* buf);
 
 /* Known-answer helpers */
 void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,

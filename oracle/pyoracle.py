@@ -61,6 +61,7 @@ class Oracle:
                                         ctypes.POINTER(_u64), _vp]
         L.or_free.argtypes = [_vp]
         L.or_update_grid_stats.argtypes = [_vp]
+        L.or_signed_flood_fill.argtypes = [_vp]
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -182,6 +183,15 @@ class Oracle:
         out[:] = buf
         if self.L.or_update_grid_stats(_p(out)) != 0:
             raise ValueError("not a float grid")
+        return out
+
+    def signed_flood_fill(self, buf):
+        """cuda::signedFloodFill<float> restated, on a copy of the float grid buffer."""
+        out = aligned_empty(buf.nbytes)
+        out[:] = buf
+        st = self.L.or_signed_flood_fill(_p(out))
+        if st != 0:
+            raise ValueError("or_signed_flood_fill failed: %d" % st)
         return out
 
     def hdda_trace(self, eye, dir, t0, t1, dim, nsteps):

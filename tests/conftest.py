@@ -110,3 +110,76 @@ def stats_grids(ref, oracle):
         "fbm": nvq.tools.create_fog_fbm([0, 0, 0, 63, 63, 63], seed=3),
         "voxels_to_grid": v2g,
     }
+
+
+def _u(b, off, dt, n=1):
+    return np.frombuffer(b[off:off + np.dtype(dt).itemsize * n].tobytes(), dt)
+
+
+def damage_inactive(g, value=None):
+    """Copy of a float grid whose inactive leaf values and non-child node entries are all set to
+    `value` (default: +background), as a level set looks before its signed flood fill."""
+    d = g.copy()
+    offs, cnt = _u(g, 672, np.int64, 4), _u(g, 704, np.uint32, 3)
+    v = np.float32(_u(g, 672 + offs[3] + 28, np.float32)[0] if value is None else value)
+    lf = 672 + offs[0]
+    leaves = d[lf:lf + 2144 * int(cnt[0])].reshape(-1, 2144)
+    mask = np.unpackbits(leaves[:, 16:80], axis=1, bitorder="little").astype(bool)
+    vals = leaves[:, 96:].copy().view(np.float32)
+    vals[~mask] = v
+    leaves[:, 96:] = vals.view(np.uint8)
+    bits = np.frombuffer(v.tobytes(), np.uint32)[0]
+    for lvl, size, cm, tab, n in ((1, 33856, 544, 1088, 4096), (2, 270400, 4128, 8256, 32768)):
+        base = 672 + offs[lvl]
+        for k in range(int(cnt[lvl])):
+            node = d[base + size * k:base + size * (k + 1)]
+            child = np.unpackbits(node[cm:cm + n // 8], bitorder="little").astype(bool)
+            table = node[tab:tab + 8 * n].copy().view(np.uint64)
+            table[~child] = np.uint64(bits)
+            node[tab:tab + 8 * n] = table.view(np.uint8)
+    return d
+
+
+def root_gap_grid(oracle, missing=False, background=3.0):
+    """A float level set (all active values -1, background 3) with two root children on the
+    z-scanline x = y = 0 (tiles z = 0 and z = 12288) and the inactive root tiles between them
+    (z = 4096, 8192; only z = 4096 when `missing`) holding +background: the case
+    SignedFloodFill's root pass repairs (tools/cuda/SignedFloodFill.cuh:91-145)."""
+    cube = np.stack(np.meshgrid(np.arange(0, 20), np.arange(0, 20), np.arange(0, 20), indexing="ij"), -1).reshape(-1, 3)
+    ijk = np.concatenate([cube + [0, 0, 4070], cube + [0, 0, 12288]])
+    g, _ = oracle.voxels_to_grid(ijk, 1, 1.0)
+    offs, cnt = _u(g, 672, np.int64, 4), _u(g, 704, np.uint32, 3)
+    lf = 672 + offs[0]
+    leaves = g[lf:lf + 2144 * int(cnt[0])].reshape(-1, 2144)
+    mask = np.unpackbits(leaves[:, 16:80], axis=1, bitorder="little").astype(bool)
+    vals = leaves[:, 96:].copy().view(np.float32)
+    vals[mask] = -1.0
+    leaves[:, 96:] = vals.view(np.uint8)
+    root = 672 + int(offs[3])
+    table = root + 64 + 32 * int(_u(g, root + 24, np.uint32)[0])
+    zs = [4096] if missing else [4096, 8192]
+    add = np.zeros(32 * len(zs), np.uint8)
+    for i, z in enumerate(zs):
+        key = np.uint64(z >> 12)  # x = y = 0
+        add[32 * i:32 * i + 8] = np.frombuffer(key.tobytes(), np.uint8)
+        add[32 * i + 20:32 * i + 24] = np.frombuffer(np.float32(background).tobytes(), np.uint8)
+    out = np.concatenate([g[:table], add, g[table:]])
+    shift = len(add)
+    raw = np.empty(len(out) + 32, np.uint8)
+    o = (-raw.ctypes.data) % 32
+    buf = raw[o:o + len(out)]
+    buf[:] = out
+
+    def put(off, val):
+        buf[off:off + val.nbytes] = np.frombuffer(val.tobytes(), np.uint8)
+    put(32, np.uint64(len(buf)))
+    put(640, np.int64(len(buf)))
+    for i in range(3):
+        put(672 + 8 * i, np.int64(offs[i] + shift))
+    put(root + 24, np.uint32(int(_u(g, root + 24, np.uint32)[0]) + len(zs)))
+    put(root + 28, np.float32(background))
+    for t in range(int(_u(buf, root + 24, np.uint32)[0])):
+        child = int(_u(buf, root + 64 + 32 * t + 8, np.int64)[0])
+        if child:
+            put(root + 64 + 32 * t + 8, np.int64(child + shift))
+    return buf
