@@ -308,6 +308,43 @@ def build_bench(torch, nvq, ijk, steps, warmup):
             "stages_ms": stages, "grid_bytes": int(gi.bytes), "leaves": int(gi.node_count[0])}
 
 
+def inplace_bench(torch, nvq, host_grid, fn_name, steps, warmup):
+    """Time one in-place device tool (nvq_update_grid_stats / nvq_signed_flood_fill) on a device
+    copy of host_grid: wall ms per call (each call waits for its stream) and grid GB/s."""
+    import ctypes
+    g = nvq.DeviceGrid(host_grid)
+    fn = getattr(nvq.lib(), fn_name)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def call():
+        st = fn(g.handle, stream)
+        if st != 0:
+            raise RuntimeError(f"{fn_name} failed: {st}")
+    wall, _ = timed_launches(torch, [call], steps, warmup)
+    ms = wall * 1e3 / steps
+    return {"ms": ms, "grid_bytes": int(host_grid.nbytes), "grid_gb_per_s": host_grid.nbytes / (ms * 1e6)}
+
+
+def inplace_cpu_baselines(fog_np, torus_np):
+    """Host counterparts: the reference's tools::updateGridStats (its util::forEach threads) on
+    the fog, and the oracle's single-thread port of the device flood fill on the torus."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, Reference, have_reference
+    out = {}
+    if have_reference():
+        ref = Reference()
+        t0 = time.perf_counter()
+        ref.update_grid_stats(fog_np)
+        out["grid_stats"] = {"ms": (time.perf_counter() - t0) * 1e3, "kind": "reference",
+                             "cores": max(1, ref.hw_threads() // 2),
+                             "sample": "tools::updateGridStats(StatsMode::Default) on the same fog grid"}
+    t0 = time.perf_counter()
+    Oracle().signed_flood_fill(torus_np)
+    out["signed_flood_fill"] = {"ms": (time.perf_counter() - t0) * 1e3, "kind": "port", "cores": 1,
+                                "sample": "oracle C port of cuda::signedFloodFill on the same torus"}
+    return out
+
+
 def build_cpu_baseline(ijk_np, log2n):
     """The reference host path for the same job (build::Grid<float>::setValue per voxel, then
     createNanoGrid<ValueOnIndex>), on the first 2^log2n voxels, one thread, median of 3."""
@@ -507,6 +544,13 @@ def main():
             log(f"voxelsToGrid: {b['voxels']/1e6:.1f} M voxels in {b['ms']:.1f} ms "
                 f"({b['voxels_per_s']/1e9:.2f} G voxels/s); stages {b['stages_ms']}")
             del vox
+            extra["grid_stats"] = inplace_bench(torch, nvq, fog_np, "nvq_update_grid_stats", 5, 1)
+            extra["signed_flood_fill"] = inplace_bench(torch, nvq, torus_np, "nvq_signed_flood_fill", 5, 1)
+            if not args.no_cpu:
+                for k, v in inplace_cpu_baselines(fog_np, torus_np).items():
+                    extra[k]["cpu_baseline"] = v
+            log(f"updateGridStats fog: {extra['grid_stats']['ms']:.2f} ms; "
+                f"signedFloodFill torus: {extra['signed_flood_fill']['ms']:.2f} ms")
 
     cpu = None
     if not args.no_cpu:
