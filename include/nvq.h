@@ -193,7 +193,9 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
  * receives each input voxel's value index, so that channel[index[i]] = value[i] fills a float
  * channel for nvq_grid_set_channel.  The new grid owns its device buffer.  The input may span at
  * most 2^28 upper-node tiles (4096^3 voxels each) in its bbox, else NVQ_ERR_INVALID_ARG.
- * Synchronises `stream` twice (to size the buffer) and allocates scratch of about 40 B per voxel. */
+ * Waits for `stream` (twice to size the buffer, and before returning).  Scratch of about 16 B
+ * (inputs within one upper node) or 24 B per voxel, plus the sort's temporaries, is kept per device
+ * between calls and grows to the largest build; builds on one device run one at a time. */
 int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
                        uint64_t* index, void* stream, nvq_grid** out);
 /* Copy the first `bytes` bytes (at most nvq_grid_info.bytes; pass that for the whole grid) of a

@@ -73,18 +73,6 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t m, uint32_t v)
     if (i < m) p[i] = v;
 }
 
-// Keep freed stream-ordered allocations in the device's default pool instead of returning them
-// to the driver at every synchronisation, so repeated large queries do not re-map memory.
-void keepPoolMemory()
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
-    uint64_t thr = ~uint64_t(0);
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-}
-
 __device__ __forceinline__ void loadPoint(const GridView& g, const float* xyz, uint64_t i, int world, float p[3])
 {
     p[0] = xyz[3 * i];

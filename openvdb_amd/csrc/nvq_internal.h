@@ -133,9 +133,22 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
                      int shift, hipStream_t s);
 int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hipStream_t s);
 
+// Keep freed stream-ordered allocations in the device's default pool instead of returning them
+// to the driver at every synchronisation, so repeated large calls do not re-map memory.
+inline void keepPoolMemory()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
+    uint64_t thr = ~uint64_t(0);
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+}
+
 // Handle for a grid buffer of `bytes` at device address d, its headers read from the host copy b
-// (nvq_kernels.hip).
-int makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hipStream_t s, nvq_grid** out);
+// (nvq_kernels.hip).  `owner` says how nvq_grid_free releases d.
+enum GridOwner { kOwnerCaller = 0, kOwnerMalloc = 1, kOwnerPool = 2 };
+int makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, int owner, hipStream_t s, nvq_grid** out);
 
 // Where a grid's node arrays lie in its device buffer (offsets relative to the tree, i.e. to
 // base + 672), for the kernels that rewrite nodes in place.

@@ -26,7 +26,7 @@ struct nvq_grid {
     GridView* dview; // device copy of `view` (persistent multi-grid queries)
     uint64_t bytes;
     int device;
-    bool owns; // false when the device memory belongs to the caller (nvq_grid_upload_into)
+    int owner; // GridOwner: who frees dbuf (the caller for nvq_grid_upload_into)
     GridView view;
     nvq_grid_info info;
     const float* ownChannel; // index grids: the buffer's own float channel (blind data #0) or null
@@ -334,13 +334,13 @@ const char* nvq_status_string(int status)
 // Build the handle of a grid whose `bytes`-long buffer already is (or is being copied) at device
 // address d.  `b` is a host copy of at least its grid, tree and root headers, or of the whole
 // buffer when an index grid's blind channel must be located.  On failure nothing is freed.
-int nvq::makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hipStream_t stream, nvq_grid** out)
+int nvq::makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, int owner, hipStream_t stream, nvq_grid** out)
 {
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
     nvq_grid* g = new (std::nothrow) nvq_grid;
     if (!g) return NVQ_ERR_HIP;
-    g->owns = owns;
+    g->owner = owner;
     g->dbuf = d;
     g->dview = nullptr;
     g->bytes = bytes;
@@ -387,9 +387,10 @@ int nvq::makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, bool owns, hi
     for (int i = 0; i < 3; ++i) in.voxel_size[i] = v.voxelSize[i];
     in.background = v.background;
     in.device_buffer = d;
-    if (hipMalloc(reinterpret_cast<void**>(&g->dview), sizeof(GridView)) != hipSuccess ||
+    keepPoolMemory();
+    if (hipMallocAsync(reinterpret_cast<void**>(&g->dview), sizeof(GridView), stream) != hipSuccess ||
         hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, stream) != hipSuccess) {
-        if (g->dview) (void)hipFree(g->dview);
+        if (g->dview) (void)hipFreeAsync(g->dview, stream);
         delete g;
         return NVQ_ERR_HIP;
     }
@@ -410,7 +411,7 @@ static int uploadImpl(const void* host_buf, uint64_t bytes, void* device_dst, vo
     if (!d && hipMalloc(&d, bytes) != hipSuccess) return NVQ_ERR_HIP;
     int rc = NVQ_ERR_HIP;
     if (hipMemcpyAsync(d, host_buf, bytes, hipMemcpyHostToDevice, S(stream)) == hipSuccess)
-        rc = makeGridHandle(b, bytes, d, device_dst == nullptr, S(stream), out);
+        rc = makeGridHandle(b, bytes, d, device_dst ? kOwnerCaller : kOwnerMalloc, S(stream), out);
     if (rc != NVQ_OK && !device_dst) (void)hipFree(d);
     return rc;
 }
@@ -430,8 +431,10 @@ int nvq_grid_free(nvq_grid* grid)
 {
     if (!grid) return NVQ_ERR_INVALID_ARG;
     const hipError_t e1 = hipDeviceSynchronize();
-    const hipError_t e2 = grid->owns ? hipFree(grid->dbuf) : hipSuccess;
-    if (grid->dview) (void)hipFree(grid->dview);
+    hipError_t e2 = hipSuccess;
+    if (grid->owner == kOwnerMalloc) e2 = hipFree(grid->dbuf);
+    if (grid->owner == kOwnerPool) e2 = hipFreeAsync(grid->dbuf, nullptr); // the device is idle
+    if (grid->dview) (void)hipFreeAsync(grid->dview, nullptr);
     delete grid;
     return (e1 == hipSuccess && e2 == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
 }

@@ -6,10 +6,11 @@
 // in-tile keys segment by segment (countNodes, :619-826).  Here the voxels' upper-node tiles are
 // numbered densely over the bbox of the input (x-major, which is the order of the reference's
 // tile keys), so one radix sort of (tile << 36 | in-tile key) over 36 + log2(tiles) bits gives
-// the final order.  One scan of the run heads at the four levels (voxel, leaf, lower, upper)
-// then numbers every node, and each node is written by one thread (headers, child tables) or
-// one 64-wide wavefront (leaf masks, bboxes): no per-voxel atomics, no host round trip
-// between the levels.
+// the final order.  The run heads at the four levels (voxel, leaf, lower, upper) are counted per
+// chunk of 4096 sorted positions and the chunk totals scanned; each chunk then numbers its own
+// positions with a workgroup scan while it writes the nodes that start there.  Each node is
+// written by one thread (headers, child tables) or one 64-wide wavefront (leaf masks, bboxes): no
+// per-voxel atomics, and no host round trip between the levels.
 #include "nvq_device.h"
 #include "nvq_internal.h"
 #include "nvq_profile.h"
@@ -22,6 +23,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -45,41 +47,66 @@ struct CntSum {
     }
 };
 // Position p of the sorted keys starts a new voxel / leaf / lower / upper when the key differs
-// from its predecessor in the bits above 0 / 9 / 21 / 36.
-struct HeadOp {
-    const uint64_t* keys;
-    __host__ __device__ Cnt operator()(uint32_t p) const
-    {
-        if (p == 0) return Cnt{1u, 1u, 1u, 1u};
-        const uint64_t k = keys[p], q = keys[p - 1];
-        return Cnt{uint32_t(k != q), uint32_t((k >> 9) != (q >> 9)), uint32_t((k >> 21) != (q >> 21)),
-                   uint32_t((k >> 36) != (q >> 36))};
-    }
-};
+// from its predecessor in the bits above 0 / 9 / 21 / 36 (32-bit keys hold one upper node).
+template<typename K>
+__host__ __device__ __forceinline__ Cnt headFlags(K k, K q)
+{
+    const uint64_t a = k, b = q;
+    return Cnt{uint32_t(a != b), uint32_t((a >> 9) != (b >> 9)), uint32_t((a >> 21) != (b >> 21)),
+               uint32_t((a >> 36) != (b >> 36))};
+}
 
 struct Layout {
     uint64_t offUpper, offLower, offLeaf, total;
     uint32_t rootHdr, leafSize, isIndex;
     int32_t tlo[3];   // smallest tile coordinate (x >> 12) per axis
     uint32_t dim[3];  // tiles per axis in the bbox of the input
+    int32_t llo[3];   // smallest lower-node coordinate (x >> 7) per axis
+    uint32_t ldim[3]; // lower nodes per axis in the bbox of the input
 };
 
-__device__ __forceinline__ void tileOf(const Layout& L, uint64_t tile, int32_t t[3])
+// Upper-node tile coordinates t (x >> 12) and upper-node child offset n2 of a sort key.  64-bit
+// keys: dense tile number << 36 | n2 << 21 | ...; 32-bit keys (the input lies in one tile):
+// dense lower-node number over the input's bbox << 21 | ..., x-major, which is n2's order.
+__device__ __forceinline__ void nodeOf(const Layout& L, uint64_t key, int32_t t[3], uint32_t& n2)
 {
+    const uint64_t tile = key >> 36;
     t[2] = int32_t(tile % L.dim[2]) + L.tlo[2];
     const uint64_t r = tile / L.dim[2];
     t[1] = int32_t(r % L.dim[1]) + L.tlo[1];
     t[0] = int32_t(r / L.dim[1]) + L.tlo[0];
+    n2 = uint32_t(key >> 21) & 32767u;
+}
+__device__ __forceinline__ void nodeOf(const Layout& L, uint32_t key, int32_t t[3], uint32_t& n2)
+{
+    const uint32_t dl = key >> 21;
+    const int32_t z7 = int32_t(dl % L.ldim[2]) + L.llo[2];
+    const uint32_t r = dl / L.ldim[2];
+    const int32_t y7 = int32_t(r % L.ldim[1]) + L.llo[1], x7 = int32_t(r / L.ldim[1]) + L.llo[0];
+    t[0] = x7 >> 5;
+    t[1] = y7 >> 5;
+    t[2] = z7 >> 5;
+    n2 = (uint32_t(x7 & 31) << 10) | (uint32_t(y7 & 31) << 5) | uint32_t(z7 & 31);
 }
 
-__global__ void k_v2g_bounds_init(int32_t* bnd)
+constexpr uint32_t kBoundBlocks = 1024;
+
+__device__ __forceinline__ void waveMinMax(int32_t lo[3], int32_t hi[3])
 {
-    if (threadIdx.x < 6) bnd[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
+        }
 }
 
-// Exact bbox of the input voxels (= RootData::mBBox of the result, processBBox :1386-1414).
-__global__ __launch_bounds__(kB) void k_v2g_bounds(const int32_t* __restrict__ ijk, uint64_t n, int32_t* __restrict__ bnd)
+// Exact bbox of the input voxels (= RootData::mBBox of the result, processBBox :1386-1414): one
+// partial bbox per workgroup, then one workgroup reduces the partials (no atomics).
+__global__ __launch_bounds__(kB) void k_v2g_bounds(const int32_t* __restrict__ ijk, uint64_t n, int32_t* __restrict__ part)
 {
+    __shared__ int32_t sb[kB / 64][6];
     int32_t lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
     const uint64_t stride = uint64_t(gridDim.x) * kB;
     for (uint64_t i = uint64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += stride) {
@@ -90,86 +117,176 @@ __global__ __launch_bounds__(kB) void k_v2g_bounds(const int32_t* __restrict__ i
             hi[a] = max(hi[a], v);
         }
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
-            hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
-        }
-    if ((threadIdx.x & 63u) == 0) {
-#pragma unroll
+    waveMinMax(lo, hi);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0)
         for (int a = 0; a < 3; ++a) {
-            atomicMin(bnd + a, lo[a]);
-            atomicMax(bnd + 3 + a, hi[a]);
+            sb[w][a] = lo[a];
+            sb[w][3 + a] = hi[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int32_t v = sb[0][threadIdx.x];
+        for (uint32_t j = 1; j < kB / 64; ++j) v = threadIdx.x < 3 ? min(v, sb[j][threadIdx.x]) : max(v, sb[j][threadIdx.x]);
+        part[6 * blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_v2g_bounds_final(const int32_t* __restrict__ part, uint32_t blocks,
+                                                         int32_t* __restrict__ bnd)
+{
+    __shared__ int32_t sb[kB / 64][6];
+    int32_t lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (uint32_t b = threadIdx.x; b < blocks; b += kB)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], part[6 * b + a]);
+            hi[a] = max(hi[a], part[6 * b + 3 + a]);
+        }
+    waveMinMax(lo, hi);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0)
+        for (int a = 0; a < 3; ++a) {
+            sb[w][a] = lo[a];
+            sb[w][3 + a] = hi[a];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int32_t v = sb[0][threadIdx.x];
+        for (uint32_t j = 1; j < kB / 64; ++j) v = threadIdx.x < 3 ? min(v, sb[j][threadIdx.x]) : max(v, sb[j][threadIdx.x]);
+        bnd[threadIdx.x] = v;
     }
 }
 
 // Sort key: dense tile number << 36 | upper offset << 21 | lower offset << 9 | leaf offset
-// (InternalNode::CoordToOffset NanoVDB.h:3560-3566, LeafNode::CoordToOffset :4516-4519).
+// (InternalNode::CoordToOffset NanoVDB.h:3560-3566, LeafNode::CoordToOffset :4516-4519), or for
+// 32-bit keys dense lower-node number << 21 | lower offset << 9 | leaf offset.
+template<typename K>
 __global__ __launch_bounds__(kB) void k_v2g_keys(const int32_t* __restrict__ ijk, uint32_t n, Layout L,
-                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ ids)
+                                                 K* __restrict__ keys, uint32_t* __restrict__ ids)
 {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= n) return;
     const int32_t x = ijk[3ull * i], y = ijk[3ull * i + 1], z = ijk[3ull * i + 2];
-    const uint64_t tile = (uint64_t(uint32_t((x >> 12) - L.tlo[0])) * L.dim[1] + uint32_t((y >> 12) - L.tlo[1])) * L.dim[2] +
-                          uint32_t((z >> 12) - L.tlo[2]);
-    const uint64_t up = (uint64_t((x & 4095) >> 7) << 10) | (uint64_t((y & 4095) >> 7) << 5) | uint64_t((z & 4095) >> 7);
     const uint64_t lo = (uint64_t((x & 127) >> 3) << 8) | (uint64_t((y & 127) >> 3) << 4) | uint64_t((z & 127) >> 3);
     const uint64_t lf = (uint64_t(x & 7) << 6) | (uint64_t(y & 7) << 3) | uint64_t(z & 7);
-    keys[i] = (tile << 36) | (up << 21) | (lo << 9) | lf;
+    if constexpr (sizeof(K) == 8) {
+        const uint64_t tile = (uint64_t(uint32_t((x >> 12) - L.tlo[0])) * L.dim[1] + uint32_t((y >> 12) - L.tlo[1])) *
+                                  L.dim[2] +
+                              uint32_t((z >> 12) - L.tlo[2]);
+        const uint64_t up = (uint64_t((x & 4095) >> 7) << 10) | (uint64_t((y & 4095) >> 7) << 5) | uint64_t((z & 4095) >> 7);
+        keys[i] = (tile << 36) | (up << 21) | (lo << 9) | lf;
+    } else {
+        const uint32_t dl = (uint32_t((x >> 7) - L.llo[0]) * L.ldim[1] + uint32_t((y >> 7) - L.llo[1])) * L.ldim[2] +
+                            uint32_t((z >> 7) - L.llo[2]);
+        keys[i] = K((uint64_t(dl) << 21) | (lo << 9) | lf);
+    }
     ids[i] = i;
 }
 
-// One thread per sorted position: the node that starts there writes its root tile / parent
-// child-table entry / child-mask bit (BuildUpperNodesFunctor :1034-1061, BuildLowerNodesFunctor
-// :1085-1104, ProcessLeafMetaDataFunctor :1126-1154) and records where its children start.
-__global__ __launch_bounds__(kB) void k_v2g_heads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids,
-                                                  const Cnt* __restrict__ cnt, uint32_t n, Layout L,
-                                                  uint8_t* __restrict__ buf, uint32_t* __restrict__ leafFirst,
-                                                  uint32_t* __restrict__ lowerFirst, uint32_t* __restrict__ upperFirst,
-                                                  uint64_t* __restrict__ index)
+constexpr uint32_t kItems = 16, kChunk = kB * kItems; // sorted positions per workgroup
+
+// Run-head totals of each chunk of kChunk sorted positions.
+template<typename K>
+__global__ __launch_bounds__(kB) void k_v2g_count(const K* __restrict__ keys, uint32_t n, Cnt* __restrict__ chunkSums)
 {
-    const uint32_t p = blockIdx.x * kB + threadIdx.x;
-    if (p >= n) return;
-    const Cnt c = cnt[p];
-    const uint64_t k = keys[p], q = p ? keys[p - 1] : 0;
-    const bool first = p == 0;
-    const bool hU = first || (k >> 36) != (q >> 36), hM = first || (k >> 21) != (q >> 21), hL = first || (k >> 9) != (q >> 9);
-    const uint32_t u = c.u - 1u, m = c.m - 1u, l = c.l - 1u;
-    const uint32_t n2 = uint32_t(k >> 21) & 32767u, n1 = uint32_t(k >> 9) & 4095u;
-    const uint64_t oU = L.offUpper + kUpperBytes * u, oL = L.offLower + kLowerBytes * m, oF = L.offLeaf + uint64_t(L.leafSize) * l;
-    if (hU) {
+    using Reduce = hipcub::BlockReduce<Cnt, kB>;
+    __shared__ typename Reduce::TempStorage tmp;
+    const uint32_t base = blockIdx.x * kChunk;
+    Cnt c{0u, 0u, 0u, 0u};
+#pragma unroll 4
+    for (uint32_t i = 0; i < kItems; ++i) {
+        const uint32_t p = base + i * kB + threadIdx.x;
+        if (p < n) c = CntSum()(c, p ? headFlags(keys[p], keys[p - 1]) : Cnt{1u, 1u, 1u, 1u});
+    }
+    const Cnt total = Reduce(tmp).Reduce(c, CntSum());
+    if (threadIdx.x == 0) chunkSums[blockIdx.x] = total;
+}
+
+// One workgroup per chunk: each thread takes kItems consecutive sorted positions, numbers them
+// (chunk prefix + block scan + running count) and, for each node that starts there, writes its
+// root tile / parent child-table entry / child-mask bit (BuildUpperNodesFunctor :1034-1061,
+// BuildLowerNodesFunctor :1085-1104, ProcessLeafMetaDataFunctor :1126-1154) and records where its
+// children start.  Every voxel's value index goes to its input position.
+template<typename K>
+__global__ __launch_bounds__(kB) void k_v2g_heads(const K* __restrict__ keys, const uint32_t* __restrict__ ids,
+                                                  const Cnt* __restrict__ chunkIncl, uint32_t n, Layout L,
+                                                  uint8_t* __restrict__ buf, uint32_t* __restrict__ leafFirst,
+                                                  uint32_t* __restrict__ leafRank, uint32_t* __restrict__ lowerFirst,
+                                                  uint32_t* __restrict__ upperFirst, uint64_t* __restrict__ index)
+{
+    using Scan = hipcub::BlockScan<Cnt, kB>;
+    __shared__ typename Scan::TempStorage tmp;
+    const uint32_t p0 = blockIdx.x * kChunk + threadIdx.x * kItems;
+    K k[kItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) k[j] = p0 + j < n ? keys[p0 + j] : K(0);
+    const K before = p0 > 0 && p0 < n ? keys[p0 - 1] : K(0);
+    auto flags = [&](uint32_t j) {
+        const uint32_t p = p0 + j;
+        if (p >= n) return Cnt{0u, 0u, 0u, 0u};
+        if (p == 0) return Cnt{1u, 1u, 1u, 1u};
+        return headFlags(k[j], j ? k[j - 1] : before);
+    };
+    Cnt mine{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) mine = CntSum()(mine, flags(j));
+    const Cnt chunkPrefix = blockIdx.x ? chunkIncl[blockIdx.x - 1] : Cnt{0u, 0u, 0u, 0u};
+    Cnt c;
+    Scan(tmp).ExclusiveScan(mine, c, chunkPrefix, CntSum());
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t p = p0 + j;
+        if (p >= n) break;
+        const Cnt h = flags(j);
+        c = CntSum()(c, h);
+        const K key = k[j];
+        const uint32_t u = c.u - 1u, m = c.m - 1u, l = c.l - 1u;
         int32_t t[3];
-        tileOf(L, k >> 36, t);
-        uint8_t* tile = buf + kRootOff + L.rootHdr + 32ull * u;
-        // RootData::CoordToKey (NanoVDB.h:2633-2640) of the tile origin, Tile::setChild
-        const uint64_t key = uint64_t(uint32_t(t[2] * 4096) >> 12) | (uint64_t(uint32_t(t[1] * 4096) >> 12) << 21) |
-                             (uint64_t(uint32_t(t[0] * 4096) >> 12) << 42);
-        *reinterpret_cast<uint64_t*>(tile) = key;
-        *reinterpret_cast<int64_t*>(tile + 8) = int64_t(oU - kRootOff);
-        *reinterpret_cast<uint64_t*>(buf + oU + 24) = 2u; // GridFlags::HasBBox
-        upperFirst[u] = m;
+        uint32_t n2;
+        nodeOf(L, key, t, n2);
+        const uint32_t n1 = uint32_t(key >> 9) & 4095u;
+        const uint64_t oU = L.offUpper + kUpperBytes * u, oL = L.offLower + kLowerBytes * m,
+                       oF = L.offLeaf + uint64_t(L.leafSize) * l;
+        if (h.u) {
+            uint8_t* tile = buf + kRootOff + L.rootHdr + 32ull * u;
+            // RootData::CoordToKey (NanoVDB.h:2633-2640) of the tile origin, Tile::setChild
+            *reinterpret_cast<uint64_t*>(tile) = uint64_t(uint32_t(t[2] * 4096) >> 12) |
+                                                 (uint64_t(uint32_t(t[1] * 4096) >> 12) << 21) |
+                                                 (uint64_t(uint32_t(t[0] * 4096) >> 12) << 42);
+            *reinterpret_cast<int64_t*>(tile + 8) = int64_t(oU - kRootOff);
+            *reinterpret_cast<uint64_t*>(buf + oU + 24) = 2u; // GridFlags::HasBBox
+            upperFirst[u] = m;
+        }
+        if (h.m) {
+            atomicOr(reinterpret_cast<unsigned long long*>(buf + oU + 4128 + 8u * (n2 >> 6)), 1ull << (n2 & 63u));
+            *reinterpret_cast<int64_t*>(buf + oU + 8256 + 8ull * n2) = int64_t(oL - oU);
+            *reinterpret_cast<uint64_t*>(buf + oL + 24) = 2u;
+            lowerFirst[m] = l;
+        }
+        if (h.l) {
+            atomicOr(reinterpret_cast<unsigned long long*>(buf + oL + 544 + 8u * (n1 >> 6)), 1ull << (n1 & 63u));
+            *reinterpret_cast<int64_t*>(buf + oL + 1088 + 8ull * n1) = int64_t(oF - oL);
+            leafFirst[l] = p;
+            leafRank[l] = c.v; // 1 + rank of the leaf's first voxel: LeafData<ValueOnIndex>::mOffset
+        }
+        if (index) index[ids[p]] = c.v; // 1 + rank of the voxel: its value index (index 0 is the background)
+        if (p == n - 1) {
+            leafFirst[c.l] = n;
+            lowerFirst[c.m] = c.l;
+            upperFirst[c.u] = c.m;
+        }
     }
-    if (hM) {
-        atomicOr(reinterpret_cast<unsigned long long*>(buf + oU + 4128 + 8u * (n2 >> 6)), 1ull << (n2 & 63u));
-        *reinterpret_cast<int64_t*>(buf + oU + 8256 + 8ull * n2) = int64_t(oL - oU);
-        *reinterpret_cast<uint64_t*>(buf + oL + 24) = 2u;
-        lowerFirst[m] = l;
-    }
-    if (hL) {
-        atomicOr(reinterpret_cast<unsigned long long*>(buf + oL + 544 + 8u * (n1 >> 6)), 1ull << (n1 & 63u));
-        *reinterpret_cast<int64_t*>(buf + oL + 1088 + 8ull * n1) = int64_t(oF - oL);
-        leafFirst[l] = p;
-    }
-    if (index) index[ids[p]] = c.v; // 1 + rank of the voxel: its value index (index 0 is the background)
-    if (p == n - 1) {
-        leafFirst[c.l] = n;
-        lowerFirst[c.m] = c.l;
-        upperFirst[c.u] = c.m;
-    }
+}
+
+// Grid, tree and root headers, passed by value so the launch owns a copy of them.
+constexpr uint32_t kHeaderMax = 832; // 672 + 64 + the 96-byte index-grid root header
+struct Header {
+    uint32_t bytes;
+    uint32_t w[kHeaderMax / 4];
+};
+__global__ void k_v2g_header(uint8_t* __restrict__ buf, Header h)
+{
+    for (uint32_t i = threadIdx.x; i < h.bytes / 4u; i += blockDim.x) reinterpret_cast<uint32_t*>(buf)[i] = h.w[i];
 }
 
 template<typename T>
@@ -184,7 +301,8 @@ __device__ __forceinline__ T laneSelect8(const T (&w)[8], uint32_t j)
 // One wavefront per leaf: value mask from its voxels, LeafNode::updateBBox (NanoVDB.h:4596-4624),
 // flags, and the index offset + prefix sums (leafPrefixSumKernel :459-479) or the float values
 // (1 active, 0 inactive; SetLeafActiveVoxelStateAndValuesFunctor :1156-1174).
-__global__ __launch_bounds__(kB) void k_v2g_leaves(const uint64_t* __restrict__ keys, const Cnt* __restrict__ cnt,
+template<typename K>
+__global__ __launch_bounds__(kB) void k_v2g_leaves(const K* __restrict__ keys, const uint32_t* __restrict__ leafRank,
                                                    const uint32_t* __restrict__ leafFirst, uint32_t nLeaf, Layout L,
                                                    uint8_t* __restrict__ buf)
 {
@@ -205,10 +323,11 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves(const uint64_t* __restrict__ 
 
     uint8_t* F = buf + L.offLeaf + uint64_t(L.leafSize) * l;
     if (lane == 0) {
-        const uint64_t k0 = keys[a];
+        const K k0 = keys[a];
         int32_t t[3];
-        tileOf(L, k0 >> 36, t);
-        const uint32_t n2 = uint32_t(k0 >> 21) & 32767u, n1 = uint32_t(k0 >> 9) & 4095u;
+        uint32_t n2;
+        nodeOf(L, k0, t, n2);
+        const uint32_t n1 = uint32_t(k0 >> 9) & 4095u;
         const int32_t origin[3] = {t[0] * 4096 + int32_t((n2 >> 10) & 31u) * 128 + int32_t((n1 >> 8) & 15u) * 8,
                                    t[1] * 4096 + int32_t((n2 >> 5) & 31u) * 128 + int32_t((n1 >> 4) & 15u) * 8,
                                    t[2] * 4096 + int32_t(n2 & 31u) * 128 + int32_t(n1 & 15u) * 8};
@@ -239,7 +358,7 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves(const uint64_t* __restrict__ 
                 sum += uint64_t(__popcll(w[j]));
                 prefix |= sum << (9 * j);
             }
-            *reinterpret_cast<uint64_t*>(F + 80) = uint64_t(cnt[a].v); // 1 + rank of the leaf's first voxel
+            *reinterpret_cast<uint64_t*>(F + 80) = uint64_t(leafRank[l]); // 1 + rank of the leaf's first voxel
             *reinterpret_cast<uint64_t*>(F + 88) = prefix;
         }
     }
@@ -389,24 +508,163 @@ void writeHeaders(uint8_t* h, const Layout& L, const uint32_t counts[4], const i
     put(h, 656, uint64_t(L.isIndex ? uint64_t(counts[0]) + 1u : 0u));
 }
 
-// Device allocations of one build, released when the build returns.
-struct Scratch {
-    std::vector<void*> ptrs;
-    template<typename T>
-    T* get(size_t count)
+// Per-device scratch of the builder, kept between builds and grown when a build needs more.  A
+// build holds the device's lock from start to finish and waits for its stream before it returns,
+// so no kernel of an earlier build still uses the memory when the next one starts.
+struct Workspace {
+    std::mutex lock;
+    uint8_t* base = nullptr;
+    size_t bytes = 0;
+    // device memory of at least `need` bytes (the contents are not kept when it grows)
+    uint8_t* reserve(size_t need)
     {
-        void* p = nullptr;
-        if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) return nullptr;
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
-    }
-    ~Scratch()
-    {
-        for (void* p : ptrs) (void)hipFree(p);
+        if (need <= bytes) return base;
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        bytes = 0;
+        const size_t grow = need + need / 4;
+        if (hipMalloc(reinterpret_cast<void**>(&base), grow) != hipSuccess) return nullptr;
+        bytes = grow;
+        return base;
     }
 };
+Workspace& workspaceOf(int device)
+{
+    static Workspace spaces[64];
+    return spaces[device & 63];
+}
+
+// Carves aligned arrays out of one block of device memory.
+struct Carve {
+    uint8_t* base;
+    size_t used = 0;
+    template<typename T>
+    T* take(size_t count)
+    {
+        used = (used + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += count * sizeof(T);
+        return p;
+    }
+};
+template<typename T>
+size_t carveBytes(size_t count)
+{
+    return ((count * sizeof(T) + 255) & ~size_t(255));
+}
 
 uint32_t blocksFor(uint64_t items, uint32_t perBlock) { return uint32_t((items + perBlock - 1) / perBlock); }
+
+// Steps after the bbox: sort keys of width K over endBit bits, node numbering, node contents.
+template<typename K>
+int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int endBit, const int32_t box[6],
+                double voxel_size, const char* name, uint32_t grid_type, uint64_t* index, hipStream_t s,
+                nvq_grid** out)
+{
+    // 2. keys, one radix sort, run-head totals per chunk and their scan
+    const uint32_t chunks = uint32_t((uint64_t(n32) + kChunk - 1) / kChunk);
+    size_t sortBytes = 0, scanBytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, static_cast<K*>(nullptr), static_cast<K*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), int(n32), 0,
+                                           endBit, s) != hipSuccess ||
+        hipcub::DeviceScan::InclusiveScan(nullptr, scanBytes, static_cast<Cnt*>(nullptr), static_cast<Cnt*>(nullptr),
+                                          CntSum(), int(chunks), s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    const size_t tempBytes = std::max(sortBytes, scanBytes);
+    // live until the end: sorted keys and positions, chunk counts; dead after the scan: the
+    // unsorted keys and positions and the library temporaries (reused for the node starts)
+    const size_t live = carveBytes<K>(n32) + carveBytes<uint32_t>(n32) + 2 * carveBytes<Cnt>(chunks);
+    const size_t dead = carveBytes<K>(n32) + carveBytes<uint32_t>(n32) + carveBytes<uint8_t>(tempBytes);
+    uint8_t* ws = W.reserve(live + dead);
+    if (!ws) return NVQ_ERR_HIP;
+    Carve c{ws};
+    K* keys2 = c.take<K>(n32);
+    uint32_t* ids2 = c.take<uint32_t>(n32);
+    Cnt* chunkSums = c.take<Cnt>(chunks);
+    Cnt* chunkIncl = c.take<Cnt>(chunks);
+    const size_t deadStart = (c.used + 255) & ~size_t(255);
+    K* keys = c.take<K>(n32);
+    uint32_t* ids = c.take<uint32_t>(n32);
+    void* temp = c.take<uint8_t>(tempBytes);
+    k_v2g_keys<K><<<blocksFor(n32, kB), kB, 0, s>>>(ijk, n32, L, keys, ids);
+    {
+        ProfScope prof("v2g_sort", s);
+        if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, ids, ids2, int(n32), 0, endBit, s) !=
+            hipSuccess)
+            return NVQ_ERR_HIP;
+    }
+    {
+        ProfScope prof("v2g_scan", s);
+        k_v2g_count<K><<<chunks, kB, 0, s>>>(keys2, n32, chunkSums);
+        if (hipcub::DeviceScan::InclusiveScan(temp, scanBytes, chunkSums, chunkIncl, CntSum(), int(chunks), s) != hipSuccess)
+            return NVQ_ERR_HIP;
+    }
+    Cnt tot;
+    if (hipMemcpyAsync(&tot, chunkIncl + (chunks - 1), sizeof(Cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return NVQ_ERR_HIP;
+
+    // 3. layout (GridT::memUsage sums of PointsToGrid::getBuffer :848-856), zeroed buffer
+    L.rootHdr = L.isIndex ? 96u : 64u;
+    L.leafSize = L.isIndex ? 96u : 2144u;
+    L.offUpper = kRootOff + L.rootHdr + 32ull * tot.u;
+    L.offLower = L.offUpper + kUpperBytes * tot.u;
+    L.offLeaf = L.offLower + kLowerBytes * tot.m;
+    L.total = L.offLeaf + uint64_t(L.leafSize) * tot.l;
+    void* dbuf = nullptr;
+    keepPoolMemory();
+    if (hipMallocAsync(&dbuf, L.total, s) != hipSuccess) return NVQ_ERR_HIP;
+    uint8_t* buf = static_cast<uint8_t*>(dbuf);
+    Header hdr{};
+    hdr.bytes = uint32_t(kRootOff + L.rootHdr);
+    const uint32_t counts[4] = {tot.v, tot.l, tot.m, tot.u};
+    writeHeaders(reinterpret_cast<uint8_t*>(hdr.w), L, counts, box, voxel_size, name, grid_type);
+    // node starts: in the dead region when they fit, else in a block of their own
+    const size_t starts = carveBytes<uint32_t>(uint64_t(tot.l) + 1) + carveBytes<uint32_t>(tot.l) +
+                          carveBytes<uint32_t>(uint64_t(tot.m) + 1) + carveBytes<uint32_t>(uint64_t(tot.u) + 1);
+    void* extra = nullptr;
+    Carve st{ws + deadStart};
+    if (starts > dead) {
+        if (hipMallocAsync(&extra, starts, s) != hipSuccess) {
+            (void)hipFreeAsync(dbuf, s);
+            return NVQ_ERR_HIP;
+        }
+        st = Carve{static_cast<uint8_t*>(extra)};
+    }
+    uint32_t* leafFirst = st.take<uint32_t>(uint64_t(tot.l) + 1);
+    uint32_t* leafRank = st.take<uint32_t>(tot.l);
+    uint32_t* lowerFirst = st.take<uint32_t>(uint64_t(tot.m) + 1);
+    uint32_t* upperFirst = st.take<uint32_t>(uint64_t(tot.u) + 1);
+    int rc = NVQ_ERR_HIP;
+    if (hipMemsetAsync(buf, 0, L.total, s) == hipSuccess) {
+        k_v2g_header<<<1, 256, 0, s>>>(buf, hdr);
+        // 4. nodes, then leaves, then bboxes bottom-up
+        {
+            ProfScope prof("k_v2g_heads", s);
+            k_v2g_heads<K><<<chunks, kB, 0, s>>>(keys2, ids2, chunkIncl, n32, L, buf, leafFirst, leafRank, lowerFirst,
+                                              upperFirst, index);
+        }
+        {
+            ProfScope prof("k_v2g_leaves", s);
+            k_v2g_leaves<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafRank, leafFirst, tot.l, L, buf);
+        }
+        k_v2g_lowers<<<blocksFor(tot.m, kWaves), kB, 0, s>>>(lowerFirst, tot.m, L, buf);
+        k_v2g_uppers<<<blocksFor(tot.u, kWaves), kB, 0, s>>>(upperFirst, tot.u, L, buf);
+        // complete before returning, as PointsToGrid::getHandle does (:537): the grid may be used
+        // on any stream next
+        if (hipGetLastError() == hipSuccess)
+            rc = makeGridHandle(reinterpret_cast<const uint8_t*>(hdr.w), L.total, dbuf, kOwnerPool, s, out);
+    }
+    if (extra) (void)hipFreeAsync(extra, s);
+    if (rc != NVQ_OK) (void)hipFreeAsync(dbuf, s);
+    // the workspace is free for the next build only once this stream has drained
+    if (hipStreamSynchronize(s) != hipSuccess && rc == NVQ_OK) {
+        (void)nvq_grid_free(*out);
+        *out = nullptr;
+        rc = NVQ_ERR_HIP;
+    }
+    return rc;
+}
 
 } // namespace
 
@@ -423,13 +681,20 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
     if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t n32 = uint32_t(n);
-    Scratch S;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
+    Workspace& W = workspaceOf(device);
+    std::lock_guard<std::mutex> hold(W.lock);
 
     // 1. bbox of the input, read back to size the dense tile numbering
-    int32_t* bnd = S.get<int32_t>(6);
-    if (!bnd) return NVQ_ERR_HIP;
-    k_v2g_bounds_init<<<1, 64, 0, s>>>(bnd);
-    k_v2g_bounds<<<std::min<uint32_t>(blocksFor(n, kB), 2048u), kB, 0, s>>>(ijk, n, bnd);
+    uint8_t* ws = W.reserve(carveBytes<int32_t>(6 * kBoundBlocks) + carveBytes<int32_t>(6));
+    if (!ws) return NVQ_ERR_HIP;
+    Carve c{ws};
+    int32_t* part = c.take<int32_t>(6 * kBoundBlocks);
+    int32_t* bnd = c.take<int32_t>(6);
+    const uint32_t nb = std::min<uint32_t>(blocksFor(n, kB), kBoundBlocks);
+    k_v2g_bounds<<<nb, kB, 0, s>>>(ijk, n, part);
+    k_v2g_bounds_final<<<1, kB, 0, s>>>(part, nb, bnd);
     int32_t box[6];
     if (hipMemcpyAsync(box, bnd, sizeof(box), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         return NVQ_ERR_HIP;
@@ -441,77 +706,22 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
         tiles *= L.dim[a];
     }
     const int tileBits = tiles > 1 ? 64 - __builtin_clzll(tiles - 1) : 0;
-    if (36 + tileBits > 64) return NVQ_ERR_INVALID_ARG; // input spans more than 2^28 upper-node tiles
-
-    // 2. keys, one radix sort, run-head scan
-    uint64_t* keys = S.get<uint64_t>(n);
-    uint64_t* keys2 = S.get<uint64_t>(n);
-    uint32_t* ids = S.get<uint32_t>(n);
-    uint32_t* ids2 = S.get<uint32_t>(n);
-    Cnt* cnt = S.get<Cnt>(n);
-    if (!keys || !keys2 || !ids || !ids2 || !cnt) return NVQ_ERR_HIP;
-    k_v2g_keys<<<blocksFor(n, kB), kB, 0, s>>>(ijk, n32, L, keys, ids);
-    hipcub::TransformInputIterator<Cnt, HeadOp, hipcub::CountingInputIterator<uint32_t>> heads(
-        hipcub::CountingInputIterator<uint32_t>(0u), HeadOp{keys2});
-    size_t sortBytes = 0, scanBytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keys2, ids, ids2, int(n32), 0, 36 + tileBits, s) !=
-            hipSuccess ||
-        hipcub::DeviceScan::InclusiveScan(nullptr, scanBytes, heads, cnt, CntSum(), int(n32), s) != hipSuccess)
-        return NVQ_ERR_HIP;
-    void* temp = S.get<uint8_t>(std::max(sortBytes, scanBytes));
-    if (!temp) return NVQ_ERR_HIP;
-    {
-        ProfScope prof("v2g_sort", s);
-        if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, ids, ids2, int(n32), 0, 36 + tileBits, s) !=
-            hipSuccess)
-            return NVQ_ERR_HIP;
+    uint64_t lowers = 1;
+    for (int a = 0; a < 3; ++a) {
+        L.llo[a] = box[a] >> 7;
+        L.ldim[a] = uint32_t((box[3 + a] >> 7) - L.llo[a] + 1);
+        lowers *= L.ldim[a];
     }
-    {
-        ProfScope prof("v2g_scan", s);
-        if (hipcub::DeviceScan::InclusiveScan(temp, scanBytes, heads, cnt, CntSum(), int(n32), s) != hipSuccess)
-            return NVQ_ERR_HIP;
-    }
-    Cnt tot;
-    if (hipMemcpyAsync(&tot, cnt + (n32 - 1), sizeof(Cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return NVQ_ERR_HIP;
-
-    // 3. layout (GridT::memUsage sums of PointsToGrid::getBuffer :848-856), zeroed buffer
+    const int lowerBits = lowers > 1 ? 64 - __builtin_clzll(lowers - 1) : 0;
     L.isIndex = grid_type == GT_ONINDEX;
-    L.rootHdr = L.isIndex ? 96u : 64u;
-    L.leafSize = L.isIndex ? 96u : 2144u;
-    L.offUpper = kRootOff + L.rootHdr + 32ull * tot.u;
-    L.offLower = L.offUpper + kUpperBytes * tot.u;
-    L.offLeaf = L.offLower + kLowerBytes * tot.m;
-    L.total = L.offLeaf + uint64_t(L.leafSize) * tot.l;
-    void* dbuf = nullptr;
-    if (hipMalloc(&dbuf, L.total) != hipSuccess) return NVQ_ERR_HIP;
-    uint8_t* buf = static_cast<uint8_t*>(dbuf);
-    std::vector<uint8_t> hdr(kRootOff + L.rootHdr, 0);
-    const uint32_t counts[4] = {tot.v, tot.l, tot.m, tot.u};
-    writeHeaders(hdr.data(), L, counts, box, voxel_size, name, grid_type);
-    uint32_t* leafFirst = S.get<uint32_t>(uint64_t(tot.l) + 1);
-    uint32_t* lowerFirst = S.get<uint32_t>(uint64_t(tot.m) + 1);
-    uint32_t* upperFirst = S.get<uint32_t>(uint64_t(tot.u) + 1);
-    int rc = NVQ_ERR_HIP;
-    if (leafFirst && lowerFirst && upperFirst && hipMemsetAsync(buf, 0, L.total, s) == hipSuccess &&
-        hipMemcpyAsync(buf, hdr.data(), hdr.size(), hipMemcpyHostToDevice, s) == hipSuccess) {
-        // 4. nodes, then leaves, then bboxes bottom-up
-        {
-            ProfScope prof("k_v2g_heads", s);
-            k_v2g_heads<<<blocksFor(n, kB), kB, 0, s>>>(keys2, ids2, cnt, n32, L, buf, leafFirst, lowerFirst, upperFirst,
-                                                        index);
-        }
-        {
-            ProfScope prof("k_v2g_leaves", s);
-            k_v2g_leaves<<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, cnt, leafFirst, tot.l, L, buf);
-        }
-        k_v2g_lowers<<<blocksFor(tot.m, kWaves), kB, 0, s>>>(lowerFirst, tot.m, L, buf);
-        k_v2g_uppers<<<blocksFor(tot.u, kWaves), kB, 0, s>>>(upperFirst, tot.u, L, buf);
-        if (hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess)
-            rc = makeGridHandle(hdr.data(), L.total, dbuf, true, s, out);
-    }
-    if (rc != NVQ_OK) (void)hipFree(dbuf);
+    int rc;
+    if (tiles == 1 && 21 + lowerBits <= 32)
+        rc = buildSorted<uint32_t>(W, ijk, n32, L, 21 + lowerBits, box, voxel_size, name, grid_type, index, s, out);
+    else if (36 + tileBits > 64)
+        return NVQ_ERR_INVALID_ARG; // input spans more than 2^28 upper-node tiles
+    else
+        rc = buildSorted<uint64_t>(W, ijk, n32, L, 36 + tileBits, box, voxel_size, name, grid_type, index, s, out);
+    if (rc != NVQ_OK) (void)hipStreamSynchronize(s); // no kernel may still use the workspace
     return rc;
 }
 

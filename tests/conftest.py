@@ -78,7 +78,8 @@ def grid_bbox(buf):
 def voxel_cases():
     """Voxel lists for the device builder (voxelsToGrid): the reference's own test inputs
     (unittest/TestNanoVDB.cu: one voxel (1,2,3); three voxels; the 125..145 cube; random voxels in
-    [-512, 512)) plus duplicates, negative coordinates and inputs spanning several upper tiles."""
+    [-512, 512)) plus duplicates, negative coordinates, inputs spanning several upper tiles, and a
+    large input inside one upper tile (the builder's 32-bit sort keys)."""
     r = rng(11)
     cube = np.stack(np.meshgrid(*[np.arange(125, 146)] * 3, indexing="ij"), -1).reshape(-1, 3)
     rand = r.integers(-512, 512, (20000, 3))
@@ -87,7 +88,10 @@ def voxel_cases():
     shell = r.normal(size=(30000, 3))
     shell = np.rint(shell / np.linalg.norm(shell, axis=1, keepdims=True) * r.uniform(198, 202, (30000, 1)) +
                     np.array([-37.0, 11.0, 5.0]))
+    single = r.integers(0, 1500, (200000, 3))
+    single = np.concatenate([single, single[:5000]])
     return {
+        "single_tile": single,
         "one": np.array([[1, 2, 3]]),
         "three": np.array([[1, 2, 3], [1, 2, 4], [8, 2, 3]]),
         "cube": cube,
