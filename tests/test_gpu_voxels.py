@@ -129,3 +129,35 @@ def test_index_grid_from_float_voxels_answers_like_the_float_grid():
         ok &= (a == 1) & (d == 1)
     assert int(ok.sum()) > 1000
     assert torch.equal(is_[ok], fs[ok])
+
+
+def test_device_built_index_grid_marches_and_differentiates_like_the_fog():
+    """voxelsToGrid<ValueOnIndex> from a fog's active voxels, channel filled through the returned
+    indices: the fog march (radiance, transmittance) and its adjoint on the index grid equal those
+    on the fog bit for bit (the fog has no tiles, no empty leaves and inactive values equal to its
+    background, so both grids hold the same values everywhere).  The adjoint's fixed-point sums
+    are compared per active voxel: leaf slot of the fog against channel index of the index grid."""
+    nvq = _nvq()
+    fog = nvq.tools.create_fog_fbm([0, 0, 0, 95, 95, 95], seed=5)
+    ijk, val = nvq.tools.active_voxels(fog)
+    fg = nvq.DeviceGrid(fog)
+    ig, idx = nvq.DeviceGrid.from_voxels(torch.from_numpy(ijk).cuda(), "OnIndex", fg.voxel_size[0], return_index=True)
+    channel = torch.zeros(ig.value_count, dtype=torch.float32, device="cuda")
+    channel[idx] = torch.from_numpy(val).cuda()
+    ig.set_channel(channel)
+    eye, dir = nvq.tools.camera_rays(96, 96, (150.0, 130.0, 190.0), (48.0, 48.0, 48.0), (0.0, 1.0, 0.0), 45.0)
+    E, D = torch.from_numpy(eye).cuda(), torch.from_numpy(dir).cuda()
+    Lf, Tf = nvq.ops.fog_march(fg, E, D, 0.5, 0.05, image_width=96)
+    Li, Ti = nvq.ops.fog_march(ig, E, D, 0.5, 0.05, image_width=96)
+    assert float(Lf.max()) > 0.0
+    assert torch.equal(Lf, Li) and torch.equal(Tf, Ti)
+    gL = torch.from_numpy(rng(21).normal(0, 1, len(eye)).astype(np.float32)).cuda()
+    af = nvq.ops.fog_march_backward(fg, E, D, Lf, gL, image_width=96)
+    ai = nvq.ops.fog_march_backward(ig, E, D, Li, gL, image_width=96)
+    offs = np.frombuffer(fog[672:704].tobytes(), np.int64)
+    count = int(np.frombuffer(fog[704:708].tobytes(), np.uint32)[0])
+    leaves = fog[672 + offs[0]:672 + offs[0] + 2144 * count].reshape(count, 2144)
+    li, vi = np.nonzero(np.unpackbits(leaves[:, 16:80], axis=1, bitorder="little").astype(bool))
+    slots = torch.from_numpy(li.astype(np.int64) * 512 + vi).cuda()
+    assert int((af[slots] != 0).sum()) > 1000
+    assert torch.equal(af[slots], ai[idx])
