@@ -198,6 +198,17 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
  * between calls and grows to the largest build; builds on one device run one at a time. */
 int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
                        uint64_t* index, void* stream, nvq_grid** out);
+/* Device-side construction of a point grid: tools::cuda::pointsToGrid(d_points, n, voxelSize, type)
+ * (tools/cuda/PointsToGrid.cuh:57-64) for n float world-space points xyz[n][3] (a DEVICE array).
+ * Each point lies in voxel Round(Map(voxel_size).applyInverseMapF(p)).  The grid (GridType
+ * PointIndex, class PointData, or PointIndex for point_type 1) has per-voxel point counts in its
+ * leaves and the points, sorted by voxel, as blind data encoded by point_type: 1 PointID (uint32),
+ * 2 World64, 3 World32, 4 Grid64, 5 Grid32, 6 Voxel32, 7 Voxel16, 8 Voxel8, 9 Default (= World32
+ * for float points) (PointType, NanoVDB.h:395-405).  The buffer equals the reference's byte for
+ * byte.  A leaf may hold at most 65535 points, else NVQ_ERR_INVALID_ARG (the reference throws).
+ * Point grids are built and downloaded here; the queries of this library do not read them. */
+int nvq_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                       void* stream, nvq_grid** out);
 /* Copy the first `bytes` bytes (at most nvq_grid_info.bytes; pass that for the whole grid) of a
  * grid's device buffer to host memory and wait for it: GridHandle::deviceDownload
  * (GridHandle.h:186-194). */

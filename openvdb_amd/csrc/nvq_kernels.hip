@@ -59,9 +59,20 @@ int nvq::gridRefreshBBox(nvq_grid* g)
     return NVQ_OK;
 }
 
-// A grid can be queried when its values are reachable: value grids always, index grids once they
-// have a float channel.
-static bool queryable(const nvq_grid* g) { return g->view.type != GT_ONINDEX || g->view.channel != nullptr; }
+// A grid can be queried when the kernels decode its type and its values are reachable: value grids
+// always, index grids once they have a float channel.  Point grids (built on the device) are not.
+static bool queryable(const nvq_grid* g)
+{
+    switch (g->view.type) {
+    case GT_FLOAT:
+    case GT_FP4:
+    case GT_FP8:
+    case GT_FP16:
+    case GT_FPN: return true;
+    case GT_ONINDEX: return g->view.channel != nullptr;
+    default: return false;
+    }
+}
 
 namespace {
 

@@ -63,6 +63,19 @@ struct Layout {
     uint32_t dim[3];  // tiles per axis in the bbox of the input
     int32_t llo[3];   // smallest lower-node coordinate (x >> 7) per axis
     uint32_t ldim[3]; // lower nodes per axis in the bbox of the input
+    uint32_t isPoint; // Point grid: leaves hold point counts, the points follow as blind data
+    uint64_t offMeta; // blind metadata (point grids) or the end of the leaves
+};
+
+// Point grids from float world points (pointsToGrid, tools/cuda/PointsToGrid.cuh:57-64).
+struct PointArgs {
+    const float* xyz;
+    uint32_t type;      // PointType 1..9 (NanoVDB.h:395-405)
+    uint32_t size;      // bytes per encoded point
+    uint32_t semantic, dataClass, dataType, gridClass;
+    const char* name;   // blind-data name
+    float invF;         // Map::mInvMatF diagonal
+    double invD;        // Map::mInvMatD diagonal
 };
 
 // Upper-node tile coordinates t (x >> 12) and upper-node child offset n2 of a sort key.  64-bit
@@ -278,6 +291,122 @@ __global__ __launch_bounds__(kB) void k_v2g_heads(const K* __restrict__ keys, co
     }
 }
 
+// Map(dx)::applyInverseMapF of a float point (matMult with mInvMatF, math/Math.h:1704-1709)
+__device__ __forceinline__ void invMapF(float inv, const float p[3], float q[3])
+{
+    q[0] = __builtin_fmaf(p[0], inv, __builtin_fmaf(p[1], 0.0f, p[2] * 0.0f));
+    q[1] = __builtin_fmaf(p[0], 0.0f, __builtin_fmaf(p[1], inv, p[2] * 0.0f));
+    q[2] = __builtin_fmaf(p[0], 0.0f, __builtin_fmaf(p[1], 0.0f, p[2] * inv));
+}
+// Map(dx)::applyInverseMap of a float point: FP64 matMult (math/Math.h:1716-1722), returned as float
+__device__ __forceinline__ void invMapD(double inv, const float p[3], float q[3])
+{
+    const double x = p[0], y = p[1], z = p[2];
+    q[0] = float(__builtin_fma(x, inv, __builtin_fma(y, 0.0, z * 0.0)));
+    q[1] = float(__builtin_fma(x, 0.0, __builtin_fma(y, inv, z * 0.0)));
+    q[2] = float(__builtin_fma(x, 0.0, __builtin_fma(y, 0.0, z * inv)));
+}
+
+// Voxel of each point: Round(applyInverseMapF(p)) (TileKeyFunctor :553-580, Vec3<float>::round,
+// math/Math.h:1515-1524).
+__global__ __launch_bounds__(kB) void k_p2g_voxels(const float* __restrict__ xyz, uint32_t n, float invF,
+                                                   int32_t* __restrict__ ijk)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const float p[3] = {xyz[3ull * i], xyz[3ull * i + 1], xyz[3ull * i + 2]};
+    float q[3];
+    invMapF(invF, p, q);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ijk[3ull * i + a] = int32_t(floorf(q[a] + 0.5f));
+}
+
+// One wavefront per Point leaf: mOffset (points before the leaf), mPointCount, and per voxel the
+// number of the leaf's points up to and including it (SetLeafActiveVoxelStateAndValuesFunctor
+// :1156-1174); an inactive voxel repeats the previous active voxel's value, else 0
+// (SetLeafInactiveVoxelValuesFunctor :1176-1191).
+template<typename K>
+__global__ __launch_bounds__(kB) void k_p2g_leaf_values(const K* __restrict__ keys, const uint32_t* __restrict__ leafFirst,
+                                                        uint32_t nLeaf, Layout L, uint8_t* __restrict__ buf,
+                                                        uint32_t* __restrict__ maxCount)
+{
+    __shared__ uint16_t vals[kWaves][512];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, l = blockIdx.x * kWaves + wv;
+    const bool valid = l < nLeaf;
+    const uint32_t a = valid ? leafFirst[l] : 0u, b = valid ? leafFirst[l + 1] : 0u;
+    for (uint32_t j = lane; j < 512u; j += 64u) vals[wv][j] = 0;
+    __syncthreads();
+    for (uint32_t q = a + lane; q < b; q += 64u) {
+        const K k = keys[q];
+        if (q + 1u == b || keys[q + 1u] != k) vals[wv][uint32_t(k) & 511u] = uint16_t(q + 1u - a);
+    }
+    __syncthreads();
+    if (!valid) return;
+    uint8_t* F = buf + L.offLeaf + uint64_t(L.leafSize) * l;
+    const uint64_t* mask = reinterpret_cast<const uint64_t*>(F + 16);
+    uint16_t out[8];
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+        const uint32_t v = lane * 8u + e;
+        // Mask::findPrev<true>(v), NanoVDB.h:1399-1409
+        int32_t wi = int32_t(v >> 6);
+        uint64_t bits = mask[wi] & (~0ull >> (63u - (v & 63u)));
+        while (!bits && --wi >= 0) bits = mask[wi];
+        out[e] = bits ? vals[wv][uint32_t(wi) * 64u + 63u - uint32_t(__builtin_clzll(bits))] : uint16_t(0);
+    }
+    uint4 o;
+    o.x = uint32_t(out[0]) | (uint32_t(out[1]) << 16);
+    o.y = uint32_t(out[2]) | (uint32_t(out[3]) << 16);
+    o.z = uint32_t(out[4]) | (uint32_t(out[5]) << 16);
+    o.w = uint32_t(out[6]) | (uint32_t(out[7]) << 16);
+    *reinterpret_cast<uint4*>(F + 96 + 16u * lane) = o;
+    if (lane == 0) {
+        *reinterpret_cast<uint64_t*>(F + 80) = a;
+        *reinterpret_cast<uint64_t*>(F + 88) = b - a;
+        atomicMax(maxCount, b - a);
+    }
+}
+
+// The points in sorted order as blind data (encodePoints :1262-1317, worldToVoxel :197-236).
+__global__ __launch_bounds__(kB) void k_p2g_blind(const uint32_t* __restrict__ ids, uint32_t n, PointArgs P,
+                                                  uint8_t* __restrict__ out)
+{
+    const uint32_t q = blockIdx.x * kB + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t id = ids[q];
+    const float w[3] = {P.xyz[3ull * id], P.xyz[3ull * id + 1], P.xyz[3ull * id + 2]};
+    uint8_t* o = out + uint64_t(P.size) * q;
+    float g[3];
+    switch (P.type) {
+    case 1: *reinterpret_cast<uint32_t*>(o) = id; break;
+    case 2:
+        for (int a = 0; a < 3; ++a) reinterpret_cast<double*>(o)[a] = double(w[a]);
+        break;
+    case 4:
+        invMapD(P.invD, w, g);
+        for (int a = 0; a < 3; ++a) reinterpret_cast<double*>(o)[a] = double(g[a]);
+        break;
+    case 5:
+        invMapF(P.invF, w, g);
+        for (int a = 0; a < 3; ++a) reinterpret_cast<float*>(o)[a] = g[a];
+        break;
+    case 6:
+    case 7:
+    case 8:
+        invMapD(P.invD, w, g);
+        for (int a = 0; a < 3; ++a) {
+            const double x = double(g[a]);
+            const double f = x - double(int32_t(floor(x + 0.5)));
+            if (P.type == 6) reinterpret_cast<float*>(o)[a] = float(f);
+            else if (P.type == 7) reinterpret_cast<uint16_t*>(o)[a] = uint16_t(65535.0 * (f + 0.5));
+            else o[a] = uint8_t(255.0 * (f + 0.5));
+        }
+        break;
+    default:
+        for (int a = 0; a < 3; ++a) reinterpret_cast<float*>(o)[a] = w[a];
+    }
+}
+
 // Grid, tree and root headers, passed by value so the launch owns a copy of them.
 constexpr uint32_t kHeaderMax = 832; // 672 + 64 + the 96-byte index-grid root header
 struct Header {
@@ -363,7 +492,7 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves(const K* __restrict__ keys, c
         }
     }
     if (lane < 8) *reinterpret_cast<uint64_t*>(F + 16 + 8 * lane) = laneSelect8(w, lane);
-    if (!L.isIndex) {
+    if (!L.isIndex && !L.isPoint) { // float leaves only: 2048 B of values
         // values 8*lane .. 8*lane+7 live in byte (lane & 7) of mask word lane >> 3
         const uint32_t bits = uint32_t(laneSelect8(w, lane >> 3) >> (8u * (lane & 7u))) & 255u;
         float4 v0, v1;
@@ -448,7 +577,7 @@ void put(uint8_t* b, uint64_t off, const T& v)
 // Grid, tree and root headers (BuildGridTreeRootFunctor :877-1008 with GridData::init
 // NanoVDB.h:1968-2000, Map(double) :1441-1451, UpdateRootWorldBBoxFunctor :1374-1384).
 void writeHeaders(uint8_t* h, const Layout& L, const uint32_t counts[4], const int32_t box[6], double voxelSize,
-                  const char* name, uint32_t gridType)
+                  const char* name, uint32_t gridType, uint32_t gridClass = 0, uint64_t points = 0)
 {
     const uint64_t offTree = kTreeOff, offRoot = kRootOff;
     for (int a = 0; a < 6; ++a) put(h, offRoot + 4 * a, box[a]);
@@ -502,10 +631,11 @@ void writeHeaders(uint8_t* h, const Layout& L, const uint32_t counts[4], const i
     apply(zero, p0);
     const double vs[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
     put(h, 608, vs);
-    put(h, 632, uint32_t(L.isIndex ? 8u : 0u)); // GridClass IndexGrid / Unknown
+    put(h, 632, uint32_t(L.isIndex ? 8u : L.isPoint ? gridClass : 0u)); // IndexGrid / PointIndex|PointData / Unknown
     put(h, 636, gridType);
-    put(h, 640, int64_t(L.total)); // no blind metadata
-    put(h, 656, uint64_t(L.isIndex ? uint64_t(counts[0]) + 1u : 0u));
+    put(h, 640, int64_t(L.offMeta)); // blind metadata: only point grids have some
+    put(h, 648, uint32_t(L.isPoint ? 1u : 0u));
+    put(h, 656, uint64_t(L.isIndex ? uint64_t(counts[0]) + 1u : L.isPoint ? points : 0u));
 }
 
 // Per-device scratch of the builder, kept between builds and grown when a build needs more.  A
@@ -558,8 +688,8 @@ uint32_t blocksFor(uint64_t items, uint32_t perBlock) { return uint32_t((items +
 // Steps after the bbox: sort keys of width K over endBit bits, node numbering, node contents.
 template<typename K>
 int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int endBit, const int32_t box[6],
-                double voxel_size, const char* name, uint32_t grid_type, uint64_t* index, hipStream_t s,
-                nvq_grid** out)
+                double voxel_size, const char* name, uint32_t grid_type, uint64_t* index, const PointArgs* P,
+                hipStream_t s, nvq_grid** out)
 {
     // 2. keys, one radix sort, run-head totals per chunk and their scan
     const uint32_t chunks = uint32_t((uint64_t(n32) + kChunk - 1) / kChunk);
@@ -605,12 +735,13 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
         return NVQ_ERR_HIP;
 
     // 3. layout (GridT::memUsage sums of PointsToGrid::getBuffer :848-856), zeroed buffer
-    L.rootHdr = L.isIndex ? 96u : 64u;
-    L.leafSize = L.isIndex ? 96u : 2144u;
+    L.rootHdr = (L.isIndex || L.isPoint) ? 96u : 64u;
+    L.leafSize = L.isIndex ? 96u : L.isPoint ? 1120u : 2144u;
     L.offUpper = kRootOff + L.rootHdr + 32ull * tot.u;
     L.offLower = L.offUpper + kUpperBytes * tot.u;
     L.offLeaf = L.offLower + kLowerBytes * tot.m;
-    L.total = L.offLeaf + uint64_t(L.leafSize) * tot.l;
+    L.offMeta = L.offLeaf + uint64_t(L.leafSize) * tot.l;
+    L.total = P ? L.offMeta + 288u + uint64_t(P->size) * n32 : L.offMeta;
     void* dbuf = nullptr;
     keepPoolMemory();
     if (hipMallocAsync(&dbuf, L.total, s) != hipSuccess) return NVQ_ERR_HIP;
@@ -618,10 +749,12 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     Header hdr{};
     hdr.bytes = uint32_t(kRootOff + L.rootHdr);
     const uint32_t counts[4] = {tot.v, tot.l, tot.m, tot.u};
-    writeHeaders(reinterpret_cast<uint8_t*>(hdr.w), L, counts, box, voxel_size, name, grid_type);
+    writeHeaders(reinterpret_cast<uint8_t*>(hdr.w), L, counts, box, voxel_size, name, grid_type, P ? P->gridClass : 0u,
+                 n32);
     // node starts: in the dead region when they fit, else in a block of their own
     const size_t starts = carveBytes<uint32_t>(uint64_t(tot.l) + 1) + carveBytes<uint32_t>(tot.l) +
-                          carveBytes<uint32_t>(uint64_t(tot.m) + 1) + carveBytes<uint32_t>(uint64_t(tot.u) + 1);
+                          carveBytes<uint32_t>(uint64_t(tot.m) + 1) + carveBytes<uint32_t>(uint64_t(tot.u) + 1) +
+                          carveBytes<uint32_t>(1);
     void* extra = nullptr;
     Carve st{ws + deadStart};
     if (starts > dead) {
@@ -635,6 +768,7 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     uint32_t* leafRank = st.take<uint32_t>(tot.l);
     uint32_t* lowerFirst = st.take<uint32_t>(uint64_t(tot.m) + 1);
     uint32_t* upperFirst = st.take<uint32_t>(uint64_t(tot.u) + 1);
+    uint32_t* maxCount = st.take<uint32_t>(1);
     int rc = NVQ_ERR_HIP;
     if (hipMemsetAsync(buf, 0, L.total, s) == hipSuccess) {
         k_v2g_header<<<1, 256, 0, s>>>(buf, hdr);
@@ -648,6 +782,23 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
             ProfScope prof("k_v2g_leaves", s);
             k_v2g_leaves<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafRank, leafFirst, tot.l, L, buf);
         }
+        if (P) {
+            // Point grids: leaf values, offsets and counts, the blind metadata and the points
+            (void)hipMemsetAsync(maxCount, 0, sizeof(uint32_t), s);
+            k_p2g_leaf_values<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafFirst, tot.l, L, buf, maxCount);
+            Header meta{};
+            meta.bytes = 288u;
+            uint8_t* mb = reinterpret_cast<uint8_t*>(meta.w);
+            put(mb, 0, int64_t(288));
+            put(mb, 8, uint64_t(n32));
+            put(mb, 16, P->size);
+            put(mb, 20, P->semantic);
+            put(mb, 24, P->dataClass);
+            put(mb, 28, P->dataType);
+            std::memcpy(mb + 32, P->name, std::min<size_t>(std::strlen(P->name), 255));
+            k_v2g_header<<<1, 256, 0, s>>>(buf + L.offMeta, meta);
+            k_p2g_blind<<<blocksFor(n32, kB), kB, 0, s>>>(ids2, n32, *P, buf + L.offMeta + 288u);
+        }
         k_v2g_lowers<<<blocksFor(tot.m, kWaves), kB, 0, s>>>(lowerFirst, tot.m, L, buf);
         k_v2g_uppers<<<blocksFor(tot.u, kWaves), kB, 0, s>>>(upperFirst, tot.u, L, buf);
         // complete before returning, as PointsToGrid::getHandle does (:537): the grid may be used
@@ -655,8 +806,18 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
         if (hipGetLastError() == hipSuccess)
             rc = makeGridHandle(reinterpret_cast<const uint8_t*>(hdr.w), L.total, dbuf, kOwnerPool, s, out);
     }
+    uint32_t most = 0;
+    if (rc == NVQ_OK && P &&
+        (hipMemcpyAsync(&most, maxCount, sizeof(most), hipMemcpyDeviceToHost, s) != hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess || most > 65535u)) {
+        // a leaf with more points than its uint16 values can count (the reference throws)
+        (void)nvq_grid_free(*out);
+        *out = nullptr;
+        rc = most > 65535u ? NVQ_ERR_INVALID_ARG : NVQ_ERR_HIP;
+        dbuf = nullptr;
+    }
     if (extra) (void)hipFreeAsync(extra, s);
-    if (rc != NVQ_OK) (void)hipFreeAsync(dbuf, s);
+    if (rc != NVQ_OK && dbuf) (void)hipFreeAsync(dbuf, s);
     // the workspace is free for the next build only once this stream has drained
     if (hipStreamSynchronize(s) != hipSuccess && rc == NVQ_OK) {
         (void)nvq_grid_free(*out);
@@ -666,25 +827,11 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     return rc;
 }
 
-} // namespace
-
-extern "C" {
-
-int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
-                       uint64_t* index, void* stream, nvq_grid** out)
+// From the input voxels on: bbox, key width, then buildSorted.  The caller holds W's lock.
+int buildFromVoxels(Workspace& W, const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size,
+                    const char* name, uint64_t* index, const PointArgs* P, hipStream_t s, nvq_grid** out)
 {
-    if (!out) return NVQ_ERR_INVALID_ARG;
-    *out = nullptr;
-    if (!ijk || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size))
-        return NVQ_ERR_INVALID_ARG;
-    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
-    if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
-    const hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t n32 = uint32_t(n);
-    int device = 0;
-    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
-    Workspace& W = workspaceOf(device);
-    std::lock_guard<std::mutex> hold(W.lock);
 
     // 1. bbox of the input, read back to size the dense tile numbering
     uint8_t* ws = W.reserve(carveBytes<int32_t>(6 * kBoundBlocks) + carveBytes<int32_t>(6));
@@ -714,14 +861,76 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
     }
     const int lowerBits = lowers > 1 ? 64 - __builtin_clzll(lowers - 1) : 0;
     L.isIndex = grid_type == GT_ONINDEX;
+    L.isPoint = P != nullptr;
     int rc;
     if (tiles == 1 && 21 + lowerBits <= 32)
-        rc = buildSorted<uint32_t>(W, ijk, n32, L, 21 + lowerBits, box, voxel_size, name, grid_type, index, s, out);
+        rc = buildSorted<uint32_t>(W, ijk, n32, L, 21 + lowerBits, box, voxel_size, name, grid_type, index, P, s, out);
     else if (36 + tileBits > 64)
         return NVQ_ERR_INVALID_ARG; // input spans more than 2^28 upper-node tiles
     else
-        rc = buildSorted<uint64_t>(W, ijk, n32, L, 36 + tileBits, box, voxel_size, name, grid_type, index, s, out);
+        rc = buildSorted<uint64_t>(W, ijk, n32, L, 36 + tileBits, box, voxel_size, name, grid_type, index, P, s, out);
     if (rc != NVQ_OK) (void)hipStreamSynchronize(s); // no kernel may still use the workspace
+    return rc;
+}
+
+
+} // namespace
+
+extern "C" {
+
+int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
+                       uint64_t* index, void* stream, nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!ijk || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size))
+        return NVQ_ERR_INVALID_ARG;
+    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
+    Workspace& W = workspaceOf(device);
+    std::lock_guard<std::mutex> hold(W.lock);
+    return buildFromVoxels(W, ijk, n, grid_type, voxel_size, name, index, nullptr, static_cast<hipStream_t>(stream), out);
+}
+
+int nvq_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                       void* stream, nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!xyz || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size) ||
+        point_type == 0 || point_type > 9)
+        return NVQ_ERR_INVALID_ARG;
+    // PointType -> blind data (processGridTreeRoot, tools/cuda/PointsToGrid.cuh:913-1001, float points)
+    static const struct {
+        uint32_t size, semantic, dataClass, dataType, gridClass;
+        const char* name;
+    } kinds[10] = {{0, 0, 0, 0, 0, ""},
+                   {4, 6, 1, 10, 4, "PointID: uint32_t indices to points"},
+                   {24, 7, 2, 7, 5, "World64: Vec3<double> point coordinates in world space"},
+                   {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"},
+                   {24, 8, 2, 7, 5, "Grid64: Vec3<double> point coordinates in grid space"},
+                   {12, 8, 2, 6, 5, "Grid32: Vec3<float> point coordinates in grid space"},
+                   {12, 9, 2, 6, 5, "Voxel32: Vec3<float> point coordinates in voxel space"},
+                   {6, 9, 2, 25, 5, "Voxel16: Vec3<uint16_t> point coordinates in voxel space"},
+                   {3, 9, 2, 24, 5, "Voxel8: Vec3<uint8_t> point coordinates in voxel space"},
+                   {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"}};
+    const auto& k = kinds[point_type];
+    PointArgs P{xyz, point_type, k.size, k.semantic, k.dataClass, k.dataType, k.gridClass, k.name,
+                1.0f / float(voxel_size), 1.0 / voxel_size};
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
+    Workspace& W = workspaceOf(device);
+    std::lock_guard<std::mutex> hold(W.lock);
+    keepPoolMemory();
+    int32_t* ijk = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+    k_p2g_voxels<<<blocksFor(n, kB), kB, 0, s>>>(xyz, uint32_t(n), P.invF, ijk);
+    const int rc = buildFromVoxels(W, ijk, n, 23u, voxel_size, name, nullptr, &P, s, out);
+    (void)hipFreeAsync(ijk, s);
+    (void)hipStreamSynchronize(s);
     return rc;
 }
 

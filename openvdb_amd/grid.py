@@ -11,8 +11,8 @@ import numpy as np
 
 from ._lib import GridInfo, check, lib
 
-GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN", 20: "OnIndex"}
-GRID_CLASSES = {0: "Unknown", 1: "LevelSet", 2: "FogVolume", 8: "IndexGrid"}
+GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN", 20: "OnIndex", 23: "PointIndex"}
+GRID_CLASSES = {0: "Unknown", 1: "LevelSet", 2: "FogVolume", 4: "PointIndex", 5: "PointData", 8: "IndexGrid"}
 
 
 def aligned_buffer(data):
@@ -87,6 +87,34 @@ class DeviceGrid:
         g.info = info
         g._host = g.download(stream, limit=1024)  # grid, tree and root headers
         return (g, index) if return_index else g
+
+    POINT_TYPES = {"PointID": 1, "World64": 2, "World32": 3, "Grid64": 4, "Grid32": 5, "Voxel32": 6, "Voxel16": 7,
+                   "Voxel8": 8, "Default": 9}
+
+    @classmethod
+    def from_points(cls, xyz, voxel_size=1.0, point_type="Default", name=None, stream=None):
+        """Build a point grid on the device from float world-space points: tools::cuda::pointsToGrid
+        (tools/cuda/PointsToGrid.cuh:57-64).  ``xyz`` is an [n, 3] float32 CUDA tensor; the points
+        are stored as blind data encoded by ``point_type`` (a key of POINT_TYPES).  The queries of
+        this library do not read point grids; ``download`` returns the NanoVDB buffer."""
+        import torch
+        if point_type not in cls.POINT_TYPES:
+            raise ValueError(f"point_type must be one of {sorted(cls.POINT_TYPES)}")
+        if not (xyz.is_cuda and xyz.dtype == torch.float32 and xyz.dim() == 2 and xyz.shape[1] == 3):
+            raise ValueError("xyz must be an [n, 3] float32 CUDA tensor")
+        xyz = xyz.contiguous()
+        handle = ctypes.c_void_p()
+        check(lib().nvq_points_to_grid(ctypes.c_void_p(xyz.data_ptr()), xyz.shape[0], float(voxel_size),
+                                       cls.POINT_TYPES[point_type], None if name is None else name.encode(),
+                                       _stream_ptr(stream), ctypes.byref(handle)), "nvq_points_to_grid")
+        g = cls.__new__(cls)
+        g.handle = handle
+        g.device_bytes = None
+        info = GridInfo()
+        check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
+        g.info = info
+        g._host = g.download(stream, limit=1024)
+        return g
 
     def download(self, stream=None, limit=None):
         """The grid buffer (or its first ``limit`` bytes) as a 32-byte aligned uint8 numpy array

@@ -62,6 +62,11 @@ void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, u
 int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
                       void** out, uint64_t* bytes_out, uint64_t* out_index);
 void or_free(void* p);
+/* tools/cuda/PointsToGrid.cuh pointsToGrid restated for n float world points xyz[n][3]: a Point
+ * grid (GridType PointIndex) with the points as blind data encoded by point_type (PointType
+ * 1..9).  0 ok, -1 bad argument, -3 more than 65535 points in one leaf. */
+int or_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                      void** out, uint64_t* bytes_out);
 /* tools/cuda/GridStats.cuh updateGridStats<float>(StatsMode::Default) restated, in place on a host
  * float grid buffer (bboxes, node statistics, flags, world bbox).  0 ok, -4 not a float grid. */
 int or_update_grid_stats(void* buf);

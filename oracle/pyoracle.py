@@ -62,6 +62,8 @@ class Oracle:
         L.or_free.argtypes = [_vp]
         L.or_update_grid_stats.argtypes = [_vp]
         L.or_signed_flood_fill.argtypes = [_vp]
+        L.or_points_to_grid.argtypes = [_vp, _u64, _d, ctypes.c_uint32, ctypes.c_char_p, ctypes.POINTER(_vp),
+                                        ctypes.POINTER(_u64)]
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -185,6 +187,19 @@ class Oracle:
             raise ValueError("not a float grid")
         return out
 
+    def points_to_grid(self, xyz, voxel_size=1.0, point_type=9, name=None):
+        """pointsToGrid restated for float world points: the Point grid buffer."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        p, nb = _vp(), _u64()
+        st = self.L.or_points_to_grid(_p(xyz), len(xyz), voxel_size, point_type,
+                                      None if name is None else name.encode(), ctypes.byref(p), ctypes.byref(nb))
+        if st != 0:
+            raise ValueError("or_points_to_grid failed: %d" % st)
+        buf = aligned_empty(nb.value)
+        ctypes.memmove(buf.ctypes.data, p.value, nb.value)
+        self.L.or_free(p)
+        return buf
+
     def signed_flood_fill(self, buf):
         """cuda::signedFloodFill<float> restated, on a copy of the float grid buffer."""
         out = aligned_empty(buf.nbytes)
@@ -254,6 +269,7 @@ class Reference:
         L.ref_grid_from_voxels.restype = _vp
         L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp]
         L.ref_update_grid_stats.argtypes = [_vp]
+        L.ref_point_voxels.argtypes = [_vp, _vp, _u64, _vp, _vp]
 
     def hw_threads(self):
         return int(self.L.ref_hw_threads())
@@ -364,6 +380,15 @@ class Reference:
         if st < 0:
             raise ValueError("grid type not covered by the reference dispatch")
         return st == 1
+
+    def point_voxels(self, buf, ijk):
+        """PointAccessor::voxelPoints per voxel: (point count, first position in the blind data)."""
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        count = np.zeros(len(ijk), np.uint64)
+        first = np.zeros(len(ijk), np.uint64)
+        if self.L.ref_point_voxels(_p(buf), _p(ijk), len(ijk), _p(count), _p(first)) != 0:
+            raise ValueError("not a point grid")
+        return count, first
 
     def update_grid_stats(self, buf):
         """Host tools::updateGridStats(StatsMode::Default) on a copy of a float grid buffer."""

@@ -398,6 +398,8 @@ void* ref_index_grid(const void* float_buf, uint32_t channels, int include_stats
 // -4 for a grid type the dispatch does not cover.
 int ref_validate(const void* buf, int full)
 {
+    if (reinterpret_cast<const GridData*>(buf)->mGridType == GridType::PointIndex)
+        return tools::isValid(asGrid<Point>(buf), full ? CheckMode::Full : CheckMode::Half, false) ? 1 : 0;
     return dispatch(buf, [&](auto* grid) {
         return tools::isValid(grid, full ? CheckMode::Full : CheckMode::Half, false) ? 1 : 0;
     });
@@ -433,6 +435,34 @@ int ref_update_grid_stats(void* buf)
     if (reinterpret_cast<const GridData*>(buf)->mGridType != GridType::Float) return -4;
     tools::updateGridStats(reinterpret_cast<NanoGrid<float>*>(buf), tools::StatsMode::Default);
     return 0;
+}
+
+// PointAccessor<AttT, Point>::voxelPoints (NanoVDB.h:5655-5716): for each voxel, the number of its
+// points and the position of the first one in the grid's blind data.  AttT follows the blind
+// data's value size (4: uint32 ids, 12: Vec3f, 24: Vec3d, 6: Vec3u16, 3: Vec3u8).
+int ref_point_voxels(const void* buf, const int32_t* ijk, uint64_t n, uint64_t* count, uint64_t* first)
+{
+    const auto* grid = asGrid<Point>(buf);
+    if (grid->gridType() != GridType::PointIndex || grid->blindDataCount() != 1) return -4;
+    auto run = [&](auto tag) {
+        using AttT = decltype(tag);
+        PointAccessor<AttT, Point> acc(*grid);
+        const AttT *begin = nullptr, *end = nullptr, *base = nullptr;
+        acc.gridPoints(base, end);
+        for (uint64_t i = 0; i < n; ++i) {
+            count[i] = acc.voxelPoints(Coord(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]), begin, end);
+            first[i] = count[i] ? uint64_t(begin - base) : 0;
+        }
+        return 0;
+    };
+    switch (grid->blindMetaData(0).mValueSize) {
+    case 4: return run(uint32_t());
+    case 12: return run(Vec3f());
+    case 24: return run(Vec3d());
+    case 6: return run(Vec3u16());
+    case 3: return run(Vec3u8());
+    default: return -4;
+    }
 }
 
 // ReadAccessor<ValueOnIndex>::getValue: the value index of each voxel of an index grid

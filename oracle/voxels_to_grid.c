@@ -79,13 +79,60 @@ static void bbox_expand(int32_t b[6], const int32_t lo[3], const int32_t hi[3])
     }
 }
 
-int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
-                      void** out, uint64_t* bytes_out, uint64_t* out_index)
+static void put_u16(uint8_t* p, uint16_t v) { memcpy(p, &v, 2); }
+
+/* Blind data of a point grid: PointType (NanoVDB.h:395-405) -> value size, GridBlindDataSemantic,
+ * GridBlindDataClass, GridType of the values and the name processGridTreeRoot writes
+ * (tools/cuda/PointsToGrid.cuh:913-1001, float input points). */
+typedef struct {
+    uint32_t size, semantic, dataClass, dataType, gridClass;
+    const char* name;
+} PointKind;
+static int point_kind(uint32_t t, PointKind* k)
+{
+    static const PointKind kinds[10] = {
+        {0, 0, 0, 0, 0, ""},
+        {4, 6, 1, 10, 4, "PointID: uint32_t indices to points"},
+        {24, 7, 2, 7, 5, "World64: Vec3<double> point coordinates in world space"},
+        {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"},
+        {24, 8, 2, 7, 5, "Grid64: Vec3<double> point coordinates in grid space"},
+        {12, 8, 2, 6, 5, "Grid32: Vec3<float> point coordinates in grid space"},
+        {12, 9, 2, 6, 5, "Voxel32: Vec3<float> point coordinates in voxel space"},
+        {6, 9, 2, 25, 5, "Voxel16: Vec3<uint16_t> point coordinates in voxel space"},
+        {3, 9, 2, 24, 5, "Voxel8: Vec3<uint8_t> point coordinates in voxel space"},
+        {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"}};
+    if (t == 0 || t > 9) return -1;
+    *k = kinds[t];
+    return 0;
+}
+
+/* Map(dx)::applyInverseMapF of a float point (matMult with mInvMatF, math/Math.h:1704-1709) */
+static void inv_map_f(float invF, const float p[3], float q[3])
+{
+    for (int r = 0; r < 3; ++r) {
+        const float m[3] = {r == 0 ? invF : 0.0f, r == 1 ? invF : 0.0f, r == 2 ? invF : 0.0f};
+        q[r] = fmaf(p[0], m[0], fmaf(p[1], m[1], p[2] * m[2]));
+    }
+}
+/* Map(dx)::applyInverseMap of a float point: FP64 matMult (math/Math.h:1716-1722), returned as
+ * the input's float type */
+static void inv_map_d(double inv, const float p[3], float q[3])
+{
+    for (int r = 0; r < 3; ++r) {
+        const double m[3] = {r == 0 ? inv : 0.0, r == 1 ? inv : 0.0, r == 2 ? inv : 0.0};
+        q[r] = (float)fma((double)p[0], m[0], fma((double)p[1], m[1], (double)p[2] * m[2]));
+    }
+}
+
+/* mode 1 float, 20 ValueOnIndex, 23 Point (pts and point_type then describe the points) */
+static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
+                      const float* pts, uint32_t point_type, void** out, uint64_t* bytes_out, uint64_t* out_index)
 {
     *out = NULL;
     *bytes_out = 0;
-    if (!ijk || n == 0 || (grid_type != GT_FLOAT_ && grid_type != GT_ONINDEX_)) return -1;
-    if (out_index && grid_type != GT_ONINDEX_) return -1;
+    const int pt = grid_type == 23u;
+    PointKind kind = {0, 0, 0, 0, 0, ""};
+    if (pt && (!pts || point_kind(point_type, &kind) != 0)) return -1;
     VKey* k = (VKey*)malloc(n * sizeof(VKey));
     if (!k) return -2;
     for (uint64_t i = 0; i < n; ++i) {
@@ -112,10 +159,11 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
         nVox += newTile || k[p].key != k[p - 1].key;
     }
     const int idx = grid_type == GT_ONINDEX_;
-    const uint64_t rootHdr = idx ? 96u : 64u, leafSize = idx ? 96u : 2144u;
+    const uint64_t rootHdr = (idx || pt) ? 96u : 64u, leafSize = idx ? 96u : pt ? 1120u : 2144u;
     const uint64_t offTree = 672, offRoot = offTree + 64, offUpper = offRoot + rootHdr + 32u * nUpper;
     const uint64_t offLower = offUpper + 270400u * nUpper, offLeaf = offLower + 33856u * nLower;
-    const uint64_t total = offLeaf + leafSize * nLeaf;
+    const uint64_t offMeta = offLeaf + leafSize * nLeaf;
+    const uint64_t total = pt ? offMeta + 288u + (uint64_t)kind.size * n : offMeta;
     uint8_t* b = (uint8_t*)aligned_alloc(32, (total + 31) & ~(uint64_t)31);
     if (!b) {
         free(k);
@@ -131,7 +179,9 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
     int32_t* upperBox = (int32_t*)malloc(6 * sizeof(int32_t) * nUpper);
     int64_t* lowerParent = (int64_t*)malloc(sizeof(int64_t) * nLower);
     int64_t* leafParent = (int64_t*)malloc(sizeof(int64_t) * nLeaf);
-    if (!lowerBox || !upperBox || !lowerParent || !leafParent) {
+    uint64_t* leafStart = (uint64_t*)malloc(sizeof(uint64_t) * (nLeaf + 1));
+    if (!lowerBox || !upperBox || !lowerParent || !leafParent || !leafStart) {
+        free(leafStart);
         free(k);
         free(b);
         free(lowerBox);
@@ -183,6 +233,8 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
             put_i32(b + oF + 8, oz);
             b[oF + 15] = 2u; /* HasBBox */
             if (idx) put_u64(b + oF + 80, (uint64_t)(v + 1) + 1u); /* 1 + active voxels before this leaf */
+            if (pt) put_u64(b + oF + 80, p);                       /* points before this leaf */
+            leafStart[l] = p;
             leafParent[l] = m;
         }
         if (newVox) {
@@ -190,16 +242,41 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
             const uint64_t oF = offLeaf + leafSize * (uint64_t)l;
             uint8_t* w = b + oF + 16 + 8 * (n0 >> 6);
             put_u64(w, get_u64(w) | ((uint64_t)1 << (n0 & 63)));
-            if (!idx) put_f32(b + oF + 96 + 4 * n0, 1.0f);
+            if (!idx && !pt) put_f32(b + oF + 96 + 4 * n0, 1.0f);
         }
+        /* Point leaves: the points of the leaf up to and including this voxel
+         * (SetLeafActiveVoxelStateAndValuesFunctor, tools/cuda/PointsToGrid.cuh:1156-1174) */
+        if (pt && (p + 1 == n || k[p + 1].key != k[p].key || memcmp(k[p + 1].t, k[p].t, sizeof(k[p].t)) != 0))
+            put_u16(b + offLeaf + leafSize * (uint64_t)l + 96 + 2 * n0, (uint16_t)(p + 1 - leafStart[l]));
         if (out_index) out_index[k[p].id] = (uint64_t)v + 1u;
     }
 
     /* leaves: updateBBox, prefix sums; bboxes up the tree (processBBox) */
+    leafStart[nLeaf] = n;
     for (uint64_t i = 0; i < nLeaf; ++i) {
         uint8_t* L = b + offLeaf + leafSize * i;
         uint64_t w[8];
         for (int j = 0; j < 8; ++j) w[j] = get_u64(L + 16 + 8 * j);
+        if (pt) {
+            if (leafStart[i + 1] - leafStart[i] > 65535u) { /* the reference throws: too many points per leaf */
+                free(k);
+                free(b);
+                free(lowerBox);
+                free(upperBox);
+                free(lowerParent);
+                free(leafParent);
+                free(leafStart);
+                return -3;
+            }
+            put_u64(L + 88, leafStart[i + 1] - leafStart[i]);
+            /* inactive voxels: the value of the previous active voxel, else 0
+             * (SetLeafInactiveVoxelValuesFunctor, :1176-1191) */
+            uint16_t last = 0;
+            for (uint32_t q = 0; q < 512; ++q) {
+                if ((w[q >> 6] >> (q & 63)) & 1u) memcpy(&last, L + 96 + 2 * q, 2);
+                else put_u16(L + 96 + 2 * q, last);
+            }
+        }
         uint64_t word = w[0];
         uint32_t xmin = w[0] ? 0u : 8u, xmax = xmin;
         for (int j = 1; j < 8; ++j)
@@ -300,20 +377,100 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
         apply_map(voxel_size, zero, p0);
         for (int a = 0; a < 3; ++a) put_f64(b + 608 + 8 * a, p1[a] - p0[a]);
     }
-    put_u32(b + 632, idx ? 8u : 0u); /* GridClass IndexGrid / Unknown */
+    put_u32(b + 632, idx ? 8u : pt ? kind.gridClass : 0u); /* GridClass IndexGrid / Point* / Unknown */
     put_u32(b + 636, grid_type);
-    put_i64(b + 640, (int64_t)total); /* blind metadata offset: none follows */
-    put_u32(b + 648, 0u);
-    put_u64(b + 656, idx ? nVox + 1u : 0u);
+    put_i64(b + 640, (int64_t)offMeta); /* blind metadata: none follows unless this is a point grid */
+    put_u32(b + 648, pt ? 1u : 0u);
+    put_u64(b + 656, idx ? nVox + 1u : pt ? n : 0u);
+    if (pt) {
+        /* GridBlindMetaData (NanoVDB.h:1591-1601), then the points in sorted order (encodePoints,
+         * tools/cuda/PointsToGrid.cuh:1262-1317) */
+        uint8_t* M = b + offMeta;
+        put_i64(M, 288);
+        put_u64(M + 8, n);
+        put_u32(M + 16, kind.size);
+        put_u32(M + 20, kind.semantic);
+        put_u32(M + 24, kind.dataClass);
+        put_u32(M + 28, kind.dataType);
+        memcpy(M + 32, kind.name, strlen(kind.name));
+        uint8_t* D = M + 288;
+        const float invF = 1.0f / (float)voxel_size;
+        const double invD = 1.0 / voxel_size;
+        for (uint64_t q = 0; q < n; ++q) {
+            const uint64_t id = k[q].id;
+            const float* w = pts + 3 * id;
+            uint8_t* o = D + (uint64_t)kind.size * q;
+            float g[3];
+            switch (point_type) {
+            case 1: put_u32(o, (uint32_t)id); break;
+            case 2:
+                for (int a = 0; a < 3; ++a) put_f64(o + 8 * a, (double)w[a]);
+                break;
+            case 3:
+            case 9: memcpy(o, w, 12); break;
+            case 4:
+                inv_map_d(invD, w, g);
+                for (int a = 0; a < 3; ++a) put_f64(o + 8 * a, (double)g[a]);
+                break;
+            case 5:
+                inv_map_f(invF, w, g);
+                memcpy(o, g, 12);
+                break;
+            default: { /* worldToVoxel (:197-236): FP64 offsets from the voxel centre */
+                inv_map_d(invD, w, g);
+                for (int a = 0; a < 3; ++a) {
+                    const double x = (double)g[a];
+                    const double f = x - (double)(int32_t)floor(x + 0.5);
+                    if (point_type == 6) put_f32(o + 4 * a, (float)f);
+                    else if (point_type == 7) put_u16(o + 2 * a, (uint16_t)(65535.0 * (f + 0.5)));
+                    else o[a] = (uint8_t)(255.0 * (f + 0.5));
+                }
+            }
+            }
+        }
+    }
 
     free(k);
     free(lowerBox);
     free(upperBox);
     free(lowerParent);
     free(leafParent);
+    free(leafStart);
     *out = b;
     *bytes_out = total;
     return 0;
 }
 
 void or_free(void* p) { free(p); }
+
+int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
+                      void** out, uint64_t* bytes_out, uint64_t* out_index)
+{
+    *out = NULL;
+    *bytes_out = 0;
+    if (!ijk || n == 0 || (grid_type != GT_FLOAT_ && grid_type != GT_ONINDEX_)) return -1;
+    if (out_index && grid_type != GT_ONINDEX_) return -1;
+    return build_grid(ijk, n, grid_type, voxel_size, name, NULL, 0, out, bytes_out, out_index);
+}
+
+/* tools/cuda/PointsToGrid.cuh pointsToGrid(d_points, n, voxelSize, type) (:57-64) for float world
+ * points: each point's voxel is Round(Map(voxelSize).applyInverseMapF(p)) (TileKeyFunctor,
+ * :553-580; Vec3<float>::round, math/Math.h:1515-1524). */
+int or_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                      void** out, uint64_t* bytes_out)
+{
+    *out = NULL;
+    *bytes_out = 0;
+    if (!xyz || n == 0) return -1;
+    int32_t* ijk = (int32_t*)malloc(sizeof(int32_t) * 3 * n);
+    if (!ijk) return -2;
+    const float invF = 1.0f / (float)voxel_size;
+    for (uint64_t i = 0; i < n; ++i) {
+        float q[3];
+        inv_map_f(invF, xyz + 3 * i, q);
+        for (int a = 0; a < 3; ++a) ijk[3 * i + a] = (int32_t)floorf(q[a] + 0.5f);
+    }
+    const int rc = build_grid(ijk, n, 23u, voxel_size, name, xyz, point_type, out, bytes_out, NULL);
+    free(ijk);
+    return rc;
+}
