@@ -308,6 +308,46 @@ def build_bench(torch, nvq, ijk, steps, warmup):
             "stages_ms": stages, "grid_bytes": int(gi.bytes), "leaves": int(gi.node_count[0])}
 
 
+def points_bench(torch, nvq, xyz, voxel_size, steps, warmup):
+    """Device-side point grid (nvq_points_to_grid, World32 blind data) from float world points:
+    wall time per build including its allocations, its host reads and the final stream wait."""
+    import ctypes
+    lib = nvq.lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def build_and_free():
+        h = ctypes.c_void_p()
+        st = lib.nvq_points_to_grid(ctypes.c_void_p(xyz.data_ptr()), len(xyz), voxel_size, 3, None, stream,
+                                    ctypes.byref(h))
+        if st != 0:
+            raise RuntimeError(f"nvq_points_to_grid failed: {st}")
+        from openvdb_amd._lib import GridInfo
+        gi = GridInfo()
+        lib.nvq_grid_get_info(h, ctypes.byref(gi))
+        build_and_free.info = (int(gi.bytes), int(gi.node_count[0]))
+        lib.nvq_grid_free(h)
+    wall, _ = timed_launches(torch, [build_and_free], steps, warmup)
+    nbytes, leaves = build_and_free.info
+    return {"points": len(xyz), "ms": wall * 1e3 / steps, "points_per_s": len(xyz) * steps / wall,
+            "grid_bytes": nbytes, "leaves": leaves}
+
+
+def points_cpu_baseline(xyz_np, voxel_size, log2n):
+    """The oracle's single-thread restatement of pointsToGrid on the first 2^log2n points (the
+    reference builds point grids only on the device), median of 3."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle
+    o = Oracle()
+    sample = np.ascontiguousarray(xyz_np[: 1 << log2n])
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.points_to_grid(sample, voxel_size, 3)
+        times.append(time.perf_counter() - t0)
+    return {"value": len(sample) / float(np.median(times)), "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": f"first 2^{log2n} of the same points, World32, median of 3"}
+
+
 def inplace_bench(torch, nvq, host_grid, fn_name, steps, warmup):
     """Time one in-place device tool (nvq_update_grid_stats / nvq_signed_flood_fill) on a device
     copy of host_grid: wall ms per call (each call waits for its stream) and grid GB/s."""
@@ -551,6 +591,14 @@ def main():
                     extra[k]["cpu_baseline"] = v
             log(f"updateGridStats fog: {extra['grid_stats']['ms']:.2f} ms; "
                 f"signedFloodFill torus: {extra['signed_flood_fill']['ms']:.2f} ms")
+            pts = torch.randn((1 << 24, 3), generator=gen, device="cuda") * 100.0
+            extra["points_to_grid"] = points_bench(torch, nvq, pts, 0.5, 5, 2)
+            if not args.no_cpu:
+                extra["points_to_grid"]["cpu_baseline"] = points_cpu_baseline(pts.cpu().numpy(), 0.5, 21)
+            b = extra["points_to_grid"]
+            log(f"pointsToGrid: {b['points']/1e6:.1f} M points in {b['ms']:.1f} ms "
+                f"({b['points_per_s']/1e9:.2f} G points/s, {b['leaves']} leaves)")
+            del pts
 
     cpu = None
     if not args.no_cpu:

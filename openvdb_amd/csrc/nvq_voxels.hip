@@ -363,7 +363,8 @@ __global__ __launch_bounds__(kB) void k_p2g_leaf_values(const K* __restrict__ ke
     if (lane == 0) {
         *reinterpret_cast<uint64_t*>(F + 80) = a;
         *reinterpret_cast<uint64_t*>(F + 88) = b - a;
-        atomicMax(maxCount, b - a);
+        // only an overfull leaf touches the flag: one address hit by every leaf serialises them
+        if (b - a > 65535u) atomicMax(maxCount, b - a);
     }
 }
 
