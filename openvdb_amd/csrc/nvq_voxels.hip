@@ -511,6 +511,111 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves(const K* __restrict__ keys, c
     }
 }
 
+// Sparse inputs (fewer than 64 sorted keys per leaf on average): eight lanes per leaf, lane j
+// owning voxels 64j .. 64j+63, i.e. mask word j and that stretch of the value array.  The leaf's
+// keys are sorted by voxel offset, so each lane binary-searches its stretch and walks it once.
+// Writes the same bytes as k_v2g_leaves and, for Point leaves, k_p2g_leaf_values: a voxel's
+// point value is the number of the leaf's points at or before it, which is the count at the
+// previous active voxel for an inactive one, or 0 before the first.
+template<typename K>
+__global__ __launch_bounds__(kB) void k_v2g_leaves_sparse(const K* __restrict__ keys, const uint32_t* __restrict__ leafRank,
+                                                          const uint32_t* __restrict__ leafFirst, uint32_t nLeaf, Layout L,
+                                                          uint8_t* __restrict__ buf, uint32_t* __restrict__ maxCount)
+{
+    const uint32_t lane = threadIdx.x & 63u, sub = lane & 7u;
+    const uint32_t l = (blockIdx.x * kB + threadIdx.x) >> 3;
+    if (l >= nLeaf) return; // whole groups of eight leave together
+    const uint32_t a = leafFirst[l], b = leafFirst[l + 1];
+    const uint32_t lo = 64u * sub;
+    // first key of the stretch: lower bound of lo among the leaf's voxel offsets
+    uint32_t q0 = a, hiq = b;
+    while (q0 < hiq) {
+        const uint32_t mid = (q0 + hiq) >> 1;
+        if ((uint32_t(keys[mid]) & 511u) < lo) q0 = mid + 1u;
+        else hiq = mid;
+    }
+    uint64_t word = 0;
+    uint32_t p = q0, n0 = p < b ? uint32_t(keys[p]) & 511u : 512u;
+    uint8_t* F = buf + L.offLeaf + uint64_t(L.leafSize) * l;
+    if (L.isPoint) {
+        // values lo .. lo+63: points of the leaf with voxel offset <= v
+        uint32_t packed[32];
+#pragma unroll
+        for (uint32_t e = 0; e < 64u; ++e) {
+            while (n0 <= lo + e) {
+                word |= 1ull << e;
+                n0 = ++p < b ? uint32_t(keys[p]) & 511u : 512u;
+            }
+            const uint32_t c = (p - a) & 0xffffu;
+            if (e & 1u) packed[e >> 1] |= c << 16;
+            else packed[e >> 1] = c;
+        }
+        uint4* dst = reinterpret_cast<uint4*>(F + 96 + 128u * sub);
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) dst[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+    } else {
+        while (n0 < lo + 64u) {
+            word |= 1ull << (n0 - lo);
+            n0 = ++p < b ? uint32_t(keys[p]) & 511u : 512u;
+        }
+        if (!L.isIndex) { // float leaves: 1 for an active voxel, 0 otherwise
+            float4* dst = reinterpret_cast<float4*>(F + 96 + 256u * sub);
+#pragma unroll
+            for (uint32_t k = 0; k < 16u; ++k) {
+                const uint32_t bits = uint32_t(word >> (4u * k)) & 15u;
+                dst[k] = make_float4((bits & 1u) ? 1.0f : 0.0f, (bits & 2u) ? 1.0f : 0.0f, (bits & 4u) ? 1.0f : 0.0f,
+                                     (bits & 8u) ? 1.0f : 0.0f);
+            }
+        }
+    }
+    *reinterpret_cast<uint64_t*>(F + 16 + 8u * sub) = word;
+    // the group's words: x extent from which words are set, y and z extents from their union
+    const uint32_t xbits = uint32_t(__ballot(word != 0ull) >> (lane & ~7u)) & 255u;
+    uint64_t any = word;
+    any |= __shfl_xor(any, 4, 64);
+    any |= __shfl_xor(any, 2, 64);
+    any |= __shfl_xor(any, 1, 64);
+    // inclusive prefix of the per-word active counts (index grids)
+    uint32_t pc = uint32_t(__popcll(word));
+#pragma unroll
+    for (uint32_t off = 1; off < 8u; off <<= 1) {
+        const uint32_t t = __shfl_up(pc, off, 8);
+        if (sub >= off) pc += t;
+    }
+    uint64_t prefix = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 7u; ++j) prefix |= uint64_t(__shfl(pc, (lane & ~7u) + j, 64)) << (9u * j);
+    if (sub != 0u) return;
+    const K k0 = keys[a];
+    int32_t t[3];
+    uint32_t n2;
+    nodeOf(L, k0, t, n2);
+    const uint32_t n1 = uint32_t(k0 >> 9) & 4095u;
+    const int32_t origin[3] = {t[0] * 4096 + int32_t((n2 >> 10) & 31u) * 128 + int32_t((n1 >> 8) & 15u) * 8,
+                               t[1] * 4096 + int32_t((n2 >> 5) & 31u) * 128 + int32_t((n1 >> 4) & 15u) * 8,
+                               t[2] * 4096 + int32_t(n2 & 31u) * 128 + int32_t(n1 & 15u) * 8};
+    const uint32_t xmin = uint32_t(__builtin_ctz(xbits)), xmax = 31u - uint32_t(__builtin_clz(xbits));
+    const uint32_t ymin = uint32_t(__builtin_ctzll(any)) >> 3, ymax = (63u - uint32_t(__builtin_clzll(any))) >> 3;
+    const uint32_t w32 = uint32_t(any) | uint32_t(any >> 32);
+    const uint32_t w16 = (w32 & 0xffffu) | (w32 >> 16);
+    const uint32_t byte = (w16 & 0xffu) | (w16 >> 8);
+    const uint32_t zmin = uint32_t(__builtin_ctz(byte)), zmax = 31u - uint32_t(__builtin_clz(byte));
+    int4 h;
+    h.x = origin[0] + int32_t(xmin);
+    h.y = origin[1] + int32_t(ymin);
+    h.z = origin[2] + int32_t(zmin);
+    h.w = int32_t((xmax - xmin) | ((ymax - ymin) << 8) | ((zmax - zmin) << 16) | (2u << 24)); // dif[3], flags HasBBox
+    *reinterpret_cast<int4*>(F) = h;
+    if (L.isIndex) {
+        *reinterpret_cast<uint64_t*>(F + 80) = uint64_t(leafRank[l]); // 1 + rank of the leaf's first voxel
+        *reinterpret_cast<uint64_t*>(F + 88) = prefix;
+    } else if (L.isPoint) {
+        *reinterpret_cast<uint64_t*>(F + 80) = a;
+        *reinterpret_cast<uint64_t*>(F + 88) = b - a;
+        if (b - a > 65535u) atomicMax(maxCount, b - a);
+    }
+}
+
 __device__ __forceinline__ void waveBoxReduce(int32_t lo[3], int32_t hi[3])
 {
 #pragma unroll
@@ -779,14 +884,21 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
             k_v2g_heads<K><<<chunks, kB, 0, s>>>(keys2, ids2, chunkIncl, n32, L, buf, leafFirst, leafRank, lowerFirst,
                                               upperFirst, index);
         }
+        // fewer than 64 keys per leaf on average: eight lanes per leaf, else a wavefront per leaf
+        const bool sparse = uint64_t(n32) < 64ull * tot.l;
+        if (P) (void)hipMemsetAsync(maxCount, 0, sizeof(uint32_t), s);
         {
             ProfScope prof("k_v2g_leaves", s);
-            k_v2g_leaves<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafRank, leafFirst, tot.l, L, buf);
+            if (sparse)
+                k_v2g_leaves_sparse<K><<<blocksFor(uint64_t(tot.l) * 8u, kB), kB, 0, s>>>(keys2, leafRank, leafFirst, tot.l,
+                                                                                      L, buf, maxCount);
+            else
+                k_v2g_leaves<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafRank, leafFirst, tot.l, L, buf);
         }
         if (P) {
             // Point grids: leaf values, offsets and counts, the blind metadata and the points
-            (void)hipMemsetAsync(maxCount, 0, sizeof(uint32_t), s);
-            k_p2g_leaf_values<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafFirst, tot.l, L, buf, maxCount);
+            if (!sparse)
+                k_p2g_leaf_values<K><<<blocksFor(tot.l, kWaves), kB, 0, s>>>(keys2, leafFirst, tot.l, L, buf, maxCount);
             Header meta{};
             meta.bytes = 288u;
             uint8_t* mb = reinterpret_cast<uint8_t*>(meta.w);
