@@ -209,6 +209,10 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
  * Point grids are built and downloaded here; the queries of this library do not read them. */
 int nvq_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
                        void* stream, nvq_grid** out);
+/* The same for double world points (PtrT = Vec3d*, tools/cuda/PointsToGrid.cuh:571-572): voxels
+ * are rounded in FP64, and point_type 9 (Default) stores the points as Vec3<double>. */
+int nvq_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                           void* stream, nvq_grid** out);
 /* Copy the first `bytes` bytes (at most nvq_grid_info.bytes; pass that for the whole grid) of a
  * grid's device buffer to host memory and wait for it: GridHandle::deviceDownload
  * (GridHandle.h:186-194). */

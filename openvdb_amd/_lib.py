@@ -65,6 +65,7 @@ SIGNATURES = {
     "nvq_grad_fixed_to_float": ([_vp, _u64, _i, _vp, _vp], _i),
     "nvq_voxels_to_grid": ([_vp, _u64, _u32, _d, ctypes.c_char_p, _vp, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_points_to_grid": ([_vp, _u64, _d, _u32, ctypes.c_char_p, _vp, ctypes.POINTER(_vp)], _i),
+    "nvq_points_to_grid_f64": ([_vp, _u64, _d, _u32, ctypes.c_char_p, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_grid_download": ([_vp, _vp, _u64, _vp], _i),
     "nvq_update_grid_stats": ([_vp, _vp], _i),
     "nvq_signed_flood_fill": ([_vp, _vp], _i),

@@ -69,7 +69,8 @@ struct Layout {
 
 // Point grids from float world points (pointsToGrid, tools/cuda/PointsToGrid.cuh:57-64).
 struct PointArgs {
-    const float* xyz;
+    const void* xyz;    // float or double (f64) world points, [n][3]
+    uint32_t f64;
     uint32_t type;      // PointType 1..9 (NanoVDB.h:395-405)
     uint32_t size;      // bytes per encoded point
     uint32_t semantic, dataClass, dataType, gridClass;
@@ -307,18 +308,34 @@ __device__ __forceinline__ void invMapD(double inv, const float p[3], float q[3]
     q[2] = float(__builtin_fma(x, 0.0, __builtin_fma(y, 0.0, z * inv)));
 }
 
-// Voxel of each point: Round(applyInverseMapF(p)) (TileKeyFunctor :553-580, Vec3<float>::round,
-// math/Math.h:1515-1524).
-__global__ __launch_bounds__(kB) void k_p2g_voxels(const float* __restrict__ xyz, uint32_t n, float invF,
+// Map(dx)::applyInverseMap of a double point: FP64 matMult, kept in double
+__device__ __forceinline__ void invMapDD(double inv, const double p[3], double q[3])
+{
+    q[0] = __builtin_fma(p[0], inv, __builtin_fma(p[1], 0.0, p[2] * 0.0));
+    q[1] = __builtin_fma(p[0], 0.0, __builtin_fma(p[1], inv, p[2] * 0.0));
+    q[2] = __builtin_fma(p[0], 0.0, __builtin_fma(p[1], 0.0, p[2] * inv));
+}
+
+// Voxel of each point (TileKeyFunctor :553-580): Round(applyInverseMapF(p)) for float points,
+// Round(applyInverseMap(p)) in FP64 for double points (Vec3<T>::round, math/Math.h:1515-1524).
+template<typename T>
+__global__ __launch_bounds__(kB) void k_p2g_voxels(const T* __restrict__ xyz, uint32_t n, float invF, double invD,
                                                    int32_t* __restrict__ ijk)
 {
     const uint32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= n) return;
-    const float p[3] = {xyz[3ull * i], xyz[3ull * i + 1], xyz[3ull * i + 2]};
-    float q[3];
-    invMapF(invF, p, q);
+    const T p[3] = {xyz[3ull * i], xyz[3ull * i + 1], xyz[3ull * i + 2]};
+    if constexpr (std::is_same<T, double>::value) {
+        double q[3];
+        invMapDD(invD, p, q);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) ijk[3ull * i + a] = int32_t(floorf(q[a] + 0.5f));
+        for (int a = 0; a < 3; ++a) ijk[3ull * i + a] = int32_t(floor(q[a] + 0.5));
+    } else {
+        float q[3];
+        invMapF(invF, p, q);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ijk[3ull * i + a] = int32_t(floorf(q[a] + 0.5f));
+    }
 }
 
 // One wavefront per Point leaf: mOffset (points before the leaf), mPointCount, and per voxel the
@@ -369,42 +386,61 @@ __global__ __launch_bounds__(kB) void k_p2g_leaf_values(const K* __restrict__ ke
 }
 
 // The points in sorted order as blind data (encodePoints :1262-1317, worldToVoxel :197-236).
+// Double points keep FP64 through Grid64 and the voxel offsets; Grid32 maps the float-cast
+// coordinates (applyInverseMapF casts each one, math/Math.h:1704-1709); Default stores the input.
+template<typename T>
 __global__ __launch_bounds__(kB) void k_p2g_blind(const uint32_t* __restrict__ ids, uint32_t n, PointArgs P,
                                                   uint8_t* __restrict__ out)
 {
+    constexpr bool kF64 = std::is_same<T, double>::value;
     const uint32_t q = blockIdx.x * kB + threadIdx.x;
     if (q >= n) return;
     const uint32_t id = ids[q];
-    const float w[3] = {P.xyz[3ull * id], P.xyz[3ull * id + 1], P.xyz[3ull * id + 2]};
+    const T* xyz = static_cast<const T*>(P.xyz);
+    const T w[3] = {xyz[3ull * id], xyz[3ull * id + 1], xyz[3ull * id + 2]};
+    const float wf[3] = {float(w[0]), float(w[1]), float(w[2])};
     uint8_t* o = out + uint64_t(P.size) * q;
-    float g[3];
+    // index-space position: FP64 for double points, FP64 rounded to float for float points
+    auto indexPos = [&](double g[3]) {
+        if constexpr (kF64) {
+            invMapDD(P.invD, w, g);
+        } else {
+            float gf[3];
+            invMapD(P.invD, wf, gf);
+            for (int a = 0; a < 3; ++a) g[a] = double(gf[a]);
+        }
+    };
+    double g[3];
+    float gf[3];
     switch (P.type) {
     case 1: *reinterpret_cast<uint32_t*>(o) = id; break;
     case 2:
         for (int a = 0; a < 3; ++a) reinterpret_cast<double*>(o)[a] = double(w[a]);
         break;
     case 4:
-        invMapD(P.invD, w, g);
-        for (int a = 0; a < 3; ++a) reinterpret_cast<double*>(o)[a] = double(g[a]);
+        indexPos(g);
+        for (int a = 0; a < 3; ++a) reinterpret_cast<double*>(o)[a] = g[a];
         break;
     case 5:
-        invMapF(P.invF, w, g);
-        for (int a = 0; a < 3; ++a) reinterpret_cast<float*>(o)[a] = g[a];
+        invMapF(P.invF, wf, gf);
+        for (int a = 0; a < 3; ++a) reinterpret_cast<float*>(o)[a] = gf[a];
         break;
     case 6:
     case 7:
     case 8:
-        invMapD(P.invD, w, g);
+        indexPos(g);
         for (int a = 0; a < 3; ++a) {
-            const double x = double(g[a]);
-            const double f = x - double(int32_t(floor(x + 0.5)));
+            const double f = g[a] - double(int32_t(floor(g[a] + 0.5)));
             if (P.type == 6) reinterpret_cast<float*>(o)[a] = float(f);
             else if (P.type == 7) reinterpret_cast<uint16_t*>(o)[a] = uint16_t(65535.0 * (f + 0.5));
             else o[a] = uint8_t(255.0 * (f + 0.5));
         }
         break;
-    default:
-        for (int a = 0; a < 3; ++a) reinterpret_cast<float*>(o)[a] = w[a];
+    case 9:
+        for (int a = 0; a < 3; ++a) reinterpret_cast<T*>(o)[a] = w[a];
+        break;
+    default: // 3: World32
+        for (int a = 0; a < 3; ++a) reinterpret_cast<float*>(o)[a] = wf[a];
     }
 }
 
@@ -764,10 +800,11 @@ struct Workspace {
         return base;
     }
 };
+constexpr int kMaxDevices = 64; // devices per process with a builder workspace
 Workspace& workspaceOf(int device)
 {
-    static Workspace spaces[64];
-    return spaces[device & 63];
+    static Workspace spaces[kMaxDevices];
+    return spaces[device];
 }
 
 // Carves aligned arrays out of one block of device memory.
@@ -910,7 +947,10 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
             put(mb, 28, P->dataType);
             std::memcpy(mb + 32, P->name, std::min<size_t>(std::strlen(P->name), 255));
             k_v2g_header<<<1, 256, 0, s>>>(buf + L.offMeta, meta);
-            k_p2g_blind<<<blocksFor(n32, kB), kB, 0, s>>>(ids2, n32, *P, buf + L.offMeta + 288u);
+            if (P->f64)
+                k_p2g_blind<double><<<blocksFor(n32, kB), kB, 0, s>>>(ids2, n32, *P, buf + L.offMeta + 288u);
+            else
+                k_p2g_blind<float><<<blocksFor(n32, kB), kB, 0, s>>>(ids2, n32, *P, buf + L.offMeta + 288u);
         }
         k_v2g_lowers<<<blocksFor(tot.m, kWaves), kB, 0, s>>>(lowerFirst, tot.m, L, buf);
         k_v2g_uppers<<<blocksFor(tot.u, kWaves), kB, 0, s>>>(upperFirst, tot.u, L, buf);
@@ -986,6 +1026,49 @@ int buildFromVoxels(Workspace& W, const int32_t* ijk, uint64_t n, uint32_t grid_
     return rc;
 }
 
+// pointsToGrid for float or double world points (PtrT = Vec3f* or Vec3d*).
+template<typename T>
+int pointsToGrid(const T* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name, void* stream,
+                 nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!xyz || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size) ||
+        point_type == 0 || point_type > 9)
+        return NVQ_ERR_INVALID_ARG;
+    // PointType -> blind data (processGridTreeRoot, tools/cuda/PointsToGrid.cuh:913-1001); Default
+    // describes the input type
+    static const struct {
+        uint32_t size, semantic, dataClass, dataType, gridClass;
+        const char* name;
+    } kinds[10] = {{0, 0, 0, 0, 0, ""},
+                   {4, 6, 1, 10, 4, "PointID: uint32_t indices to points"},
+                   {24, 7, 2, 7, 5, "World64: Vec3<double> point coordinates in world space"},
+                   {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"},
+                   {24, 8, 2, 7, 5, "Grid64: Vec3<double> point coordinates in grid space"},
+                   {12, 8, 2, 6, 5, "Grid32: Vec3<float> point coordinates in grid space"},
+                   {12, 9, 2, 6, 5, "Voxel32: Vec3<float> point coordinates in voxel space"},
+                   {6, 9, 2, 25, 5, "Voxel16: Vec3<uint16_t> point coordinates in voxel space"},
+                   {3, 9, 2, 24, 5, "Voxel8: Vec3<uint8_t> point coordinates in voxel space"},
+                   {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"}};
+    constexpr bool kF64 = std::is_same<T, double>::value;
+    const auto& k = kinds[point_type == 9 && kF64 ? 2 : point_type];
+    PointArgs P{xyz, kF64 ? 1u : 0u, point_type, k.size, k.semantic, k.dataClass, k.dataType, k.gridClass, k.name,
+                1.0f / float(voxel_size), 1.0 / voxel_size};
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
+    Workspace& W = workspaceOf(device);
+    std::lock_guard<std::mutex> hold(W.lock);
+    keepPoolMemory();
+    int32_t* ijk = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+    k_p2g_voxels<T><<<blocksFor(n, kB), kB, 0, s>>>(xyz, uint32_t(n), P.invF, P.invD, ijk);
+    const int rc = buildFromVoxels(W, ijk, n, 23u, voxel_size, name, nullptr, &P, s, out);
+    (void)hipFreeAsync(ijk, s);
+    (void)hipStreamSynchronize(s);
+    return rc;
+}
 
 } // namespace
 
@@ -1001,7 +1084,7 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
     if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
     if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
     int device = 0;
-    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
     Workspace& W = workspaceOf(device);
     std::lock_guard<std::mutex> hold(W.lock);
     return buildFromVoxels(W, ijk, n, grid_type, voxel_size, name, index, nullptr, static_cast<hipStream_t>(stream), out);
@@ -1010,41 +1093,13 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
 int nvq_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
                        void* stream, nvq_grid** out)
 {
-    if (!out) return NVQ_ERR_INVALID_ARG;
-    *out = nullptr;
-    if (!xyz || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size) ||
-        point_type == 0 || point_type > 9)
-        return NVQ_ERR_INVALID_ARG;
-    // PointType -> blind data (processGridTreeRoot, tools/cuda/PointsToGrid.cuh:913-1001, float points)
-    static const struct {
-        uint32_t size, semantic, dataClass, dataType, gridClass;
-        const char* name;
-    } kinds[10] = {{0, 0, 0, 0, 0, ""},
-                   {4, 6, 1, 10, 4, "PointID: uint32_t indices to points"},
-                   {24, 7, 2, 7, 5, "World64: Vec3<double> point coordinates in world space"},
-                   {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"},
-                   {24, 8, 2, 7, 5, "Grid64: Vec3<double> point coordinates in grid space"},
-                   {12, 8, 2, 6, 5, "Grid32: Vec3<float> point coordinates in grid space"},
-                   {12, 9, 2, 6, 5, "Voxel32: Vec3<float> point coordinates in voxel space"},
-                   {6, 9, 2, 25, 5, "Voxel16: Vec3<uint16_t> point coordinates in voxel space"},
-                   {3, 9, 2, 24, 5, "Voxel8: Vec3<uint8_t> point coordinates in voxel space"},
-                   {12, 7, 2, 6, 5, "World32: Vec3<float> point coordinates in world space"}};
-    const auto& k = kinds[point_type];
-    PointArgs P{xyz, point_type, k.size, k.semantic, k.dataClass, k.dataType, k.gridClass, k.name,
-                1.0f / float(voxel_size), 1.0 / voxel_size};
-    const hipStream_t s = static_cast<hipStream_t>(stream);
-    int device = 0;
-    if (hipGetDevice(&device) != hipSuccess) return NVQ_ERR_NO_DEVICE;
-    Workspace& W = workspaceOf(device);
-    std::lock_guard<std::mutex> hold(W.lock);
-    keepPoolMemory();
-    int32_t* ijk = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
-    k_p2g_voxels<<<blocksFor(n, kB), kB, 0, s>>>(xyz, uint32_t(n), P.invF, ijk);
-    const int rc = buildFromVoxels(W, ijk, n, 23u, voxel_size, name, nullptr, &P, s, out);
-    (void)hipFreeAsync(ijk, s);
-    (void)hipStreamSynchronize(s);
-    return rc;
+    return pointsToGrid(xyz, n, voxel_size, point_type, name, stream, out);
+}
+
+int nvq_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                           void* stream, nvq_grid** out)
+{
+    return pointsToGrid(xyz, n, voxel_size, point_type, name, stream, out);
 }
 
 int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void* stream)

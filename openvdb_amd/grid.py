@@ -94,19 +94,21 @@ class DeviceGrid:
     @classmethod
     def from_points(cls, xyz, voxel_size=1.0, point_type="Default", name=None, stream=None):
         """Build a point grid on the device from float world-space points: tools::cuda::pointsToGrid
-        (tools/cuda/PointsToGrid.cuh:57-64).  ``xyz`` is an [n, 3] float32 CUDA tensor; the points
-        are stored as blind data encoded by ``point_type`` (a key of POINT_TYPES).  The queries of
-        this library do not read point grids; ``download`` returns the NanoVDB buffer."""
+        (tools/cuda/PointsToGrid.cuh:57-64).  ``xyz`` is an [n, 3] float32 (Vec3f) or float64 (Vec3d)
+        CUDA tensor; the points are stored as blind data encoded by ``point_type`` (a key of
+        POINT_TYPES).  The queries of this library do not read point grids; ``download`` returns
+        the NanoVDB buffer."""
         import torch
         if point_type not in cls.POINT_TYPES:
             raise ValueError(f"point_type must be one of {sorted(cls.POINT_TYPES)}")
-        if not (xyz.is_cuda and xyz.dtype == torch.float32 and xyz.dim() == 2 and xyz.shape[1] == 3):
-            raise ValueError("xyz must be an [n, 3] float32 CUDA tensor")
+        if not (xyz.is_cuda and xyz.dtype in (torch.float32, torch.float64) and xyz.dim() == 2 and xyz.shape[1] == 3):
+            raise ValueError("xyz must be an [n, 3] float32 or float64 CUDA tensor")
         xyz = xyz.contiguous()
         handle = ctypes.c_void_p()
-        check(lib().nvq_points_to_grid(ctypes.c_void_p(xyz.data_ptr()), xyz.shape[0], float(voxel_size),
-                                       cls.POINT_TYPES[point_type], None if name is None else name.encode(),
-                                       _stream_ptr(stream), ctypes.byref(handle)), "nvq_points_to_grid")
+        fn = "nvq_points_to_grid_f64" if xyz.dtype == torch.float64 else "nvq_points_to_grid"
+        check(getattr(lib(), fn)(ctypes.c_void_p(xyz.data_ptr()), xyz.shape[0], float(voxel_size),
+                                 cls.POINT_TYPES[point_type], None if name is None else name.encode(),
+                                 _stream_ptr(stream), ctypes.byref(handle)), fn)
         g = cls.__new__(cls)
         g.handle = handle
         g.device_bytes = None

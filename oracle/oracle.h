@@ -67,6 +67,9 @@ void or_free(void* p);
  * 1..9).  0 ok, -1 bad argument, -3 more than 65535 points in one leaf. */
 int or_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
                       void** out, uint64_t* bytes_out);
+/* The same for double world points (Vec3d input). */
+int or_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                          void** out, uint64_t* bytes_out);
 /* tools/cuda/GridStats.cuh updateGridStats<float>(StatsMode::Default) restated, in place on a host
  * float grid buffer (bboxes, node statistics, flags, world bbox).  0 ok, -4 not a float grid. */
 int or_update_grid_stats(void* buf);

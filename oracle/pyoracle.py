@@ -64,6 +64,7 @@ class Oracle:
         L.or_signed_flood_fill.argtypes = [_vp]
         L.or_points_to_grid.argtypes = [_vp, _u64, _d, ctypes.c_uint32, ctypes.c_char_p, ctypes.POINTER(_vp),
                                         ctypes.POINTER(_u64)]
+        L.or_points_to_grid_f64.argtypes = L.or_points_to_grid.argtypes
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -188,11 +189,13 @@ class Oracle:
         return out
 
     def points_to_grid(self, xyz, voxel_size=1.0, point_type=9, name=None):
-        """pointsToGrid restated for float world points: the Point grid buffer."""
-        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        """pointsToGrid restated for float or double (Vec3d) world points: the Point grid buffer."""
+        f64 = np.asarray(xyz).dtype == np.float64
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64 if f64 else np.float32).reshape(-1, 3)
         p, nb = _vp(), _u64()
-        st = self.L.or_points_to_grid(_p(xyz), len(xyz), voxel_size, point_type,
-                                      None if name is None else name.encode(), ctypes.byref(p), ctypes.byref(nb))
+        fn = self.L.or_points_to_grid_f64 if f64 else self.L.or_points_to_grid
+        st = fn(_p(xyz), len(xyz), voxel_size, point_type, None if name is None else name.encode(), ctypes.byref(p),
+                ctypes.byref(nb))
         if st != 0:
             raise ValueError("or_points_to_grid failed: %d" % st)
         buf = aligned_empty(nb.value)

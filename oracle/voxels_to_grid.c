@@ -124,15 +124,31 @@ static void inv_map_d(double inv, const float p[3], float q[3])
     }
 }
 
-/* mode 1 float, 20 ValueOnIndex, 23 Point (pts and point_type then describe the points) */
+/* Map(dx)::applyInverseMap of a double point: FP64 matMult (math/Math.h:1716-1722) */
+static void inv_map_dd(double inv, const double p[3], double q[3])
+{
+    for (int r = 0; r < 3; ++r) {
+        const double m[3] = {r == 0 ? inv : 0.0, r == 1 ? inv : 0.0, r == 2 ? inv : 0.0};
+        q[r] = fma(p[0], m[0], fma(p[1], m[1], p[2] * m[2]));
+    }
+}
+
+/* mode 1 float, 20 ValueOnIndex, 23 Point (pts or ptsd, float or double points, and point_type
+ * then describe the points) */
 static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
-                      const float* pts, uint32_t point_type, void** out, uint64_t* bytes_out, uint64_t* out_index)
+                      const float* pts, const double* ptsd, uint32_t point_type, void** out, uint64_t* bytes_out,
+                      uint64_t* out_index)
 {
     *out = NULL;
     *bytes_out = 0;
     const int pt = grid_type == 23u;
     PointKind kind = {0, 0, 0, 0, 0, ""};
-    if (pt && (!pts || point_kind(point_type, &kind) != 0)) return -1;
+    if (pt && ((!pts && !ptsd) || point_kind(point_type, &kind) != 0)) return -1;
+    if (pt && ptsd && point_type == 9) { /* Default stores the input type: Vec3<double> */
+        kind.size = 24;
+        kind.dataType = 7;
+        kind.name = "World64: Vec3<double> point coordinates in world space";
+    }
     VKey* k = (VKey*)malloc(n * sizeof(VKey));
     if (!k) return -2;
     for (uint64_t i = 0; i < n; ++i) {
@@ -398,8 +414,41 @@ static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
         const double invD = 1.0 / voxel_size;
         for (uint64_t q = 0; q < n; ++q) {
             const uint64_t id = k[q].id;
-            const float* w = pts + 3 * id;
             uint8_t* o = D + (uint64_t)kind.size * q;
+            if (ptsd) { /* double points: the same encodings computed from Vec3<double> */
+                const double* wd = ptsd + 3 * id;
+                double gd[3];
+                float wf[3], gf[3];
+                switch (point_type) {
+                case 1: put_u32(o, (uint32_t)id); break;
+                case 2:
+                case 9:
+                    for (int a = 0; a < 3; ++a) put_f64(o + 8 * a, wd[a]);
+                    break;
+                case 3:
+                    for (int a = 0; a < 3; ++a) put_f32(o + 4 * a, (float)wd[a]);
+                    break;
+                case 4:
+                    inv_map_dd(invD, wd, gd);
+                    for (int a = 0; a < 3; ++a) put_f64(o + 8 * a, gd[a]);
+                    break;
+                case 5: /* applyInverseMapF casts each coordinate to float first (math/Math.h:1704-1709) */
+                    for (int a = 0; a < 3; ++a) wf[a] = (float)wd[a];
+                    inv_map_f(invF, wf, gf);
+                    memcpy(o, gf, 12);
+                    break;
+                default:
+                    inv_map_dd(invD, wd, gd);
+                    for (int a = 0; a < 3; ++a) {
+                        const double f = gd[a] - (double)(int32_t)floor(gd[a] + 0.5);
+                        if (point_type == 6) put_f32(o + 4 * a, (float)f);
+                        else if (point_type == 7) put_u16(o + 2 * a, (uint16_t)(65535.0 * (f + 0.5)));
+                        else o[a] = (uint8_t)(255.0 * (f + 0.5));
+                    }
+                }
+                continue;
+            }
+            const float* w = pts + 3 * id;
             float g[3];
             switch (point_type) {
             case 1: put_u32(o, (uint32_t)id); break;
@@ -450,7 +499,7 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
     *bytes_out = 0;
     if (!ijk || n == 0 || (grid_type != GT_FLOAT_ && grid_type != GT_ONINDEX_)) return -1;
     if (out_index && grid_type != GT_ONINDEX_) return -1;
-    return build_grid(ijk, n, grid_type, voxel_size, name, NULL, 0, out, bytes_out, out_index);
+    return build_grid(ijk, n, grid_type, voxel_size, name, NULL, NULL, 0, out, bytes_out, out_index);
 }
 
 /* tools/cuda/PointsToGrid.cuh pointsToGrid(d_points, n, voxelSize, type) (:57-64) for float world
@@ -470,7 +519,29 @@ int or_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t 
         inv_map_f(invF, xyz + 3 * i, q);
         for (int a = 0; a < 3; ++a) ijk[3 * i + a] = (int32_t)floorf(q[a] + 0.5f);
     }
-    const int rc = build_grid(ijk, n, 23u, voxel_size, name, xyz, point_type, out, bytes_out, NULL);
+    const int rc = build_grid(ijk, n, 23u, voxel_size, name, xyz, NULL, point_type, out, bytes_out, NULL);
+    free(ijk);
+    return rc;
+}
+
+/* The same for double world points (PtrT = Vec3d*): each point's voxel is
+ * Round(Map(voxelSize).applyInverseMap(p)) in FP64 (TileKeyFunctor :571-572; Vec3<double>::round,
+ * math/Math.h:1515-1524), and the encodings start from the double coordinates. */
+int or_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
+                          void** out, uint64_t* bytes_out)
+{
+    *out = NULL;
+    *bytes_out = 0;
+    if (!xyz || n == 0) return -1;
+    int32_t* ijk = (int32_t*)malloc(sizeof(int32_t) * 3 * n);
+    if (!ijk) return -2;
+    const double invD = 1.0 / voxel_size;
+    for (uint64_t i = 0; i < n; ++i) {
+        double q[3];
+        inv_map_dd(invD, xyz + 3 * i, q);
+        for (int a = 0; a < 3; ++a) ijk[3 * i + a] = (int32_t)floor(q[a] + 0.5);
+    }
+    const int rc = build_grid(ijk, n, 23u, voxel_size, name, NULL, xyz, point_type, out, bytes_out, NULL);
     free(ijk);
     return rc;
 }

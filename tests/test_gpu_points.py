@@ -1,4 +1,5 @@
-"""Device-side point grids (nvq_points_to_grid = pointsToGrid for float world points) on the GPU:
+"""Device-side point grids (nvq_points_to_grid / _f64 = pointsToGrid for float and double world
+points) on the GPU:
 the buffer equals the oracle's byte for byte for every point encoding, the reference's validator
 accepts it, a leaf with more points than its uint16 values count is an error, and the query entry
 points refuse a point grid."""
@@ -35,6 +36,9 @@ CASES = {
     # 2^20 integer points in [-512, 512)^3 at voxel size 8: Large_CudaPointsToGrid_World64
     # (unittest/TestNanoVDB.cu:1594-1712), whose points are exactly representable in float
     "large": (rng(98765).integers(-512, 512, (1 << 20, 3)).astype(np.float32), 8.0),
+    # double (Vec3d) points: the same reference test's own input type, and a Gaussian cloud
+    "large64": (rng(98765).integers(-512, 512, (1 << 20, 3)).astype(np.float64), 8.0),
+    "cloud64": (_points(17, 60000, 30.0).astype(np.float64) + rng(18).uniform(-1e-3, 1e-3, (61200, 3)), 0.75),
 }
 
 
@@ -51,7 +55,7 @@ def test_points_match_oracle(oracle, case, point_type):
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
     assert g.grid_type == "PointIndex"
-    if case == "large":  # maxPointsPerLeaf < 1024, as the reference test expects
+    if case.startswith("large"):  # maxPointsPerLeaf < 1024, as the reference test expects
         leaf0 = 672 + int(np.frombuffer(got[672:680].tobytes(), np.int64)[0])
         counts = np.frombuffer(got[leaf0:leaf0 + 1120 * g.leaf_count].tobytes(), np.uint64).reshape(-1, 140)[:, 11]
         assert counts.sum() == len(pts) and 0 < counts.max() < 1024

@@ -85,6 +85,56 @@ def test_point_accessor_reads_back_every_point(oracle, ref, name):
         np.testing.assert_allclose((vox + off) * dx, want.astype(np.float64), atol=tol * dx + 1e-5)
 
 
+@pytest.mark.parametrize("name", sorted(TYPES))
+def test_double_points(oracle, ref, name):
+    """Vec3d input (PtrT = Vec3d*): voxels rounded in FP64, the reference's PointAccessor reads every
+    point back, and each encoding decodes to the double input within its resolution."""
+    r = np.random.default_rng(11)
+    pts = np.concatenate([r.normal(size=(40000, 3)) * 30.0, r.integers(-512, 512, (4000, 3)).astype(np.float64)])
+    dx = 0.75
+    g = oracle.points_to_grid(pts, dx, TYPES[name])
+    assert ref.validate(g, full=True)
+    ijk = np.floor(pts * (1.0 / dx) + 0.5).astype(np.int32)
+    uniq, inv = np.unique(ijk, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    count, first = ref.point_voxels(g, uniq)
+    np.testing.assert_array_equal(count.astype(np.int64), np.bincount(inv))
+    data, size = blind(g)
+    c = count.astype(np.int64)
+    order = np.argsort(inv, kind="stable")
+    pos = np.repeat(first.astype(np.int64), c) + (np.arange(len(pts)) - np.repeat(np.cumsum(c) - c, c))
+    rec = np.frombuffer(data.tobytes(), np.uint8).reshape(len(pts), size)[pos]
+    want = pts[order]
+    vox = uniq[inv[order]].astype(np.float64)
+    if name == "PointID":
+        np.testing.assert_array_equal(rec.copy().view(np.uint32).reshape(-1), order)
+    elif name in ("World64", "Default"):
+        assert size == 24
+        np.testing.assert_array_equal(rec.copy().view(np.float64), want)
+    elif name == "World32":
+        np.testing.assert_array_equal(rec.copy().view(np.float32), want.astype(np.float32))
+    elif name == "Grid64":
+        np.testing.assert_array_equal(rec.copy().view(np.float64), want * (1.0 / dx))
+    elif name == "Grid32":
+        grid = rec.copy().view(np.float32).astype(np.float64)
+        np.testing.assert_allclose(grid, want / dx, rtol=2e-7, atol=1e-6)
+    else:
+        off = {"Voxel32": lambda r: r.copy().view(np.float32).astype(np.float64),
+               "Voxel16": lambda r: r.copy().view(np.uint16).astype(np.float64) / 65535.0 - 0.5,
+               "Voxel8": lambda r: r.astype(np.float64) / 255.0 - 0.5}[name](rec)
+        tol = {"Voxel32": 1e-6, "Voxel16": 2.0 / 65535.0, "Voxel8": 2.0 / 255.0}[name]
+        np.testing.assert_allclose((vox + off) * dx, want, atol=tol * dx + 1e-9)
+
+
+def test_double_and_float_integer_points_share_the_tree(oracle):
+    """Integer points are exact in both precisions: the float and double builds differ only in
+    the blind data, and PointID blind data (indices) not at all."""
+    pts = np.random.default_rng(98765).integers(-512, 512, (1 << 14, 3))
+    a = oracle.points_to_grid(pts.astype(np.float32), 8.0, TYPES["PointID"])
+    b = oracle.points_to_grid(pts.astype(np.float64), 8.0, TYPES["PointID"])
+    np.testing.assert_array_equal(a, b)
+
+
 def test_too_many_points_in_a_leaf(oracle):
     """More than 65535 points in one leaf: the reference throws; the oracle reports -3."""
     pts = np.zeros((65536, 3), np.float32)
