@@ -213,6 +213,17 @@ int nvq_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t
  * are rounded in FP64, and point_type 9 (Default) stores the points as Vec3<double>. */
 int nvq_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
                            void* stream, nvq_grid** out);
+/* Point grid with the voxel size chosen for a target point density: PointsToGrid(maxPointsPerVoxel,
+ * tolerance, maxIterations) (tools/cuda/PointsToGrid.cuh:335-342, the search in countNodes
+ * :630-774).  Starting from voxel size 1, the largest number of points in one voxel is counted and
+ * the voxel size updated until that count is within `tolerance` of max_points_per_voxel (clamped
+ * to n), for at most max_iterations counts; the grid is then built as nvq_points_to_grid(_f64) with
+ * the last voxel size.  xyz holds float (xyz_f64 = 0) or double world points.  voxel_size and
+ * points_per_voxel (either may be NULL) receive the chosen voxel size and its largest count
+ * (PointsToGrid::maxPointsPerVoxel). */
+int nvq_points_to_grid_density(const void* xyz, int xyz_f64, uint64_t n, uint32_t max_points_per_voxel, int tolerance,
+                               int max_iterations, uint32_t point_type, const char* name, void* stream, nvq_grid** out,
+                               double* voxel_size, uint32_t* points_per_voxel);
 /* Copy the first `bytes` bytes (at most nvq_grid_info.bytes; pass that for the whole grid) of a
  * grid's device buffer to host memory and wait for it: GridHandle::deviceDownload
  * (GridHandle.h:186-194). */

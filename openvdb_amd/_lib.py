@@ -66,6 +66,8 @@ SIGNATURES = {
     "nvq_voxels_to_grid": ([_vp, _u64, _u32, _d, ctypes.c_char_p, _vp, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_points_to_grid": ([_vp, _u64, _d, _u32, ctypes.c_char_p, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_points_to_grid_f64": ([_vp, _u64, _d, _u32, ctypes.c_char_p, _vp, ctypes.POINTER(_vp)], _i),
+    "nvq_points_to_grid_density": ([_vp, _i, _u64, _u32, _i, _i, _u32, ctypes.c_char_p, _vp, ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_d), ctypes.POINTER(_u32)], _i),
     "nvq_grid_download": ([_vp, _vp, _u64, _vp], _i),
     "nvq_update_grid_stats": ([_vp, _vp], _i),
     "nvq_signed_flood_fill": ([_vp, _vp], _i),

@@ -981,6 +981,91 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
 }
 
 // From the input voxels on: bbox, key width, then buildSorted.  The caller holds W's lock.
+// Dense numbering of the upper-node tiles (and, for one tile, the lower nodes) over an input bbox:
+// the sort keys of the builder.
+void keyLayout(const int32_t box[6], Layout& L, uint64_t& tiles, int& tileBits, int& lowerBits)
+{
+    tiles = 1;
+    for (int a = 0; a < 3; ++a) {
+        L.tlo[a] = box[a] >> 12;
+        L.dim[a] = uint32_t((box[3 + a] >> 12) - L.tlo[a] + 1);
+        tiles *= L.dim[a];
+    }
+    tileBits = tiles > 1 ? 64 - __builtin_clzll(tiles - 1) : 0;
+    uint64_t lowers = 1;
+    for (int a = 0; a < 3; ++a) {
+        L.llo[a] = box[a] >> 7;
+        L.ldim[a] = uint32_t((box[3 + a] >> 7) - L.llo[a] + 1);
+        lowers *= L.ldim[a];
+    }
+    lowerBits = lowers > 1 ? 64 - __builtin_clzll(lowers - 1) : 0;
+}
+
+// Largest number of points in one voxel (the density of the voxel-size search): sort the voxel
+// keys, run-length encode them and take the longest run (DeviceRunLengthEncode + DeviceReduce::Max
+// over the sorted keys, as countNodes does, tools/cuda/PointsToGrid.cuh:722-740).
+template<typename K>
+int maxPerVoxelSorted(const int32_t* ijk, uint32_t n, const Layout& L, int endBit, hipStream_t s, uint32_t& most)
+{
+    size_t sortBytes = 0, rleBytes = 0, maxBytes = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, static_cast<K*>(nullptr), static_cast<K*>(nullptr), int(n), 0,
+                                          endBit, s) != hipSuccess ||
+        hipcub::DeviceRunLengthEncode::Encode(nullptr, rleBytes, static_cast<K*>(nullptr), static_cast<K*>(nullptr),
+                                              static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), int(n),
+                                              s) != hipSuccess ||
+        hipcub::DeviceReduce::Max(nullptr, maxBytes, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                  int(n), s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    const size_t bytes = 3 * carveBytes<K>(n) + 2 * carveBytes<uint32_t>(n) + carveBytes<uint32_t>(2) +
+                         carveBytes<uint8_t>(std::max({sortBytes, rleBytes, maxBytes}));
+    void* mem = nullptr;
+    if (hipMallocAsync(&mem, bytes, s) != hipSuccess) return NVQ_ERR_HIP;
+    Carve c{static_cast<uint8_t*>(mem)};
+    K* keys = c.take<K>(n);
+    K* sorted = c.take<K>(n);
+    K* unique = c.take<K>(n);
+    uint32_t* ids = c.take<uint32_t>(n);
+    uint32_t* counts = c.take<uint32_t>(n);
+    uint32_t* small = c.take<uint32_t>(2); // [0] runs, [1] longest run
+    void* temp = c.take<uint8_t>(std::max({sortBytes, rleBytes, maxBytes}));
+    int rc = NVQ_ERR_HIP;
+    uint32_t runs = 0;
+    k_v2g_keys<K><<<blocksFor(n, kB), kB, 0, s>>>(ijk, n, L, keys, ids);
+    if (hipGetLastError() == hipSuccess &&
+        hipcub::DeviceRadixSort::SortKeys(temp, sortBytes, keys, sorted, int(n), 0, endBit, s) == hipSuccess &&
+        hipcub::DeviceRunLengthEncode::Encode(temp, rleBytes, sorted, unique, counts, small, int(n), s) == hipSuccess &&
+        hipMemcpyAsync(&runs, small, sizeof(runs), hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess &&
+        hipcub::DeviceReduce::Max(temp, maxBytes, counts, small + 1, int(runs), s) == hipSuccess &&
+        hipMemcpyAsync(&most, small + 1, sizeof(most), hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess)
+        rc = NVQ_OK;
+    (void)hipFreeAsync(mem, s);
+    return rc;
+}
+
+int maxPerVoxel(Workspace& W, const int32_t* ijk, uint32_t n, hipStream_t s, uint32_t& most)
+{
+    uint8_t* ws = W.reserve(carveBytes<int32_t>(6 * kBoundBlocks) + carveBytes<int32_t>(6));
+    if (!ws) return NVQ_ERR_HIP;
+    Carve c{ws};
+    int32_t* part = c.take<int32_t>(6 * kBoundBlocks);
+    int32_t* bnd = c.take<int32_t>(6);
+    const uint32_t nb = std::min<uint32_t>(blocksFor(n, kB), kBoundBlocks);
+    k_v2g_bounds<<<nb, kB, 0, s>>>(ijk, n, part);
+    k_v2g_bounds_final<<<1, kB, 0, s>>>(part, nb, bnd);
+    int32_t box[6];
+    if (hipMemcpyAsync(box, bnd, sizeof(box), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return NVQ_ERR_HIP;
+    Layout L{};
+    uint64_t tiles = 1;
+    int tileBits = 0, lowerBits = 0;
+    keyLayout(box, L, tiles, tileBits, lowerBits);
+    if (tiles == 1 && 21 + lowerBits <= 32) return maxPerVoxelSorted<uint32_t>(ijk, n, L, 21 + lowerBits, s, most);
+    if (36 + tileBits > 64) return NVQ_ERR_INVALID_ARG; // input spans more than 2^28 upper-node tiles
+    return maxPerVoxelSorted<uint64_t>(ijk, n, L, 36 + tileBits, s, most);
+}
+
 int buildFromVoxels(Workspace& W, const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size,
                     const char* name, uint64_t* index, const PointArgs* P, hipStream_t s, nvq_grid** out)
 {
@@ -1000,19 +1085,8 @@ int buildFromVoxels(Workspace& W, const int32_t* ijk, uint64_t n, uint32_t grid_
         return NVQ_ERR_HIP;
     Layout L{};
     uint64_t tiles = 1;
-    for (int a = 0; a < 3; ++a) {
-        L.tlo[a] = box[a] >> 12;
-        L.dim[a] = uint32_t((box[3 + a] >> 12) - L.tlo[a] + 1);
-        tiles *= L.dim[a];
-    }
-    const int tileBits = tiles > 1 ? 64 - __builtin_clzll(tiles - 1) : 0;
-    uint64_t lowers = 1;
-    for (int a = 0; a < 3; ++a) {
-        L.llo[a] = box[a] >> 7;
-        L.ldim[a] = uint32_t((box[3 + a] >> 7) - L.llo[a] + 1);
-        lowers *= L.ldim[a];
-    }
-    const int lowerBits = lowers > 1 ? 64 - __builtin_clzll(lowers - 1) : 0;
+    int tileBits = 0, lowerBits = 0;
+    keyLayout(box, L, tiles, tileBits, lowerBits);
     L.isIndex = grid_type == GT_ONINDEX;
     L.isPoint = P != nullptr;
     int rc;
@@ -1070,6 +1144,68 @@ int pointsToGrid(const T* xyz, uint64_t n, double voxel_size, uint32_t point_typ
     return rc;
 }
 
+// The voxel-size search of PointsToGrid(maxPointsPerVoxel, tolerance, maxIterations)
+// (tools/cuda/PointsToGrid.cuh:335-342, countNodes :630-774): start from Map(1.0); after each
+// count of the largest points-per-voxel, stop when it is within the tolerance of the target (or
+// after maxIterations, or for a single point), else bracket the target between the densest
+// voxel size below it and the sparsest above it and interpolate (or scale, while no size above
+// it is known).  The grid is then built with the last voxel size.
+template<typename T>
+int pointsToGridDensity(const T* xyz, uint64_t n, uint32_t target, int tolerance, int max_iterations, uint32_t point_type,
+                        const char* name, void* stream, nvq_grid** out, double* voxel_size_out, uint32_t* density_out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!xyz || n == 0 || n > uint64_t(INT32_MAX) || target == 0 || point_type == 0 || point_type > 9)
+        return NVQ_ERR_INVALID_ARG;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
+    target = uint32_t(std::min<uint64_t>(target, n));
+    struct Sample { // voxel size and the largest points-per-voxel it gives (Foo, :634-638)
+        double dx;
+        uint32_t density;
+        bool operator<(const Sample& r) const { return density < r.density || (density == r.density && dx < r.dx); }
+    } lo{0.0, 1u}, hi{0.0, 0u};
+    double dx = 1.0;
+    uint32_t density = 0;
+    {
+        Workspace& W = workspaceOf(device);
+        std::lock_guard<std::mutex> hold(W.lock);
+        keepPoolMemory();
+        int32_t* ijk = nullptr;
+        if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+        int rc = NVQ_OK;
+        for (int iter = 0;;) {
+            if (!(dx > 0.0) || !std::isfinite(dx)) {
+                rc = NVQ_ERR_INVALID_ARG;
+                break;
+            }
+            k_p2g_voxels<T><<<blocksFor(n, kB), kB, 0, s>>>(xyz, uint32_t(n), 1.0f / float(dx), 1.0 / dx, ijk);
+            if ((rc = maxPerVoxel(W, ijk, uint32_t(n), s, density)) != NVQ_OK) break;
+            if (++iter >= max_iterations || n == 1 || std::abs(int(density) - int(target)) <= tolerance) break;
+            const Sample cur{dx, density};
+            if (density < target) {
+                if (lo < cur) lo = cur;
+            } else if (hi.density == 0 || cur < hi) {
+                hi = cur;
+            }
+            if (hi.density)
+                dx = (lo.dx * double(hi.density - target) + hi.dx * double(target - lo.density)) / double(hi.density - lo.density);
+            else if (density > 1u)
+                dx *= (target - 1.0) / (density - 1.0);
+            else
+                dx *= 10.0;
+        }
+        (void)hipFreeAsync(ijk, s);
+        (void)hipStreamSynchronize(s);
+        if (rc != NVQ_OK) return rc;
+    }
+    if (voxel_size_out) *voxel_size_out = dx;
+    if (density_out) *density_out = density;
+    return pointsToGrid(xyz, n, dx, point_type, name, stream, out);
+}
+
 } // namespace
 
 extern "C" {
@@ -1100,6 +1236,17 @@ int nvq_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uin
                            void* stream, nvq_grid** out)
 {
     return pointsToGrid(xyz, n, voxel_size, point_type, name, stream, out);
+}
+
+int nvq_points_to_grid_density(const void* xyz, int xyz_f64, uint64_t n, uint32_t max_points_per_voxel, int tolerance,
+                               int max_iterations, uint32_t point_type, const char* name, void* stream, nvq_grid** out,
+                               double* voxel_size, uint32_t* points_per_voxel)
+{
+    if (xyz_f64)
+        return pointsToGridDensity(static_cast<const double*>(xyz), n, max_points_per_voxel, tolerance, max_iterations,
+                                   point_type, name, stream, out, voxel_size, points_per_voxel);
+    return pointsToGridDensity(static_cast<const float*>(xyz), n, max_points_per_voxel, tolerance, max_iterations,
+                               point_type, name, stream, out, voxel_size, points_per_voxel);
 }
 
 int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void* stream)

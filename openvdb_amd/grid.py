@@ -118,6 +118,37 @@ class DeviceGrid:
         g._host = g.download(stream, limit=1024)
         return g
 
+    @classmethod
+    def from_points_density(cls, xyz, max_points_per_voxel, tolerance=1, max_iterations=10, point_type="Default",
+                            name=None, stream=None):
+        """Point grid whose voxel size gives about ``max_points_per_voxel`` points in the densest
+        voxel: PointsToGrid(maxPointsPerVoxel, tolerance, maxIterations)
+        (tools/cuda/PointsToGrid.cuh:335-342).  Returns the grid; its ``points_per_voxel`` is the
+        largest count reached (PointsToGrid::maxPointsPerVoxel) and ``voxel_size`` the chosen size."""
+        import torch
+        if point_type not in cls.POINT_TYPES:
+            raise ValueError(f"point_type must be one of {sorted(cls.POINT_TYPES)}")
+        if not (xyz.is_cuda and xyz.dtype in (torch.float32, torch.float64) and xyz.dim() == 2 and xyz.shape[1] == 3):
+            raise ValueError("xyz must be an [n, 3] float32 or float64 CUDA tensor")
+        xyz = xyz.contiguous()
+        handle = ctypes.c_void_p()
+        dx, dens = ctypes.c_double(), ctypes.c_uint32()
+        check(lib().nvq_points_to_grid_density(ctypes.c_void_p(xyz.data_ptr()), int(xyz.dtype == torch.float64),
+                                               xyz.shape[0], int(max_points_per_voxel), int(tolerance),
+                                               int(max_iterations), cls.POINT_TYPES[point_type],
+                                               None if name is None else name.encode(), _stream_ptr(stream),
+                                               ctypes.byref(handle), ctypes.byref(dx), ctypes.byref(dens)),
+              "nvq_points_to_grid_density")
+        g = cls.__new__(cls)
+        g.handle = handle
+        g.device_bytes = None
+        info = GridInfo()
+        check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
+        g.info = info
+        g._host = g.download(stream, limit=1024)
+        g.points_per_voxel = int(dens.value)
+        return g
+
     def download(self, stream=None, limit=None):
         """The grid buffer (or its first ``limit`` bytes) as a 32-byte aligned uint8 numpy array
         (GridHandle::deviceDownload)."""

@@ -70,6 +70,11 @@ int or_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t 
 /* The same for double world points (Vec3d input). */
 int or_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,
                           void** out, uint64_t* bytes_out);
+/* PointsToGrid(maxPointsPerVoxel, tolerance, maxIterations): the voxel-size search, then the grid
+ * at the chosen voxel size (dx_out) and its largest points-per-voxel count (density_out). */
+int or_points_to_grid_density(const void* xyz, int f64, uint64_t n, uint32_t target, int tolerance, int max_iterations,
+                              uint32_t point_type, const char* name, void** out, uint64_t* bytes_out,
+                              double* dx_out, uint32_t* density_out);
 /* tools/cuda/GridStats.cuh updateGridStats<float>(StatsMode::Default) restated, in place on a host
  * float grid buffer (bboxes, node statistics, flags, world bbox).  0 ok, -4 not a float grid. */
 int or_update_grid_stats(void* buf);

@@ -65,6 +65,9 @@ class Oracle:
         L.or_points_to_grid.argtypes = [_vp, _u64, _d, ctypes.c_uint32, ctypes.c_char_p, ctypes.POINTER(_vp),
                                         ctypes.POINTER(_u64)]
         L.or_points_to_grid_f64.argtypes = L.or_points_to_grid.argtypes
+        L.or_points_to_grid_density.argtypes = [_vp, ctypes.c_int, _u64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_uint32, ctypes.c_char_p, ctypes.POINTER(_vp),
+                                                ctypes.POINTER(_u64), ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_uint32)]
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -202,6 +205,23 @@ class Oracle:
         ctypes.memmove(buf.ctypes.data, p.value, nb.value)
         self.L.or_free(p)
         return buf
+
+    def points_to_grid_density(self, xyz, max_points_per_voxel, tolerance=1, max_iterations=10, point_type=9,
+                               name=None):
+        """PointsToGrid(maxPointsPerVoxel, tolerance, maxIterations) restated: (buffer, voxel size,
+        largest points per voxel)."""
+        f64 = np.asarray(xyz).dtype == np.float64
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64 if f64 else np.float32).reshape(-1, 3)
+        p, nb, dx, dens = _vp(), _u64(), _d(), ctypes.c_uint32()
+        st = self.L.or_points_to_grid_density(_p(xyz), int(f64), len(xyz), max_points_per_voxel, tolerance,
+                                              max_iterations, point_type, None if name is None else name.encode(),
+                                              ctypes.byref(p), ctypes.byref(nb), ctypes.byref(dx), ctypes.byref(dens))
+        if st != 0:
+            raise ValueError("or_points_to_grid_density failed: %d" % st)
+        buf = aligned_empty(nb.value)
+        ctypes.memmove(buf.ctypes.data, p.value, nb.value)
+        self.L.or_free(p)
+        return buf, dx.value, dens.value
 
     def signed_flood_fill(self, buf):
         """cuda::signedFloodFill<float> restated, on a copy of the float grid buffer."""

@@ -545,3 +545,94 @@ int or_points_to_grid_f64(const double* xyz, uint64_t n, double voxel_size, uint
     free(ijk);
     return rc;
 }
+
+/* ---- voxel size for a target point density ---------------------------------------------------
+ * PointsToGrid(maxPointsPerVoxel, tolerance, maxIterations) (tools/cuda/PointsToGrid.cuh:335-342)
+ * restated: countNodes (:630-774) starts from Map(1.0), counts the largest number of points in one
+ * voxel at the current voxel size dx, and stops when that count is within the tolerance of the
+ * target (clamped to the point count), after maxIterations counts, or for a single point.
+ * Otherwise the sample (dx, count) becomes the bracket's lower end (count below the target: the
+ * largest such sample by count, then dx) or upper end (the smallest sample at or above it), and
+ * dx is interpolated linearly between the two ends; while no upper end is known dx is scaled by
+ * (target - 1) / (count - 1), or by 10 at a count of 1.  The grid is then built with the last dx. */
+static int cmp_ijk(const void* a, const void* b)
+{
+    const int32_t* p = (const int32_t*)a;
+    const int32_t* q = (const int32_t*)b;
+    for (int i = 0; i < 3; ++i)
+        if (p[i] != q[i]) return p[i] < q[i] ? -1 : 1;
+    return 0;
+}
+
+/* largest number of points that round to one voxel at voxel size dx */
+static uint32_t max_per_voxel(const float* xf, const double* xd, uint64_t n, double dx, int32_t* ijk)
+{
+    const float invF = 1.0f / (float)dx;
+    const double invD = 1.0 / dx;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (xd) {
+            double q[3];
+            inv_map_dd(invD, xd + 3 * i, q);
+            for (int a = 0; a < 3; ++a) ijk[3 * i + a] = (int32_t)floor(q[a] + 0.5);
+        } else {
+            float q[3];
+            inv_map_f(invF, xf + 3 * i, q);
+            for (int a = 0; a < 3; ++a) ijk[3 * i + a] = (int32_t)floorf(q[a] + 0.5f);
+        }
+    }
+    qsort(ijk, n, 12, cmp_ijk);
+    uint32_t most = 0, run = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        run = (i > 0 && cmp_ijk(ijk + 3 * i, ijk + 3 * (i - 1)) == 0) ? run + 1 : 1;
+        if (run > most) most = run;
+    }
+    return most;
+}
+
+typedef struct {
+    double dx;
+    uint32_t density;
+} DensitySample;
+static int sample_less(DensitySample a, DensitySample b)
+{
+    return a.density < b.density || (a.density == b.density && a.dx < b.dx);
+}
+
+int or_points_to_grid_density(const void* xyz, int f64, uint64_t n, uint32_t target, int tolerance, int max_iterations,
+                              uint32_t point_type, const char* name, void** out, uint64_t* bytes_out,
+                              double* dx_out, uint32_t* density_out)
+{
+    *out = NULL;
+    *bytes_out = 0;
+    if (!xyz || n == 0 || target == 0) return -1;
+    const float* xf = f64 ? NULL : (const float*)xyz;
+    const double* xd = f64 ? (const double*)xyz : NULL;
+    if ((uint64_t)target > n) target = (uint32_t)n;
+    int32_t* ijk = (int32_t*)malloc(sizeof(int32_t) * 3 * n);
+    if (!ijk) return -2;
+    DensitySample lo = {0.0, 1u}, hi = {0.0, 0u};
+    double dx = 1.0;
+    uint32_t density = 0;
+    for (int iter = 0;;) {
+        density = max_per_voxel(xf, xd, n, dx, ijk);
+        if (++iter >= max_iterations || n == 1 || abs((int)density - (int)target) <= tolerance) break;
+        const DensitySample cur = {dx, density};
+        if (density < target) {
+            if (sample_less(lo, cur)) lo = cur;
+        } else if (hi.density == 0 || sample_less(cur, hi)) {
+            hi = cur;
+        }
+        if (hi.density)
+            dx = (lo.dx * (double)(hi.density - target) + hi.dx * (double)(target - lo.density)) /
+                 (double)(hi.density - lo.density);
+        else if (density > 1u)
+            dx *= (target - 1.0) / (density - 1.0);
+        else
+            dx *= 10.0;
+    }
+    free(ijk);
+    *dx_out = dx;
+    *density_out = density;
+    return f64 ? or_points_to_grid_f64(xd, n, dx, point_type, name, out, bytes_out)
+               : or_points_to_grid(xf, n, dx, point_type, name, out, bytes_out);
+}

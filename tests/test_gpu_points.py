@@ -89,3 +89,29 @@ def test_queries_refuse_point_grids():
     with pytest.raises(RuntimeError):
         ops.sample(g, ijk.float())
     g.close()
+
+
+DENSITY = {
+    # Large_CudaPointsToGrid_World64_density (unittest/TestNanoVDB.cu:1714-1831)
+    "reference": (rng(98765).integers(-512, 512, (1 << 20, 3)).astype(np.float64), 60, 1, 10, "World64"),
+    "gauss32": (_points(21, 200000, 50.0), 20, 1, 10, "Default"),
+    "few_iterations": (_points(22, 50000, 20.0), 40, 0, 3, "Voxel16"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(DENSITY))
+def test_density_matches_oracle(oracle, case):
+    """The voxel-size search on the device takes the oracle's voxel sizes (bit for bit) and builds
+    the same grid."""
+    nvq = _nvq()
+    pts, target, tol, iters, ptype = DENSITY[case]
+    print(f"[points_to_grid_density] case={case} n={len(pts)} target={target}", flush=True)
+    g = nvq.DeviceGrid.from_points_density(torch.from_numpy(pts).cuda(), target, tol, iters, ptype)
+    want, dx, most = oracle.points_to_grid_density(pts, target, tol, iters, TYPES[ptype])
+    assert g.points_per_voxel == most
+    assert g.voxel_size[0] == dx
+    got = g.download()
+    assert got.nbytes == want.nbytes
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
+    g.close()

@@ -135,6 +135,38 @@ def test_double_and_float_integer_points_share_the_tree(oracle):
     np.testing.assert_array_equal(a, b)
 
 
+def test_density_search_reference_case(oracle, ref):
+    """Large_CudaPointsToGrid_World64_density (unittest/TestNanoVDB.cu:1714-1831): 2^20 Vec3d points
+    with integer coordinates in [-512, 512)^3, 60 points per voxel wanted, tolerance 1.  The
+    reference expects the densest voxel within the tolerance of 60 and at most 60 * 512 points per
+    leaf.  The reported count is recomputed here from the chosen voxel size with numpy, and the
+    grid equals the fixed-voxel-size build at that size."""
+    pts = np.random.default_rng(98765).integers(-512, 512, (1 << 20, 3)).astype(np.float64)
+    g, dx, most = oracle.points_to_grid_density(pts, 60, 1, 10, TYPES["World64"])
+    assert abs(most - 60) <= 1
+    assert ref.validate(g, full=True)
+    ijk = np.floor(pts * (1.0 / dx) + 0.5).astype(np.int64)
+    _, counts = np.unique(ijk, axis=0, return_counts=True)
+    assert counts.max() == most
+    _, leaf_counts = np.unique(ijk >> 3, axis=0, return_counts=True)
+    assert leaf_counts.max() <= 60 * 512
+    np.testing.assert_array_equal(g, oracle.points_to_grid(pts, dx, TYPES["World64"]))
+
+
+def test_density_search_stops(oracle):
+    """A single point stops at once (voxel size 1); max_iterations bounds the counts; a target
+    above the point count is clamped to it."""
+    pts = (np.random.default_rng(5).normal(size=(20000, 3)) * 50).astype(np.float32)
+    _, dx, most = oracle.points_to_grid_density(pts[:1], 20)
+    assert (dx, most) == (1.0, 1)
+    _, dx1, most1 = oracle.points_to_grid_density(pts, 20, 0, 1)
+    assert dx1 == 1.0  # one count, no update
+    ijk = np.floor(pts * np.float32(1.0) + np.float32(0.5)).astype(np.int64)
+    assert most1 == np.unique(ijk, axis=0, return_counts=True)[1].max()
+    _, dx2, most2 = oracle.points_to_grid_density(pts[:50], 1000, 0, 10)
+    assert most2 <= 50
+
+
 def test_too_many_points_in_a_leaf(oracle):
     """More than 65535 points in one leaf: the reference throws; the oracle reports -3."""
     pts = np.zeros((65536, 3), np.float32)
