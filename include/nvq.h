@@ -198,6 +198,11 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
  * between calls and grows to the largest build; builds on one device run one at a time. */
 int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double voxel_size, const char* name,
                        uint64_t* index, void* stream, nvq_grid** out);
+/* The same from index-space voxel coordinates given as floating point (PtrT = Vec3f* or Vec3d*,
+ * xyz_f64 selects double): each is rounded to the nearest voxel, Vec3<T>::round (math/Math.h
+ * :1515-1524, TileKeyFunctor tools/cuda/PointsToGrid.cuh:576-577). */
+int nvq_voxels_to_grid_fp(const void* xyz, int xyz_f64, uint64_t n, uint32_t grid_type, double voxel_size,
+                          const char* name, uint64_t* index, void* stream, nvq_grid** out);
 /* Device-side construction of a point grid: tools::cuda::pointsToGrid(d_points, n, voxelSize, type)
  * (tools/cuda/PointsToGrid.cuh:57-64) for n float world-space points xyz[n][3] (a DEVICE array).
  * Each point lies in voxel Round(Map(voxel_size).applyInverseMapF(p)).  The grid (GridType

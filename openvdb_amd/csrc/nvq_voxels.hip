@@ -338,6 +338,20 @@ __global__ __launch_bounds__(kB) void k_p2g_voxels(const T* __restrict__ xyz, ui
     }
 }
 
+// Index-space voxel coordinates given as Vec3f / Vec3d: Vec3<T>::round (math/Math.h:1515-1524),
+// as TileKeyFunctor does for non-Point build types (tools/cuda/PointsToGrid.cuh:576-577).
+template<typename T>
+__global__ __launch_bounds__(kB) void k_round_voxels(const T* __restrict__ xyz, uint32_t n, int32_t* __restrict__ ijk)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T x = xyz[3ull * i + a];
+        ijk[3ull * i + a] = std::is_same<T, double>::value ? int32_t(floor(double(x) + 0.5)) : int32_t(floorf(float(x) + 0.5f));
+    }
+}
+
 // One wavefront per Point leaf: mOffset (points before the leaf), mPointCount, and per voxel the
 // number of the leaf's points up to and including it (SetLeafActiveVoxelStateAndValuesFunctor
 // :1156-1174); an inactive voxel repeats the previous active voxel's value, else 0
@@ -1224,6 +1238,33 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
     Workspace& W = workspaceOf(device);
     std::lock_guard<std::mutex> hold(W.lock);
     return buildFromVoxels(W, ijk, n, grid_type, voxel_size, name, index, nullptr, static_cast<hipStream_t>(stream), out);
+}
+
+int nvq_voxels_to_grid_fp(const void* xyz, int xyz_f64, uint64_t n, uint32_t grid_type, double voxel_size,
+                          const char* name, uint64_t* index, void* stream, nvq_grid** out)
+{
+    if (!out) return NVQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!xyz || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size))
+        return NVQ_ERR_INVALID_ARG;
+    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    Workspace& W = workspaceOf(device);
+    std::lock_guard<std::mutex> hold(W.lock);
+    keepPoolMemory();
+    int32_t* ijk = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+    if (xyz_f64)
+        k_round_voxels<double><<<blocksFor(n, kB), kB, 0, s>>>(static_cast<const double*>(xyz), uint32_t(n), ijk);
+    else
+        k_round_voxels<float><<<blocksFor(n, kB), kB, 0, s>>>(static_cast<const float*>(xyz), uint32_t(n), ijk);
+    const int rc = buildFromVoxels(W, ijk, n, grid_type, voxel_size, name, index, nullptr, s, out);
+    (void)hipFreeAsync(ijk, s);
+    (void)hipStreamSynchronize(s);
+    return rc;
 }
 
 int nvq_points_to_grid(const float* xyz, uint64_t n, double voxel_size, uint32_t point_type, const char* name,

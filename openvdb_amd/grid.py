@@ -61,7 +61,8 @@ class DeviceGrid:
         """Build a grid on the device from integer voxels: tools::cuda::voxelsToGrid<BuildT>
         (tools/cuda/PointsToGrid.cuh:119-125) for BuildT = ValueOnIndex ("OnIndex") or float.
 
-        ``ijk`` is an [n, 3] int32 CUDA tensor (duplicates allowed).  With ``return_index`` (index
+        ``ijk`` is an [n, 3] int32 CUDA tensor (duplicates allowed), or float32 / float64 index-space
+        coordinates that are rounded to the nearest voxel (Vec3f / Vec3d input).  With ``return_index`` (index
         grids) also returns an int64 CUDA tensor holding each input voxel's value index, so that
         ``channel[index] = values`` fills a channel for ``set_channel``.  The library owns the new
         grid's device buffer (``leaf_values`` is not available on it)."""
@@ -69,16 +70,20 @@ class DeviceGrid:
         codes = {"OnIndex": 20, "float": 1}
         if grid_type not in codes:
             raise ValueError("grid_type must be 'OnIndex' or 'float'")
-        if not (ijk.is_cuda and ijk.dtype == torch.int32 and ijk.dim() == 2 and ijk.shape[1] == 3):
-            raise ValueError("ijk must be an [n, 3] int32 CUDA tensor")
+        if not (ijk.is_cuda and ijk.dtype in (torch.int32, torch.float32, torch.float64) and ijk.dim() == 2 and
+                ijk.shape[1] == 3):
+            raise ValueError("ijk must be an [n, 3] int32, float32 or float64 CUDA tensor")
         ijk = ijk.contiguous()
         n = ijk.shape[0]
         index = torch.empty(n, dtype=torch.int64, device=ijk.device) if return_index else None
         handle = ctypes.c_void_p()
-        check(lib().nvq_voxels_to_grid(ctypes.c_void_p(ijk.data_ptr()), n, codes[grid_type], float(voxel_size),
-                                       None if name is None else name.encode(),
-                                       None if index is None else ctypes.c_void_p(index.data_ptr()),
-                                       _stream_ptr(stream), ctypes.byref(handle)), "nvq_voxels_to_grid")
+        args = (n, codes[grid_type], float(voxel_size), None if name is None else name.encode(),
+                None if index is None else ctypes.c_void_p(index.data_ptr()), _stream_ptr(stream), ctypes.byref(handle))
+        if ijk.dtype == torch.int32:
+            check(lib().nvq_voxels_to_grid(ctypes.c_void_p(ijk.data_ptr()), *args), "nvq_voxels_to_grid")
+        else:  # index-space coordinates rounded to the nearest voxel (Vec3f / Vec3d input)
+            check(lib().nvq_voxels_to_grid_fp(ctypes.c_void_p(ijk.data_ptr()), int(ijk.dtype == torch.float64), *args),
+                  "nvq_voxels_to_grid_fp")
         g = cls.__new__(cls)
         g.handle = handle
         g.device_bytes = None

@@ -161,3 +161,24 @@ def test_device_built_index_grid_marches_and_differentiates_like_the_fog():
     slots = torch.from_numpy(li.astype(np.int64) * 512 + vi).cuda()
     assert int((af[slots] != 0).sum()) > 1000
     assert torch.equal(af[slots], ai[idx])
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_floating_point_voxel_coordinates(oracle, dtype):
+    """Vec3f / Vec3d index-space input (PtrT = Vec3f* or Vec3d*): each coordinate rounds to its
+    nearest voxel, Floor(x + 0.5) in the input's precision; the grid and the per-voxel indices equal
+    the integer build of the rounded coordinates."""
+    nvq = _nvq()
+    r = rng(4321)
+    x = (r.normal(size=(30000, 3)) * 40.0).astype(dtype)
+    x[:6] = np.array([[0.5, -0.5, 1.5], [-1.5, 2.5, -2.5], [0.49999997, -0.50000006, 0.0], [7.5, 8.5, -8.5],
+                      [1e-8, -1e-8, 3.4999999], [-4095.5, 4095.5, 0.5]], dtype)
+    half = np.asarray(0.5, dtype)
+    ijk = np.floor(x + half).astype(np.int32)
+    print(f"[voxels_to_grid] floating input {dtype}", flush=True)
+    g, idx = nvq.DeviceGrid.from_voxels(torch.from_numpy(x).cuda(), "OnIndex", 0.5, return_index=True)
+    ob, oidx = oracle.voxels_to_grid(ijk, 20, 0.5, want_index=True)
+    got = g.download()
+    assert got.nbytes == ob.nbytes and bool((got == ob).all())
+    np.testing.assert_array_equal(idx.cpu().numpy().astype(np.uint64), oidx)
+    g.close()
