@@ -614,6 +614,10 @@ int or_points_to_grid_density(const void* xyz, int f64, uint64_t n, uint32_t tar
     double dx = 1.0;
     uint32_t density = 0;
     for (int iter = 0;;) {
+        if (!(dx > 0.0) || !isfinite(dx)) { /* the update left the positive reals (a target of 1) */
+            free(ijk);
+            return -1;
+        }
         density = max_per_voxel(xf, xd, n, dx, ijk);
         if (++iter >= max_iterations || n == 1 || abs((int)density - (int)target) <= tolerance) break;
         const DensitySample cur = {dx, density};

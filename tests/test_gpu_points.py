@@ -115,3 +115,11 @@ def test_density_matches_oracle(oracle, case):
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
     g.close()
+
+
+def test_density_degenerate_target_is_an_error():
+    nvq = _nvq()
+    print("[points_to_grid_density] degenerate target", flush=True)
+    pts = torch.zeros((4, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(RuntimeError):
+        nvq.DeviceGrid.from_points_density(pts, 1, 0, 10)

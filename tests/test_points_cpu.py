@@ -167,6 +167,14 @@ def test_density_search_stops(oracle):
     assert most2 <= 50
 
 
+def test_density_search_degenerate_target(oracle):
+    """A target of one point per voxel with coincident points drives the interpolated voxel size
+    to 0: an error, not a grid."""
+    pts = np.zeros((4, 3), np.float32)
+    with pytest.raises(ValueError):
+        oracle.points_to_grid_density(pts, 1, 0, 10)
+
+
 def test_too_many_points_in_a_leaf(oracle):
     """More than 65535 points in one leaf: the reference throws; the oracle reports -3."""
     pts = np.zeros((65536, 3), np.float32)
