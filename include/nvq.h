@@ -184,11 +184,13 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
 /* out[i] = float(double(accum[i]) * 2^-fixed_shift) */
 int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream);
 
-/* Device-side construction: tools::cuda::voxelsToGrid<float | ValueOnIndex>(d_ijk, n, voxelSize)
+/* Device-side construction: tools::cuda::voxelsToGrid<float | ValueOnIndex | ValueIndex>(d_ijk, n, voxelSize)
  * (tools/cuda/PointsToGrid.cuh:119-125, :1417-1424).  ijk is a DEVICE array of n integer voxels
  * (nanovdb::Coord, int32 x3; duplicates allowed; 1 <= n < 2^31).  grid_type NVQ_FLOAT gives a
  * grid whose active voxels hold 1 and inactive 0; NVQ_ONINDEX an IndexGrid whose active voxels
- * hold indices 1..count in breadth-first order.  The buffer equals the reference's byte for
+ * hold indices 1..count in breadth-first order; grid_type 19 (GridType::Index, ValueIndex) an
+ * IndexGrid that indexes all 512 values of every leaf, 1 + 512 * leaf + offset (tiles and the
+ * background are index 0).  The buffer equals the reference's byte for
  * byte.  `name` may be NULL.  `index` (index grids only; may be NULL) is a DEVICE array of n that
  * receives each input voxel's value index, so that channel[index[i]] = value[i] fills a float
  * channel for nvq_grid_set_channel.  The new grid owns its device buffer.  The input may span at

@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kB = 256;
 constexpr uint32_t kWaves = kB / 64;
+constexpr uint32_t kGridTypeIndex = 19; // GridType::Index (ValueIndex build type)
 constexpr uint64_t kTreeOff = 672, kRootOff = 736;
 constexpr uint64_t kUpperBytes = 270400, kLowerBytes = 33856;
 
@@ -58,7 +59,8 @@ __host__ __device__ __forceinline__ Cnt headFlags(K k, K q)
 
 struct Layout {
     uint64_t offUpper, offLower, offLeaf, total;
-    uint32_t rootHdr, leafSize, isIndex;
+    uint32_t rootHdr, leafSize, isIndex; // isIndex: ValueOnIndex or ValueIndex
+    uint32_t allIndex;                   // ValueIndex: every leaf value has an index
     int32_t tlo[3];   // smallest tile coordinate (x >> 12) per axis
     uint32_t dim[3];  // tiles per axis in the bbox of the input
     int32_t llo[3];   // smallest lower-node coordinate (x >> 7) per axis
@@ -283,7 +285,9 @@ __global__ __launch_bounds__(kB) void k_v2g_heads(const K* __restrict__ keys, co
             leafFirst[l] = p;
             leafRank[l] = c.v; // 1 + rank of the leaf's first voxel: LeafData<ValueOnIndex>::mOffset
         }
-        if (index) index[ids[p]] = c.v; // 1 + rank of the voxel: its value index (index 0 is the background)
+        // the voxel's value index (index 0 is the background): 1 + its rank among the active voxels
+        // (ValueOnIndex), or 1 + 512 * leaf + its offset in the leaf (ValueIndex)
+        if (index) index[ids[p]] = L.allIndex ? 1ull + 512ull * l + (uint32_t(key) & 511u) : uint64_t(c.v);
         if (p == n - 1) {
             leafFirst[c.l] = n;
             lowerFirst[c.m] = c.l;
@@ -538,8 +542,10 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves(const K* __restrict__ keys, c
                 sum += uint64_t(__popcll(w[j]));
                 prefix |= sum << (9 * j);
             }
-            *reinterpret_cast<uint64_t*>(F + 80) = uint64_t(leafRank[l]); // 1 + rank of the leaf's first voxel
-            *reinterpret_cast<uint64_t*>(F + 88) = prefix;
+            // 1 + values before the leaf: active voxels (ValueOnIndex), or all 512 per leaf with a zero
+            // prefix sum (ValueIndex, tools/cuda/PointsToGrid.cuh:1147-1148)
+            *reinterpret_cast<uint64_t*>(F + 80) = L.allIndex ? 1ull + 512ull * l : uint64_t(leafRank[l]);
+            *reinterpret_cast<uint64_t*>(F + 88) = L.allIndex ? 0ull : prefix;
         }
     }
     if (lane < 8) *reinterpret_cast<uint64_t*>(F + 16 + 8 * lane) = laneSelect8(w, lane);
@@ -657,8 +663,8 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves_sparse(const K* __restrict__ 
     h.w = int32_t((xmax - xmin) | ((ymax - ymin) << 8) | ((zmax - zmin) << 16) | (2u << 24)); // dif[3], flags HasBBox
     *reinterpret_cast<int4*>(F) = h;
     if (L.isIndex) {
-        *reinterpret_cast<uint64_t*>(F + 80) = uint64_t(leafRank[l]); // 1 + rank of the leaf's first voxel
-        *reinterpret_cast<uint64_t*>(F + 88) = prefix;
+        *reinterpret_cast<uint64_t*>(F + 80) = L.allIndex ? 1ull + 512ull * l : uint64_t(leafRank[l]);
+        *reinterpret_cast<uint64_t*>(F + 88) = L.allIndex ? 0ull : prefix;
     } else if (L.isPoint) {
         *reinterpret_cast<uint64_t*>(F + 80) = a;
         *reinterpret_cast<uint64_t*>(F + 88) = b - a;
@@ -791,7 +797,9 @@ void writeHeaders(uint8_t* h, const Layout& L, const uint32_t counts[4], const i
     put(h, 636, gridType);
     put(h, 640, int64_t(L.offMeta)); // blind metadata: only point grids have some
     put(h, 648, uint32_t(L.isPoint ? 1u : 0u));
-    put(h, 656, uint64_t(L.isIndex ? uint64_t(counts[0]) + 1u : L.isPoint ? points : 0u));
+    put(h, 656, uint64_t(L.allIndex  ? 1u + 512u * uint64_t(counts[1])
+                         : L.isIndex ? uint64_t(counts[0]) + 1u
+                         : L.isPoint ? points : 0u)); // value count (index grids) or point count
 }
 
 // Per-device scratch of the builder, kept between builds and grown when a build needs more.  A
@@ -1101,7 +1109,8 @@ int buildFromVoxels(Workspace& W, const int32_t* ijk, uint64_t n, uint32_t grid_
     uint64_t tiles = 1;
     int tileBits = 0, lowerBits = 0;
     keyLayout(box, L, tiles, tileBits, lowerBits);
-    L.isIndex = grid_type == GT_ONINDEX;
+    L.isIndex = grid_type == GT_ONINDEX || grid_type == kGridTypeIndex;
+    L.allIndex = grid_type == kGridTypeIndex;
     L.isPoint = P != nullptr;
     int rc;
     if (tiles == 1 && 21 + lowerBits <= 32)
@@ -1231,8 +1240,8 @@ int nvq_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, doubl
     *out = nullptr;
     if (!ijk || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size))
         return NVQ_ERR_INVALID_ARG;
-    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
-    if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
+    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX && grid_type != kGridTypeIndex) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (index && grid_type == GT_FLOAT) return NVQ_ERR_INVALID_ARG;
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
     Workspace& W = workspaceOf(device);
@@ -1247,8 +1256,8 @@ int nvq_voxels_to_grid_fp(const void* xyz, int xyz_f64, uint64_t n, uint32_t gri
     *out = nullptr;
     if (!xyz || n == 0 || n > uint64_t(INT32_MAX) || !(voxel_size > 0.0) || !std::isfinite(voxel_size))
         return NVQ_ERR_INVALID_ARG;
-    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
-    if (index && grid_type != GT_ONINDEX) return NVQ_ERR_INVALID_ARG;
+    if (grid_type != GT_FLOAT && grid_type != GT_ONINDEX && grid_type != kGridTypeIndex) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (index && grid_type == GT_FLOAT) return NVQ_ERR_INVALID_ARG;
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
     const hipStream_t s = static_cast<hipStream_t>(stream);

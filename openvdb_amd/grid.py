@@ -11,7 +11,7 @@ import numpy as np
 
 from ._lib import GridInfo, check, lib
 
-GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN", 20: "OnIndex", 23: "PointIndex"}
+GRID_TYPES = {1: "float", 13: "Fp4", 14: "Fp8", 15: "Fp16", 16: "FpN", 20: "OnIndex", 19: "Index", 23: "PointIndex"}
 GRID_CLASSES = {0: "Unknown", 1: "LevelSet", 2: "FogVolume", 4: "PointIndex", 5: "PointData", 8: "IndexGrid"}
 
 
@@ -67,9 +67,9 @@ class DeviceGrid:
         ``channel[index] = values`` fills a channel for ``set_channel``.  The library owns the new
         grid's device buffer (``leaf_values`` is not available on it)."""
         import torch
-        codes = {"OnIndex": 20, "float": 1}
+        codes = {"OnIndex": 20, "Index": 19, "float": 1}
         if grid_type not in codes:
-            raise ValueError("grid_type must be 'OnIndex' or 'float'")
+            raise ValueError("grid_type must be 'OnIndex', 'Index' or 'float'")
         if not (ijk.is_cuda and ijk.dtype in (torch.int32, torch.float32, torch.float64) and ijk.dim() == 2 and
                 ijk.shape[1] == 3):
             raise ValueError("ijk must be an [n, 3] int32, float32 or float64 CUDA tensor")

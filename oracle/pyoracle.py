@@ -168,7 +168,8 @@ class Oracle:
         return grad
 
     def voxels_to_grid(self, ijk, grid_type=20, voxel_size=1.0, name=None, want_index=False):
-        """voxelsToGrid<float | ValueOnIndex> restated: (buffer, per-input value index or None)."""
+        """voxelsToGrid<float | ValueOnIndex | ValueIndex> restated (grid_type 1, 20, 19): (buffer,
+        per-input value index or None)."""
         ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
         n = len(ijk)
         index = np.zeros(n, np.uint64) if want_index else None
@@ -290,7 +291,7 @@ class Reference:
         L.ref_validate.argtypes = [_vp, _i]
         L.ref_grid_from_voxels.argtypes = [_vp, _u64, _i, _d, ctypes.POINTER(_u64)]
         L.ref_grid_from_voxels.restype = _vp
-        L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp]
+        L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp, _vp]
         L.ref_update_grid_stats.argtypes = [_vp]
         L.ref_point_voxels.argtypes = [_vp, _vp, _u64, _vp, _vp]
 
@@ -421,13 +422,15 @@ class Reference:
             raise ValueError("not a float grid")
         return out
 
-    def index_values(self, buf, ijk):
-        """ReadAccessor<ValueOnIndex>::getValue per voxel (uint64 value indices)."""
+    def index_values(self, buf, ijk, with_active=False):
+        """ReadAccessor<ValueOnIndex | ValueIndex>::getValue per voxel (uint64 value indices), and
+        isActive with ``with_active``."""
         ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
         out = np.zeros(len(ijk), np.uint64)
-        if self.L.ref_index_values(_p(buf), _p(ijk), len(ijk), _p(out)) != 0:
+        active = np.zeros(len(ijk), np.uint8)
+        if self.L.ref_index_values(_p(buf), _p(ijk), len(ijk), _p(out), _p(active)) != 0:
             raise ValueError("not an index grid")
-        return out
+        return (out, active.astype(bool)) if with_active else out
 
     def grid_from_voxels(self, ijk, grid_type=1, voxel_size=1.0):
         """build::Grid<float> with value 1 at each voxel, converted by createNanoGrid."""

@@ -400,6 +400,8 @@ int ref_validate(const void* buf, int full)
 {
     if (reinterpret_cast<const GridData*>(buf)->mGridType == GridType::PointIndex)
         return tools::isValid(asGrid<Point>(buf), full ? CheckMode::Full : CheckMode::Half, false) ? 1 : 0;
+    if (reinterpret_cast<const GridData*>(buf)->mGridType == GridType::Index)
+        return tools::isValid(asGrid<ValueIndex>(buf), full ? CheckMode::Full : CheckMode::Half, false) ? 1 : 0;
     return dispatch(buf, [&](auto* grid) {
         return tools::isValid(grid, full ? CheckMode::Full : CheckMode::Half, false) ? 1 : 0;
     });
@@ -465,14 +467,24 @@ int ref_point_voxels(const void* buf, const int32_t* ijk, uint64_t n, uint64_t* 
     }
 }
 
-// ReadAccessor<ValueOnIndex>::getValue: the value index of each voxel of an index grid
-// (LeafData<ValueOnIndex>::getValue, NanoVDB.h:4147-4171).
-int ref_index_values(const void* buf, const int32_t* ijk, uint64_t n, uint64_t* out)
+// ReadAccessor<ValueOnIndex | ValueIndex>::getValue: the value index of each voxel of an index
+// grid (LeafData<ValueOnIndex>::getValue, NanoVDB.h:4147-4171; LeafIndexBase for ValueIndex);
+// `active` (may be NULL) receives isActive.
+int ref_index_values(const void* buf, const int32_t* ijk, uint64_t n, uint64_t* out, uint8_t* active)
 {
-    if (reinterpret_cast<const GridData*>(buf)->mGridType != GridType::OnIndex) return -4;
-    auto acc = asGrid<ValueOnIndex>(buf)->getAccessor();
-    for (uint64_t i = 0; i < n; ++i) out[i] = acc.getValue(Coord(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]));
-    return 0;
+    auto run = [&](auto* grid) {
+        auto acc = grid->getAccessor();
+        for (uint64_t i = 0; i < n; ++i) {
+            const Coord c(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+            out[i] = acc.getValue(c);
+            if (active) active[i] = acc.isActive(c) ? 1 : 0;
+        }
+        return 0;
+    };
+    const GridType t = reinterpret_cast<const GridData*>(buf)->mGridType;
+    if (t == GridType::OnIndex) return run(asGrid<ValueOnIndex>(buf));
+    if (t == GridType::Index) return run(asGrid<ValueIndex>(buf));
+    return -4;
 }
 
 // math::isoCrossing (math/HDDA.h:225-248) on world-space rays mapped by Ray::worldToIndexF.

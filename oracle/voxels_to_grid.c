@@ -20,7 +20,7 @@
 #include <string.h>
 #include <math.h>
 
-enum { GT_FLOAT_ = 1, GT_ONINDEX_ = 20 };
+enum { GT_FLOAT_ = 1, GT_INDEX_ = 19, GT_ONINDEX_ = 20 };
 
 typedef struct {
     int32_t t[3];   /* tile coordinates (x>>12, arithmetic) */
@@ -174,7 +174,8 @@ static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
         nLeaf += newTile || (k[p].key >> 9) != (k[p - 1].key >> 9);
         nVox += newTile || k[p].key != k[p - 1].key;
     }
-    const int idx = grid_type == GT_ONINDEX_;
+    const int idx = grid_type == GT_ONINDEX_ || grid_type == GT_INDEX_;
+    const int all = grid_type == GT_INDEX_; /* ValueIndex: every leaf value has an index */
     const uint64_t rootHdr = (idx || pt) ? 96u : 64u, leafSize = idx ? 96u : pt ? 1120u : 2144u;
     const uint64_t offTree = 672, offRoot = offTree + 64, offUpper = offRoot + rootHdr + 32u * nUpper;
     const uint64_t offLower = offUpper + 270400u * nUpper, offLeaf = offLower + 33856u * nLower;
@@ -248,7 +249,9 @@ static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
             put_i32(b + oF + 4, oy);
             put_i32(b + oF + 8, oz);
             b[oF + 15] = 2u; /* HasBBox */
-            if (idx) put_u64(b + oF + 80, (uint64_t)(v + 1) + 1u); /* 1 + active voxels before this leaf */
+            /* 1 + values before this leaf: active voxels (ValueOnIndex), all leaf values (ValueIndex,
+             * tools/cuda/PointsToGrid.cuh:1147) */
+            if (idx) put_u64(b + oF + 80, all ? 1u + 512u * (uint64_t)l : (uint64_t)(v + 1) + 1u);
             if (pt) put_u64(b + oF + 80, p);                       /* points before this leaf */
             leafStart[l] = p;
             leafParent[l] = m;
@@ -264,7 +267,7 @@ static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
          * (SetLeafActiveVoxelStateAndValuesFunctor, tools/cuda/PointsToGrid.cuh:1156-1174) */
         if (pt && (p + 1 == n || k[p + 1].key != k[p].key || memcmp(k[p + 1].t, k[p].t, sizeof(k[p].t)) != 0))
             put_u16(b + offLeaf + leafSize * (uint64_t)l + 96 + 2 * n0, (uint16_t)(p + 1 - leafStart[l]));
-        if (out_index) out_index[k[p].id] = (uint64_t)v + 1u;
+        if (out_index) out_index[k[p].id] = all ? 1u + 512u * (uint64_t)l + n0 : (uint64_t)v + 1u;
     }
 
     /* leaves: updateBBox, prefix sums; bboxes up the tree (processBBox) */
@@ -314,7 +317,7 @@ static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
             put_i32(L + 4 * a, lo[a]);
             L[12 + a] = (uint8_t)(mx[a] - mn[a]);
         }
-        if (idx) {
+        if (idx && !all) { /* ValueIndex leaves keep a zero prefix sum (:1148) */
             uint64_t sum = (uint64_t)__builtin_popcountll(w[0]), prefix = sum;
             for (int j = 1; j < 7; ++j) {
                 sum += (uint64_t)__builtin_popcountll(w[j]);
@@ -397,7 +400,7 @@ static int build_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
     put_u32(b + 636, grid_type);
     put_i64(b + 640, (int64_t)offMeta); /* blind metadata: none follows unless this is a point grid */
     put_u32(b + 648, pt ? 1u : 0u);
-    put_u64(b + 656, idx ? nVox + 1u : pt ? n : 0u);
+    put_u64(b + 656, all ? 1u + 512u * (uint64_t)nLeaf : idx ? nVox + 1u : pt ? n : 0u);
     if (pt) {
         /* GridBlindMetaData (NanoVDB.h:1591-1601), then the points in sorted order (encodePoints,
          * tools/cuda/PointsToGrid.cuh:1262-1317) */
@@ -497,8 +500,8 @@ int or_voxels_to_grid(const int32_t* ijk, uint64_t n, uint32_t grid_type, double
 {
     *out = NULL;
     *bytes_out = 0;
-    if (!ijk || n == 0 || (grid_type != GT_FLOAT_ && grid_type != GT_ONINDEX_)) return -1;
-    if (out_index && grid_type != GT_ONINDEX_) return -1;
+    if (!ijk || n == 0 || (grid_type != GT_FLOAT_ && grid_type != GT_ONINDEX_ && grid_type != GT_INDEX_)) return -1;
+    if (out_index && grid_type == GT_FLOAT_) return -1;
     return build_grid(ijk, n, grid_type, voxel_size, name, NULL, NULL, 0, out, bytes_out, out_index);
 }
 

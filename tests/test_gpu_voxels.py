@@ -22,22 +22,22 @@ def _nvq():
 
 def _build(nvq, ijk, grid_type, voxel_size=0.5, name=None):
     d = torch.from_numpy(np.ascontiguousarray(ijk, np.int32)).cuda()
-    if grid_type == "OnIndex":
+    if grid_type in ("OnIndex", "Index"):
         g, idx = nvq.DeviceGrid.from_voxels(d, grid_type, voxel_size, name=name, return_index=True)
         return g, idx.cpu().numpy().astype(np.uint64)
     return nvq.DeviceGrid.from_voxels(d, grid_type, voxel_size, name=name), None
 
 
-@pytest.mark.parametrize("grid_type", ["OnIndex", "float"])
+@pytest.mark.parametrize("grid_type", ["OnIndex", "Index", "float"])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_build_matches_oracle(oracle, case, grid_type):
     nvq = _nvq()
     ijk = CASES[case]
-    gt = 20 if grid_type == "OnIndex" else 1
+    gt = {"OnIndex": 20, "Index": 19, "float": 1}[grid_type]
     print(f"[voxels_to_grid] case={case} type={grid_type} n={len(ijk)}", flush=True)
     g, idx = _build(nvq, ijk, grid_type)
     got = g.download()
-    ob, oidx = oracle.voxels_to_grid(ijk, gt, 0.5, want_index=gt == 20)
+    ob, oidx = oracle.voxels_to_grid(ijk, gt, 0.5, want_index=gt != 1)
     assert got.nbytes == ob.nbytes
     bad = np.nonzero(got != ob)[0]
     assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"

@@ -82,6 +82,46 @@ def test_index_of_each_input_voxel(oracle, ref):
     np.testing.assert_array_equal(idx, idx[first[inv.reshape(-1)]])
 
 
+def test_reference_kat_value_index(oracle, ref):
+    """Basic_CudaPointsToGrid_ValueIndex (unittest/TestNanoVDB.cu:418-487): voxels (1,2,3), (1,2,4),
+    (8,2,3) give two leaves, 1 + 2 * 512 values, and every leaf voxel (active or not) the index
+    1 + 512 * leaf + its offset in the leaf."""
+    c = np.array([[1, 2, 3], [1, 2, 4], [8, 2, 3]])
+    ob, idx = oracle.voxels_to_grid(c, 19, 1.0, want_index=True)
+    assert len(ob) == 672 + 64 + 96 + 32 + 270400 + 33856 + 2 * 96
+    assert ref.validate(ob, full=True)
+    assert list(_u(ob, 632, np.uint32, 2)) == [8, 19]  # IndexGrid, GridType::Index
+    assert _u(ob, 656, np.uint64)[0] == 1 + 2 * 512
+    q = np.array([[0, 2, 3], [1, 2, 3], [1, 2, 4], [2, 2, 3], [8, 2, 3]])
+    v, act = ref.index_values(ob, q, with_active=True)
+    assert list(v) == [1 + 19, 1 + 83, 1 + 84, 1 + 147, 1 + 512 + 19]
+    assert list(act) == [False, True, True, False, True]
+    np.testing.assert_array_equal(idx, v[[1, 2, 4]])
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_value_index_grid(oracle, ref, case):
+    """ValueIndex against ValueOnIndex on the same voxels: identical topology, masks and bboxes;
+    only the grid type, the value count and the leaves' offsets and prefix sums differ.  The
+    reference's reader gives every voxel of a leaf 1 + 512 * leaf + offset, and the per-voxel index
+    output agrees with it."""
+    ijk = CASES[case]
+    a, ia = oracle.voxels_to_grid(ijk, 19, 0.5, want_index=True)
+    b, _ = oracle.voxels_to_grid(ijk, 20, 0.5, want_index=True)
+    assert len(a) == len(b) and ref.validate(a, full=True)
+    leaf0 = 672 + int(_u(a, 672, np.int64)[0])
+    nleaf = (len(a) - leaf0) // 96
+    keep = np.ones(len(a), bool)
+    keep[636:640] = keep[656:664] = False
+    for i in range(nleaf):
+        keep[leaf0 + 96 * i + 80:leaf0 + 96 * i + 96] = False
+    assert bool((a[keep] == b[keep]).all())
+    assert _u(a, 656, np.uint64)[0] == 1 + 512 * nleaf
+    for i in range(nleaf):
+        assert list(_u(a, leaf0 + 96 * i + 80, np.uint64, 2)) == [1 + 512 * i, 0]
+    np.testing.assert_array_equal(ia, ref.index_values(a, ijk))
+
+
 def test_extreme_coordinates(oracle, ref):
     """Voxels at the int32 limits: root keys of negative tiles and a 2^20-tile span."""
     ijk = np.array([[-2**31, -2**31, -2**31], [2**31 - 1, 2**31 - 1, 2**31 - 1], [0, -1, 5], [-4097, 4096, 0]])
@@ -114,4 +154,12 @@ def test_c_abi_argument_errors():
     assert lib.nvq_voxels_to_grid(dummy, 4, 15, 1.0, None, None, None, ctypes.byref(h)) == -4
     assert lib.nvq_voxels_to_grid(dummy, 4, 1, 1.0, None, dummy, None, ctypes.byref(h)) == -1
     assert lib.nvq_voxels_to_grid(dummy, 4, 20, 1.0, None, None, None, None) == -1
+    assert h.value is None
+    # the same checks on the other builder entry points
+    assert lib.nvq_voxels_to_grid_fp(dummy, 0, 4, 2, 1.0, None, None, None, ctypes.byref(h)) == -4
+    assert lib.nvq_voxels_to_grid_fp(None, 1, 4, 19, 1.0, None, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_points_to_grid(dummy, 4, 1.0, 10, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_points_to_grid_f64(dummy, 4, -1.0, 3, None, None, ctypes.byref(h)) == -1
+    assert lib.nvq_points_to_grid_density(dummy, 0, 4, 0, 1, 10, 3, None, None, ctypes.byref(h), None, None) == -1
+    assert lib.nvq_points_to_grid_density(dummy, 1, 4, 8, 1, 10, 0, None, None, ctypes.byref(h), None, None) == -1
     assert h.value is None
