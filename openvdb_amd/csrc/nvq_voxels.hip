@@ -536,6 +536,8 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves(const K* __restrict__ keys, c
         h.z = origin[2] + int32_t(zmin);
         h.w = int32_t((xmax - xmin) | ((ymax - ymin) << 8) | ((zmax - zmin) << 16) | (2u << 24)); // dif[3], flags HasBBox
         *reinterpret_cast<int4*>(F) = h;
+        if (!L.isIndex && !L.isPoint) // float leaves: min/max/avg/std, zero as the builder leaves them
+            *reinterpret_cast<uint4*>(F + 80) = make_uint4(0u, 0u, 0u, 0u);
         if (L.isIndex) {
             uint64_t sum = uint64_t(__popcll(w[0])), prefix = sum;
 #pragma unroll
@@ -666,7 +668,9 @@ __global__ __launch_bounds__(kB) void k_v2g_leaves_sparse(const K* __restrict__ 
     if (L.isIndex) {
         *reinterpret_cast<uint64_t*>(F + 80) = L.allIndex ? 1ull + 512ull * l : uint64_t(leafRank[l]);
         *reinterpret_cast<uint64_t*>(F + 88) = L.allIndex ? 0ull : prefix;
-    } else if (L.isPoint) {
+    } else if (!L.isPoint) { // float leaves: min/max/avg/std, zero as the builder leaves them
+        *reinterpret_cast<uint4*>(F + 80) = make_uint4(0u, 0u, 0u, 0u);
+    } else {
         *reinterpret_cast<uint64_t*>(F + 80) = a;
         *reinterpret_cast<uint64_t*>(F + 88) = b - a;
         if (b - a > 65535u) atomicMax(maxCount, b - a);
@@ -953,7 +957,9 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     uint32_t* upperFirst = st.take<uint32_t>(uint64_t(tot.u) + 1);
     uint32_t* maxCount = st.take<uint32_t>(1);
     int rc = NVQ_ERR_HIP;
-    if (hipMemsetAsync(buf, 0, L.total, s) == hipSuccess) {
+    // zero the headers, root and internal nodes only: the leaf kernels write every byte of every
+    // leaf, and the point kernels every byte of the blind data
+    if (hipMemsetAsync(buf, 0, L.offLeaf, s) == hipSuccess) {
         k_v2g_header<<<1, 256, 0, s>>>(buf, hdr);
         // 4. nodes, then leaves, then bboxes bottom-up
         {

@@ -182,3 +182,23 @@ def test_floating_point_voxel_coordinates(oracle, dtype):
     assert got.nbytes == ob.nbytes and bool((got == ob).all())
     np.testing.assert_array_equal(idx.cpu().numpy().astype(np.uint64), oidx)
     g.close()
+
+
+@pytest.mark.parametrize("grid_type", ["float", "OnIndex", "Index"])
+def test_build_into_reused_pool_memory(oracle, grid_type):
+    """Only the node region of a new grid is cleared; the leaf kernels write every leaf byte.  A
+    build into stream-ordered pool memory just released by a grid full of non-zero bytes (a point
+    grid of the same extent) must still equal the oracle byte for byte."""
+    nvq = _nvq()
+    print(f"[voxels_to_grid] reused pool memory type={grid_type}", flush=True)
+    ijk = rng(777).integers(-300, 300, (150000, 3))
+    pts = torch.from_numpy(ijk.astype(np.float32)).cuda()
+    for _ in range(2):
+        p = nvq.DeviceGrid.from_points(pts.repeat(3, 1), 1.0, "Voxel8")
+        p.close()
+        g, idx = _build(nvq, ijk, grid_type)
+        gt = {"OnIndex": 20, "Index": 19, "float": 1}[grid_type]
+        ob, _ = oracle.voxels_to_grid(ijk, gt, 0.5)
+        got = g.download()
+        assert got.nbytes == ob.nbytes and bool((got == ob).all())
+        g.close()
