@@ -245,6 +245,12 @@ int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void
  * voxels keep their statistics and add nothing to their parent.  Waits for the device (the
  * grid's cached bbox is refreshed).  Other grid types: NVQ_ERR_UNSUPPORTED_TYPE. */
 int nvq_update_grid_stats(nvq_grid* grid, void* stream);
+/* The same for StatsMode `mode` (tools/cuda/GridStats.cuh:344-362): 1 BBox (bboxes and render
+ * flags only; NoopStats), 2 MinMax (plus the minimum and maximum of every node; a parent merges
+ * the extrema its children store, Extrema), 3 All (= nvq_update_grid_stats).  The grid's
+ * HasMinMax / HasAverage / HasStdDeviation flags follow the mode.  0 (Disable) and other values:
+ * NVQ_ERR_INVALID_ARG, as the reference throws for them. */
+int nvq_update_grid_stats_mode(nvq_grid* grid, int mode, void* stream);
 
 /* tools::cuda::signedFloodFill(d_grid) (tools/cuda/SignedFloodFill.cuh:240-253) for NVQ_FLOAT level
  * sets, in place on the device buffer: every inactive leaf voxel and every non-child node entry

@@ -71,6 +71,7 @@ SIGNATURES = {
                                     ctypes.POINTER(_d), ctypes.POINTER(_u32)], _i),
     "nvq_grid_download": ([_vp, _vp, _u64, _vp], _i),
     "nvq_update_grid_stats": ([_vp, _vp], _i),
+    "nvq_update_grid_stats_mode": ([_vp, _i, _vp], _i),
     "nvq_signed_flood_fill": ([_vp, _vp], _i),
     "nvq_profile_enable": ([_i], _i),
     "nvq_profile_read": ([ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_u64)], _i),

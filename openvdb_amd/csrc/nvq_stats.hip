@@ -39,6 +39,15 @@ __device__ __forceinline__ void extAdd(Acc& s, float v)
     if (v < s.mn) s.mn = v;
     if (v > s.mx) s.mx = v;
 }
+// Extrema::add(const Extrema&): min of the minima, max of the maxima
+__device__ __forceinline__ void extMerge(Acc& s, const Acc& o)
+{
+    if (o.mn < s.mn) s.mn = o.mn;
+    if (o.mx > s.mx) s.mx = o.mx;
+}
+
+// StatsMode (tools/GridStats.h:40-47): what a pass writes besides bboxes and render flags
+enum StatsModeId : int { kModeBBox = 1, kModeMinMax = 2, kModeAll = 3 };
 // Stats::add(value)
 __device__ __forceinline__ void accAdd(Acc& s, float v)
 {
@@ -115,7 +124,8 @@ __device__ __forceinline__ void leafBox(const uint8_t* F, int32_t b[6])
 }
 
 // One 32-lane half-wavefront per leaf.
-__global__ __launch_bounds__(256) void k_stats_leaf(uint8_t* __restrict__ leaf0, uint32_t count, Acc* __restrict__ acc)
+__global__ __launch_bounds__(256) void k_stats_leaf(uint8_t* __restrict__ leaf0, uint32_t count, Acc* __restrict__ acc,
+                                                    int mode)
 {
     const uint32_t lane = threadIdx.x & 31u, l = blockIdx.x * 8u + (threadIdx.x >> 5);
     const bool valid = l < count;
@@ -166,7 +176,12 @@ __global__ __launch_bounds__(256) void k_stats_leaf(uint8_t* __restrict__ leaf0,
         h.z = (h0.z & ~7) + int32_t(zmin);
         h.w = int32_t((xmax - xmin) | ((ymax - ymin) << 8) | ((zmax - zmin) << 16) | (flags << 24));
         *reinterpret_cast<int4*>(F) = h;
-        putStats(F + 80, s);
+        if (mode == kModeAll) {
+            putStats(F + 80, s);
+        } else if (mode == kModeMinMax) { // Extrema::setStats: minimum and maximum only
+            reinterpret_cast<float*>(F + 80)[0] = s.mn;
+            reinterpret_cast<float*>(F + 80)[1] = s.mx;
+        }
         acc[l] = s;
     } else {
         F[15] = uint8_t(flags & ~3u);
@@ -179,7 +194,7 @@ __global__ __launch_bounds__(256) void k_stats_leaf(uint8_t* __restrict__ leaf0,
 template<uint32_t LOG2C, uint32_t LOG2N, bool CHILD_IS_LEAF>
 __global__ __launch_bounds__(128) void k_stats_internal(uint8_t* __restrict__ node0, const uint8_t* __restrict__ child0,
                                                         uint32_t count, const Acc* __restrict__ childAcc,
-                                                        Acc* __restrict__ acc)
+                                                        Acc* __restrict__ acc, int mode)
 {
     constexpr uint32_t SIZE = 1u << (3 * LOG2N);
     constexpr uint64_t kNode = LOG2N == 4 ? kLowerBytes : kUpperBytes, kChild = CHILD_IS_LEAF ? kLeafBytes : kLowerBytes;
@@ -211,7 +226,14 @@ __global__ __launch_bounds__(128) void k_stats_internal(uint8_t* __restrict__ no
                 for (int a = 0; a < 6; ++a) cb[a] = reinterpret_cast<const int32_t*>(C)[a];
             }
             boxExpand(box, cb);
-            accMerge(s, childAcc[ci]);
+            if (mode == kModeAll) {
+                accMerge(s, childAcc[ci]);
+            } else if (mode == kModeMinMax) { // Extrema::add(child.minimum()), add(child.maximum())
+                constexpr uint32_t kChildStats = CHILD_IS_LEAF ? 80u : 1056u;
+                const float* cs = reinterpret_cast<const float*>(C + kChildStats);
+                extAdd(s, cs[0]);
+                extAdd(s, cs[1]);
+            }
         } else if (vmask[i >> 6] & bit) {
             constexpr uint32_t m = (1u << LOG2N) - 1u;
             const int32_t c[3] = {ox + int32_t(((i >> (2 * LOG2N)) & m) << LOG2C), oy + int32_t(((i >> LOG2N) & m) << LOG2C),
@@ -219,7 +241,8 @@ __global__ __launch_bounds__(128) void k_stats_internal(uint8_t* __restrict__ no
             const int32_t tb[6] = {c[0], c[1], c[2], c[0] + int32_t(1u << LOG2C) - 1, c[1] + int32_t(1u << LOG2C) - 1,
                                    c[2] + int32_t(1u << LOG2C) - 1};
             boxExpand(box, tb);
-            accAddN(s, *reinterpret_cast<const float*>(e), 1ull << (3 * LOG2C));
+            if (mode == kModeAll) accAddN(s, *reinterpret_cast<const float*>(e), 1ull << (3 * LOG2C));
+            else if (mode == kModeMinMax) extAdd(s, *reinterpret_cast<const float*>(e));
         }
     }
     sAcc[t] = s;
@@ -228,7 +251,8 @@ __global__ __launch_bounds__(128) void k_stats_internal(uint8_t* __restrict__ no
     __syncthreads();
     for (uint32_t d = 64; d; d >>= 1) {
         if (t < d) {
-            accMerge(sAcc[t], sAcc[t + d]);
+            if (mode == kModeAll) accMerge(sAcc[t], sAcc[t + d]);
+            else extMerge(sAcc[t], sAcc[t + d]);
             boxExpand(sBox[t], sBox[t + d]);
         }
         __syncthreads();
@@ -237,7 +261,12 @@ __global__ __launch_bounds__(128) void k_stats_internal(uint8_t* __restrict__ no
         int32_t* bo = reinterpret_cast<int32_t*>(N);
 #pragma unroll
         for (int a = 0; a < 6; ++a) bo[a] = sBox[0][a];
-        putStats(N + kStats, sAcc[0]);
+        if (mode == kModeAll) {
+            putStats(N + kStats, sAcc[0]);
+        } else if (mode == kModeMinMax) {
+            reinterpret_cast<float*>(N + kStats)[0] = sAcc[0].mn;
+            reinterpret_cast<float*>(N + kStats)[1] = sAcc[0].mx;
+        }
         uint64_t* flags = reinterpret_cast<uint64_t*>(N + 24);
         *flags = *flags & ~1ull;
         acc[k] = sAcc[0];
@@ -246,7 +275,7 @@ __global__ __launch_bounds__(128) void k_stats_internal(uint8_t* __restrict__ no
 
 // Root (tiles in table order) and grid: one thread.
 __global__ void k_stats_root(uint8_t* __restrict__ base, int64_t rootOff, const uint8_t* __restrict__ upper0,
-                             const Acc* __restrict__ upperAcc)
+                             const Acc* __restrict__ upperAcc, int mode)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint8_t* R = base + kGridSize + rootOff;
@@ -266,7 +295,13 @@ __global__ void k_stats_root(uint8_t* __restrict__ base, int64_t rootOff, const 
             if (child) {
                 const int32_t* cb = reinterpret_cast<const int32_t*>(R + child);
                 boxExpand(rb, cb);
-                accMerge(s, upperAcc[uint64_t((R + child) - upper0) / kUpperBytes]);
+                if (mode == kModeAll) {
+                    accMerge(s, upperAcc[uint64_t((R + child) - upper0) / kUpperBytes]);
+                } else if (mode == kModeMinMax) {
+                    const float* cs = reinterpret_cast<const float*>(R + child + 8224);
+                    extAdd(s, cs[0]);
+                    extAdd(s, cs[1]);
+                }
             } else if (*reinterpret_cast<const uint32_t*>(tile + 16)) {
                 // RootData::KeyToCoord (NanoVDB.h:2641-2648)
                 const uint64_t key = *reinterpret_cast<const uint64_t*>(tile);
@@ -274,13 +309,18 @@ __global__ void k_stats_root(uint8_t* __restrict__ base, int64_t rootOff, const 
                                       int32_t(uint32_t((key >> 21) & 0x1fffffu) << 12), int32_t(uint32_t(key & 0x1fffffu) << 12)};
                 const int32_t tb[6] = {c[0], c[1], c[2], c[0] + 4095, c[1] + 4095, c[2] + 4095};
                 boxExpand(rb, tb);
-                accAddN(s, *reinterpret_cast<const float*>(tile + 20), 1ull << 36);
+                if (mode == kModeAll) accAddN(s, *reinterpret_cast<const float*>(tile + 20), 1ull << 36);
+                else if (mode == kModeMinMax) extAdd(s, *reinterpret_cast<const float*>(tile + 20));
             }
         }
-        st[0] = s.mn;
-        st[1] = s.mx;
-        st[2] = float(s.avg);
-        st[3] = accStd(s);
+        if (mode >= kModeMinMax) {
+            st[0] = s.mn;
+            st[1] = s.mx;
+        }
+        if (mode == kModeAll) {
+            st[2] = float(s.avg);
+            st[3] = accStd(s);
+        }
     }
     int32_t* rbo = reinterpret_cast<int32_t*>(R);
     for (int a = 0; a < 6; ++a) rbo[a] = rb[a];
@@ -310,16 +350,21 @@ __global__ void k_stats_root(uint8_t* __restrict__ base, int64_t rootOff, const 
         for (int a = 0; a < 6; ++a) wb[a] = out[a];
         flags |= 2u;
     }
-    *gflags = flags | 4u | 8u | 16u; // HasMinMax | HasAverage | HasStdDeviation
+    // HasMinMax | HasAverage | HasStdDeviation as the mode computes them (:299-301)
+    flags &= ~(4u | 8u | 16u);
+    if (mode >= kModeMinMax) flags |= 4u;
+    if (mode == kModeAll) flags |= 8u | 16u;
+    *gflags = flags;
 }
 
 } // namespace
 
 extern "C" {
 
-int nvq_update_grid_stats(nvq_grid* grid, void* stream)
+int nvq_update_grid_stats_mode(nvq_grid* grid, int mode, void* stream)
 {
-    if (!grid) return NVQ_ERR_INVALID_ARG;
+    // StatsMode::Disable with a grid is an error in the reference (:346-358)
+    if (!grid || mode < kModeBBox || mode > kModeAll) return NVQ_ERR_INVALID_ARG;
     const GridLayout L = gridLayout(grid);
     if (L.type != GT_FLOAT) return NVQ_ERR_UNSUPPORTED_TYPE;
     const hipStream_t s = static_cast<hipStream_t>(stream);
@@ -333,14 +378,18 @@ int nvq_update_grid_stats(nvq_grid* grid, void* stream)
     uint8_t* leaf0 = tree + L.nodeOff[0];
     uint8_t* lower0 = tree + L.nodeOff[1];
     uint8_t* upper0 = tree + L.nodeOff[2];
-    if (L.count[0]) k_stats_leaf<<<(L.count[0] + 7u) / 8u, 256, 0, s>>>(leaf0, L.count[0], leafAcc);
-    if (L.count[1]) k_stats_internal<3, 4, true><<<L.count[1], 128, 0, s>>>(lower0, leaf0, L.count[1], leafAcc, lowerAcc);
-    if (L.count[2]) k_stats_internal<7, 5, false><<<L.count[2], 128, 0, s>>>(upper0, lower0, L.count[2], lowerAcc, upperAcc);
-    k_stats_root<<<1, 64, 0, s>>>(L.base, L.nodeOff[3], upper0, upperAcc);
+    if (L.count[0]) k_stats_leaf<<<(L.count[0] + 7u) / 8u, 256, 0, s>>>(leaf0, L.count[0], leafAcc, mode);
+    if (L.count[1])
+        k_stats_internal<3, 4, true><<<L.count[1], 128, 0, s>>>(lower0, leaf0, L.count[1], leafAcc, lowerAcc, mode);
+    if (L.count[2])
+        k_stats_internal<7, 5, false><<<L.count[2], 128, 0, s>>>(upper0, lower0, L.count[2], lowerAcc, upperAcc, mode);
+    k_stats_root<<<1, 64, 0, s>>>(L.base, L.nodeOff[3], upper0, upperAcc, mode);
     int rc = (hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
     (void)hipFree(acc);
     if (rc == NVQ_OK) rc = gridRefreshBBox(grid);
     return rc;
 }
+
+int nvq_update_grid_stats(nvq_grid* grid, void* stream) { return nvq_update_grid_stats_mode(grid, kModeAll, stream); }
 
 } // extern "C"

@@ -163,10 +163,16 @@ class DeviceGrid:
               "nvq_grid_download")
         return out
 
-    def update_stats(self, stream=None):
-        """Recompute bboxes and statistics on the device (tools::cuda::updateGridStats, float grids),
-        e.g. after the leaf values were changed through ``leaf_values``."""
-        check(lib().nvq_update_grid_stats(self.handle, _stream_ptr(stream)), "nvq_update_grid_stats")
+    STATS_MODES = {"BBox": 1, "MinMax": 2, "All": 3, "Default": 3}
+
+    def update_stats(self, mode="Default", stream=None):
+        """Recompute bboxes and statistics on the device (tools::cuda::updateGridStats(grid, mode),
+        float grids), e.g. after the leaf values were changed through ``leaf_values``.  ``mode`` is
+        a StatsMode name: "BBox", "MinMax" or "All" (= "Default")."""
+        if mode not in self.STATS_MODES:
+            raise ValueError(f"mode must be one of {sorted(self.STATS_MODES)}")
+        check(lib().nvq_update_grid_stats_mode(self.handle, self.STATS_MODES[mode], _stream_ptr(stream)),
+              "nvq_update_grid_stats_mode")
         info = GridInfo()
         check(lib().nvq_grid_get_info(self.handle, ctypes.byref(info)), "nvq_grid_get_info")
         self.info = info

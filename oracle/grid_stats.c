@@ -11,7 +11,12 @@
  *    through the grid's map, grid flags HasBBox | HasMinMax | HasAverage | HasStdDeviation.
  * Stats / Extrema arithmetic: tools/GridStats.h:72-140, :244-344 (FP64 mean and M2, FP32
  * extrema; avg and std stored as float).  Leaves without active voxels keep their statistics
- * and contribute nothing (their accumulator is empty). */
+ * and contribute nothing (their accumulator is empty).
+ * The other modes (updateGridStats :344-362): StatsMode::MinMax (Extrema) writes only the
+ * extrema, and a parent merges the minimum and maximum its children store (Extrema::add of
+ * child.minimum() and child.maximum(), :193-196, :265-268); StatsMode::BBox (NoopStats) writes
+ * no statistics.  The grid's HasMinMax / HasAverage / HasStdDeviation flags follow the mode
+ * (:299-301); an empty root gets the background as minimum and maximum in every mode. */
 #include "oracle.h"
 
 #include <float.h>
@@ -35,6 +40,13 @@ static void ext_add(St* s, float v)
 {
     if (v < s->mn) s->mn = v;
     if (v > s->mx) s->mx = v;
+}
+/* Extrema::add(const Extrema&) (tools/GridStats.h:117-121): min of the minima, max of the maxima
+ * (an empty side, FLT_MAX / -FLT_MAX, is neutral) */
+static void ext_merge(St* s, const St* o)
+{
+    if (o->mn < s->mn) s->mn = o->mn;
+    if (o->mx > s->mx) s->mx = o->mx;
 }
 static void st_add(St* s, float v)
 {
@@ -119,9 +131,10 @@ static void leaf_box(const uint8_t* L, int32_t b[6])
     }
 }
 
-int or_update_grid_stats(void* buf_)
+int or_update_grid_stats_mode(void* buf_, int mode)
 {
     uint8_t* b = (uint8_t*)buf_;
+    if (mode < 1 || mode > 3) return -1; /* Disable with a grid throws in the reference (:357-358) */
     uint32_t type;
     memcpy(&type, b + 636, 4);
     if (type != 1u) return -4;
@@ -170,19 +183,28 @@ int or_update_grid_stats(void* buf_)
                 L[12 + a] = (uint8_t)(mx[a] - mn[a]);
             }
             L[15] |= 2u;
+            if (mode == 2) { /* Extrema over the active values (order-free) */
+                St e = st_empty();
+                for (uint32_t i = 0; i < 512; ++i)
+                    if ((w[i >> 6] >> (i & 63)) & 1u) ext_add(&e, rdf(L + 96 + 4 * i));
+                wrf(L + 80, e.mn);
+                wrf(L + 84, e.mx);
+            }
             St part[32];
-            for (int lane = 0; lane < 32; ++lane) {
+            for (int lane = 0; lane < 32 && mode == 3; ++lane) {
                 part[lane] = st_empty();
                 for (uint32_t i = (uint32_t)lane; i < 512; i += 32)
                     if ((w[i >> 6] >> (i & 63)) & 1u) st_add(&part[lane], rdf(L + 96 + 4 * i));
             }
-            for (int d = 16; d; d >>= 1)
-                for (int lane = 0; lane < d; ++lane) st_merge(&part[lane], &part[lane + d]);
-            leafAcc[l] = part[0];
-            wrf(L + 80, part[0].mn);
-            wrf(L + 84, part[0].mx);
-            wrf(L + 88, (float)part[0].avg);
-            wrf(L + 92, st_std(&part[0]));
+            if (mode == 3) {
+                for (int d = 16; d; d >>= 1)
+                    for (int lane = 0; lane < d; ++lane) st_merge(&part[lane], &part[lane + d]);
+                leafAcc[l] = part[0];
+                wrf(L + 80, part[0].mn);
+                wrf(L + 84, part[0].mx);
+                wrf(L + 88, (float)part[0].avg);
+                wrf(L + 92, st_std(&part[0]));
+            }
         } else {
             L[15] &= (uint8_t)~2u;
         }
@@ -220,7 +242,13 @@ int or_update_grid_stats(void* buf_)
                         else
                             for (int a = 0; a < 6; ++a) cb[a] = rdi(C + 4 * a);
                         box_expand(pbox[t], cb);
-                        st_merge(&part[t], &childAcc[ci]);
+                        if (mode == 3) {
+                            st_merge(&part[t], &childAcc[ci]);
+                        } else if (mode == 2) { /* the child's stored extrema */
+                            const uint32_t cst = level == 1 ? 80u : 1056u;
+                            ext_add(&part[t], rdf(C + cst));
+                            ext_add(&part[t], rdf(C + cst + 4));
+                        }
                     } else if ((rd64(N + vmOff + 8 * (i >> 6)) >> (i & 63)) & 1u) {
                         const uint32_t m = (1u << log2n) - 1u;
                         const int32_t c[3] = {ox + (int32_t)(((i >> (2 * log2n)) & m) << log2c),
@@ -228,20 +256,26 @@ int or_update_grid_stats(void* buf_)
                         const int32_t tb[6] = {c[0], c[1], c[2], c[0] + (1 << log2c) - 1, c[1] + (1 << log2c) - 1,
                                                c[2] + (1 << log2c) - 1};
                         box_expand(pbox[t], tb);
-                        st_add_n(&part[t], rdf(N + tabOff + 8ull * i), childVoxels);
+                        if (mode == 3) st_add_n(&part[t], rdf(N + tabOff + 8ull * i), childVoxels);
+                        else if (mode == 2) ext_add(&part[t], rdf(N + tabOff + 8ull * i));
                     }
                 }
             }
             for (int d = 64; d; d >>= 1)
                 for (int t = 0; t < d; ++t) {
-                    st_merge(&part[t], &part[t + d]);
+                    if (mode == 2) ext_merge(&part[t], &part[t + d]);
+                    else st_merge(&part[t], &part[t + d]);
                     box_expand(pbox[t], pbox[t + d]);
                 }
             for (int a = 0; a < 6; ++a) wri(N + 4 * a, pbox[0][a]);
-            wrf(N + stOff, part[0].mn);
-            wrf(N + stOff + 4, part[0].mx);
-            wrf(N + stOff + 8, (float)part[0].avg);
-            wrf(N + stOff + 12, st_std(&part[0]));
+            if (mode >= 2) {
+                wrf(N + stOff, part[0].mn);
+                wrf(N + stOff + 4, part[0].mx);
+            }
+            if (mode == 3) {
+                wrf(N + stOff + 8, (float)part[0].avg);
+                wrf(N + stOff + 12, st_std(&part[0]));
+            }
             wr64(N + 24, rd64(N + 24) & ~(uint64_t)1);
             acc[k] = part[0];
         }
@@ -269,7 +303,12 @@ int or_update_grid_stats(void* buf_)
                 int32_t cb[6];
                 for (int a = 0; a < 6; ++a) cb[a] = rdi(C + 4 * a);
                 box_expand(rbox, cb);
-                st_merge(&s, &upperAcc[ci]);
+                if (mode == 3) {
+                    st_merge(&s, &upperAcc[ci]);
+                } else if (mode == 2) {
+                    ext_add(&s, rdf(C + 8224));
+                    ext_add(&s, rdf(C + 8228));
+                }
             } else {
                 uint32_t state;
                 memcpy(&state, tile + 16, 4);
@@ -281,14 +320,19 @@ int or_update_grid_stats(void* buf_)
                                           (int32_t)((uint32_t)(key & 0x1fffffu) << 12)};
                     const int32_t tb[6] = {c[0], c[1], c[2], c[0] + 4095, c[1] + 4095, c[2] + 4095};
                     box_expand(rbox, tb);
-                    st_add_n(&s, rdf(tile + 20), (uint64_t)1 << 36);
+                    if (mode == 3) st_add_n(&s, rdf(tile + 20), (uint64_t)1 << 36);
+                    else if (mode == 2) ext_add(&s, rdf(tile + 20));
                 }
             }
         }
-        wrf(root + 32, s.mn);
-        wrf(root + 36, s.mx);
-        wrf(root + 40, (float)s.avg);
-        wrf(root + 44, st_std(&s));
+        if (mode >= 2) {
+            wrf(root + 32, s.mn);
+            wrf(root + 36, s.mx);
+        }
+        if (mode == 3) {
+            wrf(root + 40, (float)s.avg);
+            wrf(root + 44, st_std(&s));
+        }
     }
     for (int a = 0; a < 6; ++a) wri(root + 4 * a, rbox[a]);
 
@@ -316,10 +360,14 @@ int or_update_grid_stats(void* buf_)
         for (int a = 0; a < 6; ++a) wrd(b + 560 + 8 * a, wb[a]);
         flags |= 2u;
     }
-    flags |= 4u | 8u | 16u;
+    flags &= ~(4u | 8u | 16u); /* HasMinMax, HasAverage, HasStdDeviation as the mode has them */
+    if (mode >= 2) flags |= 4u;
+    if (mode == 3) flags |= 8u | 16u;
     memcpy(b + 20, &flags, 4);
     free(leafAcc);
     free(lowerAcc);
     free(upperAcc);
     return 0;
 }
+
+int or_update_grid_stats(void* buf) { return or_update_grid_stats_mode(buf, 3); }

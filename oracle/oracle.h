@@ -78,6 +78,8 @@ int or_points_to_grid_density(const void* xyz, int f64, uint64_t n, uint32_t tar
 /* tools/cuda/GridStats.cuh updateGridStats<float>(StatsMode::Default) restated, in place on a host
  * float grid buffer (bboxes, node statistics, flags, world bbox).  0 ok, -4 not a float grid. */
 int or_update_grid_stats(void* buf);
+/* The same for StatsMode 1 (BBox), 2 (MinMax) or 3 (All = Default); -1 for any other mode. */
+int or_update_grid_stats_mode(void* buf, int mode);
 /* tools/cuda/SignedFloodFill.cuh signedFloodFill<float> restated, in place on a host float grid
  * buffer.  0 ok, -4 not a float grid, -3 a root tile the fill must write is missing. */
 int or_signed_flood_fill(void* buf);

@@ -61,6 +61,7 @@ class Oracle:
                                         ctypes.POINTER(_u64), _vp]
         L.or_free.argtypes = [_vp]
         L.or_update_grid_stats.argtypes = [_vp]
+        L.or_update_grid_stats_mode.argtypes = [_vp, ctypes.c_int]
         L.or_signed_flood_fill.argtypes = [_vp]
         L.or_points_to_grid.argtypes = [_vp, _u64, _d, ctypes.c_uint32, ctypes.c_char_p, ctypes.POINTER(_vp),
                                         ctypes.POINTER(_u64)]
@@ -183,13 +184,15 @@ class Oracle:
         self.L.or_free(p)
         return buf, index
 
-    def update_grid_stats(self, buf):
-        """cuda::updateGridStats<float> restated: a copy of the float grid buffer with its bboxes,
-        statistics and flags recomputed in the device code's merge order."""
+    def update_grid_stats(self, buf, mode=3):
+        """cuda::updateGridStats<float>(grid, mode) restated (mode 1 BBox, 2 MinMax, 3 All = Default):
+        a copy of the float grid buffer with its bboxes, statistics and flags recomputed in the
+        device code's merge order."""
         out = aligned_empty(buf.nbytes)
         out[:] = buf
-        if self.L.or_update_grid_stats(_p(out)) != 0:
-            raise ValueError("not a float grid")
+        st = self.L.or_update_grid_stats_mode(_p(out), mode)
+        if st != 0:
+            raise ValueError("or_update_grid_stats failed: %d" % st)
         return out
 
     def points_to_grid(self, xyz, voxel_size=1.0, point_type=9, name=None):
@@ -292,7 +295,7 @@ class Reference:
         L.ref_grid_from_voxels.argtypes = [_vp, _u64, _i, _d, ctypes.POINTER(_u64)]
         L.ref_grid_from_voxels.restype = _vp
         L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp, _vp]
-        L.ref_update_grid_stats.argtypes = [_vp]
+        L.ref_update_grid_stats.argtypes = [_vp, ctypes.c_int]
         L.ref_point_voxels.argtypes = [_vp, _vp, _u64, _vp, _vp]
 
     def hw_threads(self):
@@ -414,11 +417,11 @@ class Reference:
             raise ValueError("not a point grid")
         return count, first
 
-    def update_grid_stats(self, buf):
-        """Host tools::updateGridStats(StatsMode::Default) on a copy of a float grid buffer."""
+    def update_grid_stats(self, buf, mode=3):
+        """Host tools::updateGridStats(StatsMode(mode)) on a copy of a float grid buffer."""
         out = aligned_empty(buf.nbytes)
         out[:] = buf
-        if self.L.ref_update_grid_stats(_p(out)) != 0:
+        if self.L.ref_update_grid_stats(_p(out), mode) != 0:
             raise ValueError("not a float grid")
         return out
 

@@ -430,12 +430,13 @@ void* ref_grid_from_voxels(const int32_t* ijk, uint64_t n, int grid_type, double
     }
 }
 
-// Host tools::updateGridStats(grid, StatsMode::Default) in place on a float grid
-// (tools/GridStats.h): bboxes and statistics in the host's merge order.
-int ref_update_grid_stats(void* buf)
+// Host tools::updateGridStats(grid, mode) in place on a float grid (tools/GridStats.h): bboxes and
+// statistics in the host's merge order; mode 1 BBox, 2 MinMax, 3 All (= Default).
+int ref_update_grid_stats(void* buf, int mode)
 {
     if (reinterpret_cast<const GridData*>(buf)->mGridType != GridType::Float) return -4;
-    tools::updateGridStats(reinterpret_cast<NanoGrid<float>*>(buf), tools::StatsMode::Default);
+    if (mode < 1 || mode > 3) return -1;
+    tools::updateGridStats(reinterpret_cast<NanoGrid<float>*>(buf), tools::StatsMode(mode));
     return 0;
 }
 

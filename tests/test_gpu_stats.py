@@ -47,6 +47,24 @@ def test_update_stats_matches_oracle(ref, oracle, name):
     assert tuple(dg.index_bbox) == tuple(_u(want, 672 + int(_u(want, 696, np.int64)[0]), np.int32, 6))
 
 
+@pytest.mark.parametrize("mode", ["BBox", "MinMax"])
+@pytest.mark.parametrize("name", ["sphere", "fog_sphere_tiles", "torus", "fbm", "voxels_to_grid"])
+def test_update_stats_modes_match_oracle(ref, oracle, name, mode):
+    """StatsMode::BBox and StatsMode::MinMax (nvq_update_grid_stats_mode) on zeroed statistics."""
+    nvq = _nvq()
+    g = _zero_stats(stats_grids(ref, oracle)[name])
+    print(f"[update_grid_stats] grid={name} mode={mode}", flush=True)
+    dg = nvq.DeviceGrid(g)
+    dg.update_stats(mode)
+    got = dg.download()
+    want = oracle.update_grid_stats(g, {"BBox": 1, "MinMax": 2}[mode])
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
+    with pytest.raises(ValueError):
+        dg.update_stats("Disable")
+    dg.close()
+
+
 def test_update_stats_after_value_edit(oracle):
     """Change leaf values through the leaf_values view, then recompute on the device."""
     nvq = _nvq()

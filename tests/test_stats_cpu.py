@@ -52,6 +52,46 @@ def test_oracle_vs_host_update_grid_stats(grids, oracle, ref, name):
         assert ulps <= 1 or abs(float(fo) - float(fr)) <= 1e-6 * scale, (f, fo, fr)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("name", ["sphere", "fog_sphere_tiles", "torus", "fbm", "voxels_to_grid"])
+def test_bbox_and_minmax_modes_match_host(grids, oracle, ref, name, mode):
+    """StatsMode::BBox and StatsMode::MinMax involve no averages, so the device merge order cannot
+    matter: the oracle equals the host tools::updateGridStats byte for byte, starting from grids
+    whose statistics were first scrambled by a pass in the other mode."""
+    g = oracle.update_grid_stats(grids[name], 3 if mode == 1 else 1)
+    o = oracle.update_grid_stats(g, mode)
+    r = ref.update_grid_stats(g, mode)
+    bad = np.nonzero(o != r)[0]
+    assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[:8]}"
+    flags = int(_u(o, 20, np.uint32)[0])
+    assert flags & 0x1c == (0 if mode == 1 else 4)  # HasMinMax only in MinMax mode
+
+
+def test_modes_write_only_their_fields(grids, oracle):
+    """BBox leaves every statistic as it was; MinMax rewrites the extrema and nothing else;
+    Disable (0) and unknown modes are errors."""
+    g = grids["fbm"]
+    full = oracle.update_grid_stats(g, 3)
+    fields = moment_fields(g)
+    avg_std = np.zeros(len(g), bool)
+    for f in fields:
+        avg_std[f:f + 4] = True
+    mm = oracle.update_grid_stats(full, 2)
+    assert bool((mm[avg_std] == full[avg_std]).all())
+    # extrema: equal to the full pass's extrema
+    ext = np.zeros(len(g), bool)
+    for f, m in fields.items():
+        ext[m:m + 8] = True
+    assert bool((mm[ext] == full[ext]).all())
+    zero = full.copy()
+    zero[ext | avg_std] = 0
+    bb = oracle.update_grid_stats(zero, 1)
+    assert bool((bb[ext | avg_std] == 0).all())
+    for bad_mode in (0, 4):
+        with pytest.raises(ValueError):
+            oracle.update_grid_stats(g, bad_mode)
+
+
 def test_stats_do_not_depend_on_previous_stats(grids, oracle):
     """Zeroed statistics and statistics flags are recomputed to the same bytes."""
     g = grids["fog_sphere_tiles"]
