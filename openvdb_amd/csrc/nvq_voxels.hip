@@ -858,21 +858,25 @@ uint32_t blocksFor(uint64_t items, uint32_t perBlock) { return uint32_t((items +
 // Steps after the bbox: sort keys of width K over endBit bits, node numbering, node contents.
 // The builder's key sort.  32-bit keys of 25-27 bits (single-tile inputs, e.g. a 512^3 volume)
 // take three 9-bit onesweep passes instead of four 8-bit ones (1.11 vs 1.39 ms for 43.6 M keys
-// on MI355X); other widths keep the library's tuned 8-bit passes.
+// on MI355X); other widths keep the library's tuned 8-bit passes.  The input index of each voxel
+// rides along only when a per-voxel index or point data needs it.
 using Onesweep9 = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
     rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 9,
                                         rocprim::block_radix_rank_algorithm::match>>;
 template<typename K>
 hipError_t sortPairs(void* temp, size_t& bytes, const K* keys, K* keys2, const uint32_t* ids, uint32_t* ids2, uint32_t n,
-                     int endBit, hipStream_t s)
+                     int endBit, bool withIds, hipStream_t s)
 {
     if constexpr (std::is_same<K, uint32_t>::value) {
         if ((endBit + 8) / 9 < (endBit + 7) / 8)
-            return rocprim::radix_sort_pairs<Onesweep9>(temp, bytes, keys, keys2, ids, ids2, size_t(n), 0u,
-                                                        unsigned(endBit), s);
+            return withIds ? rocprim::radix_sort_pairs<Onesweep9>(temp, bytes, keys, keys2, ids, ids2, size_t(n), 0u,
+                                                                  unsigned(endBit), s)
+                           : rocprim::radix_sort_keys<Onesweep9>(temp, bytes, keys, keys2, size_t(n), 0u,
+                                                                 unsigned(endBit), s);
     }
-    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys2, ids, ids2, int(n), 0, endBit, s);
+    return withIds ? hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys2, ids, ids2, int(n), 0, endBit, s)
+                   : hipcub::DeviceRadixSort::SortKeys(temp, bytes, keys, keys2, int(n), 0, endBit, s);
 }
 
 template<typename K>
@@ -883,8 +887,10 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     // 2. keys, one radix sort, run-head totals per chunk and their scan
     const uint32_t chunks = uint32_t((uint64_t(n32) + kChunk - 1) / kChunk);
     size_t sortBytes = 0, scanBytes = 0;
+    const bool withIds = index != nullptr || P != nullptr;
     if (sortPairs<K>(nullptr, sortBytes, static_cast<K*>(nullptr), static_cast<K*>(nullptr),
-                     static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), n32, endBit, s) != hipSuccess ||
+                     static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), n32, endBit, withIds, s) !=
+            hipSuccess ||
         hipcub::DeviceScan::InclusiveScan(nullptr, scanBytes, static_cast<Cnt*>(nullptr), static_cast<Cnt*>(nullptr),
                                           CntSum(), int(chunks), s) != hipSuccess)
         return NVQ_ERR_HIP;
@@ -907,7 +913,7 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     k_v2g_keys<K><<<blocksFor(n32, kB), kB, 0, s>>>(ijk, n32, L, keys, ids);
     {
         ProfScope prof("v2g_sort", s);
-        if (sortPairs<K>(temp, sortBytes, keys, keys2, ids, ids2, n32, endBit, s) != hipSuccess)
+        if (sortPairs<K>(temp, sortBytes, keys, keys2, ids, ids2, n32, endBit, withIds, s) != hipSuccess)
             return NVQ_ERR_HIP;
     }
     {
