@@ -45,8 +45,9 @@ __device__ __forceinline__ uint32_t findPrevOn(const uint64_t* w, uint32_t words
     return b ? (n << 6) + 63u - uint32_t(__builtin_clzll(b)) : words * 64u;
 }
 
-__global__ __launch_bounds__(256) void k_sff_leaf(uint8_t* __restrict__ leaf0, uint32_t count, float bg)
+__global__ __launch_bounds__(256) void k_sff_leaf(uint8_t* __restrict__ leaf0, uint32_t count, const float* __restrict__ bgp)
 {
+    const float bg = *bgp;
     const uint32_t lane = threadIdx.x & 63u, l = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (l >= count) return;
     uint8_t* F = leaf0 + kLeafBytes * l;
@@ -87,8 +88,9 @@ __device__ __forceinline__ float upperEdge(const uint8_t* N, bool last)
 
 // One thread per table entry of the nodes of one level (LOG2N 4: lower, 5: upper).
 template<uint32_t LOG2N>
-__global__ __launch_bounds__(256) void k_sff_node(uint8_t* __restrict__ node0, uint32_t count, float bg)
+__global__ __launch_bounds__(256) void k_sff_node(uint8_t* __restrict__ node0, uint32_t count, const float* __restrict__ bgp)
 {
+    const float bg = *bgp;
     constexpr uint32_t SIZE = 1u << (3 * LOG2N), WORDS = SIZE / 64u;
     constexpr uint64_t kNode = LOG2N == 4 ? kLowerBytes : kUpperBytes;
     constexpr uint32_t kChildMask = 32u + SIZE / 8u, kTable = kChildMask + SIZE / 8u + 32u;
@@ -123,9 +125,10 @@ __device__ __forceinline__ void keyToCoord(uint64_t key, int32_t c[3])
 }
 
 // One thread per root tile; tiles that are children look for their successor on their z-scanline.
-__global__ __launch_bounds__(256) void k_sff_root(uint8_t* __restrict__ root, uint32_t tableSize, float bg,
+__global__ __launch_bounds__(256) void k_sff_root(uint8_t* __restrict__ root, uint32_t tableSize, const float* __restrict__ bgp,
                                                   int* __restrict__ missing)
 {
+    const float bg = *bgp;
     const uint32_t a = blockIdx.x * 256u + threadIdx.x;
     if (a >= tableSize) return;
     const uint8_t* ta = root + 64 + 32ull * a;
@@ -185,12 +188,11 @@ int nvq_signed_flood_fill(nvq_grid* grid, void* stream)
     const hipStream_t s = static_cast<hipStream_t>(stream);
     uint8_t* tree = L.base + kGridSize;
     uint8_t* root = tree + L.nodeOff[3];
-    float bg = 0.0f;
-    if (hipMemcpyAsync(&bg, root + kRootBackground, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return NVQ_ERR_HIP;
+    // RootData::mBackground, read by the kernels on the device
+    const float* bg = reinterpret_cast<const float*>(root + kRootBackground);
     int* missing = nullptr;
-    if (hipMalloc(&missing, sizeof(int)) != hipSuccess) return NVQ_ERR_HIP;
+    keepPoolMemory(); // stream-ordered pool: no device-wide synchronisation per call
+    if (hipMallocAsync(reinterpret_cast<void**>(&missing), sizeof(int), s) != hipSuccess) return NVQ_ERR_HIP;
     k_sff_flag_clear<<<1, 64, 0, s>>>(missing);
     if (L.count[0]) k_sff_leaf<<<(L.count[0] + 3u) / 4u, 256, 0, s>>>(tree + L.nodeOff[0], L.count[0], bg);
     if (L.count[1]) k_sff_node<4><<<uint32_t((uint64_t(L.count[1]) * 4096u + 255u) / 256u), 256, 0, s>>>(tree + L.nodeOff[1], L.count[1], bg);
@@ -198,10 +200,10 @@ int nvq_signed_flood_fill(nvq_grid* grid, void* stream)
     if (L.tableSize) k_sff_root<<<(L.tableSize + 255u) / 256u, 256, 0, s>>>(root, L.tableSize, bg, missing);
     int flag = 0;
     int rc = NVQ_ERR_HIP;
-    if (hipGetLastError() == hipSuccess && hipMemcpyAsync(&flag, missing, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
-        hipStreamSynchronize(s) == hipSuccess)
-        rc = flag ? NVQ_ERR_INVALID_ARG : NVQ_OK;
-    (void)hipFree(missing);
+    const bool launched = hipGetLastError() == hipSuccess &&
+                          hipMemcpyAsync(&flag, missing, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess;
+    (void)hipFreeAsync(missing, s);
+    if (launched && hipStreamSynchronize(s) == hipSuccess) rc = flag ? NVQ_ERR_INVALID_ARG : NVQ_OK;
     return rc;
 }
 

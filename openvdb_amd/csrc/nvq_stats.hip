@@ -370,7 +370,8 @@ int nvq_update_grid_stats_mode(nvq_grid* grid, int mode, void* stream)
     const hipStream_t s = static_cast<hipStream_t>(stream);
     const uint64_t slots = uint64_t(L.count[0]) + L.count[1] + L.count[2] + 1;
     Acc* acc = nullptr;
-    if (hipMalloc(&acc, slots * sizeof(Acc)) != hipSuccess) return NVQ_ERR_HIP;
+    keepPoolMemory(); // stream-ordered pool: no device-wide synchronisation per call
+    if (hipMallocAsync(reinterpret_cast<void**>(&acc), slots * sizeof(Acc), s) != hipSuccess) return NVQ_ERR_HIP;
     Acc* leafAcc = acc;
     Acc* lowerAcc = leafAcc + L.count[0];
     Acc* upperAcc = lowerAcc + L.count[1];
@@ -384,8 +385,9 @@ int nvq_update_grid_stats_mode(nvq_grid* grid, int mode, void* stream)
     if (L.count[2])
         k_stats_internal<7, 5, false><<<L.count[2], 128, 0, s>>>(upper0, lower0, L.count[2], lowerAcc, upperAcc, mode);
     k_stats_root<<<1, 64, 0, s>>>(L.base, L.nodeOff[3], upper0, upperAcc, mode);
-    int rc = (hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
-    (void)hipFree(acc);
+    const bool launched = hipGetLastError() == hipSuccess;
+    (void)hipFreeAsync(acc, s);
+    int rc = (launched && hipStreamSynchronize(s) == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
     if (rc == NVQ_OK) rc = gridRefreshBBox(grid);
     return rc;
 }
