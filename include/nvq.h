@@ -242,7 +242,7 @@ int nvq_grid_download(const nvq_grid* grid, void* host_dst, uint64_t bytes, void
  * bbox and the grid's statistics flags.  The merge order is the reference device code's (32
  * lanes per leaf, 128 threads per internal node), so the bits are its bits; they can differ in
  * the last place of an average from the host tools::updateGridStats.  Leaves without active
- * voxels keep their statistics and add nothing to their parent.  Waits for the device (the
+ * voxels keep their statistics and add nothing to their parent.  Waits for `stream` (the
  * grid's cached bbox is refreshed).  Other grid types: NVQ_ERR_UNSUPPORTED_TYPE. */
 int nvq_update_grid_stats(nvq_grid* grid, void* stream);
 /* The same for StatsMode `mode` (tools/cuda/GridStats.cuh:344-362): 1 BBox (bboxes and render
