@@ -38,7 +38,8 @@ enum nvq_status {
     NVQ_ERR_MISALIGNED = -5,      /* buffer not 32-byte aligned, NanoVDB.h:69-74 */
     NVQ_ERR_TRUNCATED = -6,       /* buffer shorter than GridData::mGridSize / node offsets out of range */
     NVQ_ERR_HIP = -7,             /* a HIP runtime call failed */
-    NVQ_ERR_NO_DEVICE = -8        /* no usable gfx950 device */
+    NVQ_ERR_NO_DEVICE = -8,       /* no usable gfx950 device */
+    NVQ_ERR_OVERFLOW = -9         /* a value left the fixed-point range of an accumulator (adjoint) */
 };
 
 /* NanoVDB GridType values the kernels decode (NanoVDB.h:219-247).  NVQ_ONINDEX grids (ValueOnIndex,
@@ -173,10 +174,14 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
  * outputs radiance[n] (and transmittance[n] when dL_dtransmittance is given) and the loss
  * gradients per ray.  Contributions are ADDED to grad_accum — Float grids: [leafCount*512] (leaf
  * order of the buffer, voxel offset n = x<<6|y<<3|z within the leaf); index grids: [value count]
- * (the channel's own order, a dense scatter) — as int64 fixed point with 2^-fixed_shift resolution (range +-2^(63-fixed_shift)); the
+ * (the channel's own order, a dense scatter) — as int64 fixed point with 2^-fixed_shift resolution; the
  * sum is deterministic for any schedule.  The caller zeroes grad_accum before the first call
  * and may accumulate several calls (e.g. cameras).  Transmittance is recomputed by re-marching
- * each ray; no per-step state is stored. */
+ * each ray; no per-step state is stored.  Range: each ray's contribution to one value is converted
+ * with saturation at +-2^62 (a non-finite one counts as 0); when any contribution saturates or is
+ * not finite the call returns NVQ_ERR_OVERFLOW and grad_accum holds the saturated sums.  The int64
+ * additions wrap, so a value's total is exact whenever the true total lies within
+ * +-2^(63-fixed_shift), however its partial sums ran.  Waits for `stream` (to read the overflow flag). */
 int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
                       float dt, float sigma_t, const float* radiance, const float* transmittance,
                       const float* dL_dradiance, const float* dL_dtransmittance, int64_t* grad_accum, int fixed_shift,

@@ -28,12 +28,26 @@
 namespace nvq {
 namespace {
 
+// float -> int64 fixed point, round to nearest, saturating at +-2^62 (float->int conversion is
+// undefined beyond the int64 range); a non-finite or saturated value clears `ok`.
+__device__ __forceinline__ long long toFixed(float x, bool& ok)
+{
+    constexpr float kLim = 4611686018427387904.0f; // 2^62
+    if (fabsf(x) < kLim) return __float2ll_rn(x);
+    ok = false;
+    if (x != x) return 0;
+    return x > 0.0f ? (1ll << 62) : -(1ll << 62);
+}
+
+__global__ void k_clear_flag(uint32_t* flag) { *flag = 0u; }
+
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const float* __restrict__ eye,
                                                           const float* __restrict__ dir, Sched s, float dt, float sigma,
                                                           const float* __restrict__ Lfwd, const float* __restrict__ Tfwd,
                                                           const float* __restrict__ gLin, const float* __restrict__ gTin,
-                                                          unsigned long long* __restrict__ accum, float scale)
+                                                          unsigned long long* __restrict__ accum, float scale,
+                                                          uint32_t* __restrict__ overflow)
 {
     uint64_t i;
     if (!schedule(s, i)) return;
@@ -47,8 +61,9 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
     const float tTerm = gTin ? gT * sdt * Tfwd[i] : 0.0f; // Tfwd may be NULL when no dL/dT is given
     float L = 0.0f, T = 1.0f;
     CellWindow<true> W;
+    bool range = true; // every contribution converted without saturation
     auto sink = [&](int32_t key, float sum) {
-        const long long q = __float2ll_rn(sum * scale);
+        const long long q = toFixed(sum * scale, range);
         if (q != 0) atomicAdd(accum + key, static_cast<unsigned long long>(q));
     };
     marchSteps(acc, g, e, d, dt, [&](float x0, float x1, float x2) {
@@ -75,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
             }
     });
     W.flush(sink);
+    if (!range) atomicOr(overflow, 1u);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fixed_to_float(const long long* __restrict__ in, uint64_t n, double inv,
@@ -92,11 +108,22 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
 {
     const Sched sc = makeSched(n, width);
     const float scale = ldexpf(1.0f, shift);
-    ProfScope prof("k_fog_march_bwd", s);
+    uint32_t* flag = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flag);
     auto kernel = g.type == GT_ONINDEX ? k_fog_march_bwd<GT_ONINDEX> : k_fog_march_bwd<GT_FLOAT>;
-    hipLaunchKernelGGL(kernel, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
-                       gT, reinterpret_cast<unsigned long long*>(accum), scale);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    {
+        ProfScope prof("k_fog_march_bwd", s);
+        hipLaunchKernelGGL(kernel, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
+                           gT, reinterpret_cast<unsigned long long*>(accum), scale, flag);
+    }
+    bool ok = hipGetLastError() == hipSuccess;
+    uint32_t over = 0;
+    ok = hipMemcpyAsync(&over, flag, sizeof(over), hipMemcpyDeviceToHost, s) == hipSuccess && ok;
+    ok = hipFreeAsync(flag, s) == hipSuccess && ok;
+    ok = hipStreamSynchronize(s) == hipSuccess && ok;
+    if (!ok) return -1;
+    return over ? -2 : 0;
 }
 
 int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hipStream_t s)

@@ -334,8 +334,11 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     hipLaunchKernelGGL(k_fill_u32, dim3((zeroWords + kThreads - 1) / kThreads), dim3(kThreads), 0, s, begin, zeroWords,
                        0u);
     hipLaunchKernelGGL(k_bin_keys, dim3(gridN), dim3(kThreads), 0, s, g, L, xyz, n, world, keys, idx);
-    if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, idx, idx2, n, 0, int(L.bits), s) != hipSuccess)
-        st = -4;
+    if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, keys, keys2, idx, idx2, n, 0, int(L.bits), s) != hipSuccess) {
+        // the sorted arrays were never written: launch nothing that reads them
+        (void)hipGetLastError();
+        return hipFreeAsync(base, s) == hipSuccess ? -4 : -5;
+    }
     hipLaunchKernelGGL(k_bin_bounds, dim3(gridN), dim3(kThreads), 0, s, keys2, n, begin, end);
     switch (g.type) {
     case GT_FLOAT: launchSample<GT_FLOAT>(g, L, xyz, world, idx2, begin, end, rec, wantNN, wantVal, gm, spill, spillCount, s); break;

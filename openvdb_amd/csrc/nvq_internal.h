@@ -83,6 +83,10 @@ struct MultiArgs {
 };
 int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
                int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
+// Argument checks and tile layout of a multi-grid query (nvq_kernels.hip); NVQ_OK or a status.
+int makeMultiArgs(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                  const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
+                  float dt, MultiArgs& A);
 
 // ------------------------------------------------------------------ persistent work queue
 // Tiles [0, total) are handed out in chunks of kQueueChunk through 8 counters, one per XCD
@@ -159,6 +163,6 @@ struct GridLayout {
     uint32_t tableSize, type;
 };
 GridLayout gridLayout(const nvq_grid* g);
-int gridRefreshBBox(nvq_grid* g);
+int gridRefreshBBox(nvq_grid* g, hipStream_t s);
 
 } // namespace nvq

@@ -9,6 +9,7 @@
 #include "../../include/nvq.h"
 #include "../../include/nvq_build.h"
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -80,15 +81,27 @@ struct File {
     {
         if (f) std::fclose(f);
     }
+    uint64_t size = 0; // file length in bytes
+    bool open(const char* path)
+    {
+        f = std::fopen(path, "rb");
+        if (!f || std::fseek(f, 0, SEEK_END) != 0) return false;
+        const long end = std::ftell(f);
+        if (end < 0) return false;
+        size = uint64_t(end);
+        return true;
+    }
+    // [pos, pos + n) lies inside the file (sizes read from the file are checked before use)
+    bool holds(uint64_t pos, uint64_t n) const { return pos <= size && n <= size - pos; }
     bool read(uint64_t pos, void* dst, uint64_t n)
     {
-        return std::fseek(f, long(pos), SEEK_SET) == 0 && std::fread(dst, 1, n, f) == n;
+        return holds(pos, n) && std::fseek(f, long(pos), SEEK_SET) == 0 && std::fread(dst, 1, n, f) == n;
     }
 };
 
 int take(File& F, uint64_t pos, uint64_t size, void** out, uint64_t* bytes)
 {
-    if (size < kGridData + kTreeData) return NVQ_ERR_TRUNCATED;
+    if (size < kGridData + kTreeData || !F.holds(pos, size)) return NVQ_ERR_TRUNCATED;
     uint8_t* buf = static_cast<uint8_t*>(std::aligned_alloc(32, (size + 31) & ~uint64_t(31)));
     if (!buf) return NVQ_ERR_INVALID_ARG;
     if (!F.read(pos, buf, size)) {
@@ -121,8 +134,7 @@ int nvqb_read_nvdb(const char* path, int32_t index, const char* name, void** out
     *out = nullptr;
     *bytes = 0;
     File F;
-    F.f = std::fopen(path, "rb");
-    if (!F.f) return NVQ_ERR_INVALID_ARG;
+    if (!F.open(path)) return NVQ_ERR_INVALID_ARG;
     uint8_t head[24];
     if (!F.read(0, head, sizeof(head))) return NVQ_ERR_TRUNCATED;
     const uint64_t magic = rd<uint64_t>(head);
@@ -136,7 +148,11 @@ int nvqb_read_nvdb(const char* path, int32_t index, const char* name, void** out
             const uint64_t m = rd<uint64_t>(g);
             if ((m != kMagicGrid && m != kMagicNumb) || !compatible(rd<uint32_t>(g + 16))) return NVQ_ERR_BAD_MAGIC;
             const uint64_t size = rd<uint64_t>(g + 32); // GridData::mGridSize
+            // a grid smaller than its own headers, or running past the end of the file, is corrupt
+            // (and a zero size would never advance)
+            if (size < kGridData + kTreeData || !F.holds(pos, size)) return NVQ_ERR_TRUNCATED;
             if (name ? nameMatches(g, name) : k == index) return take(F, pos, size, out, bytes);
+            if (k == INT32_MAX) return NVQ_ERR_INVALID_ARG;
             pos += size;
         }
     }
@@ -165,7 +181,10 @@ int nvqb_read_nvdb(const char* path, int32_t index, const char* name, void** out
             me.nameKey = rd<uint64_t>(fm + 16);
             const uint32_t nameSize = rd<uint32_t>(fm + 136);
             pos += kFileMeta;
-            std::vector<char> nm(nameSize + 1, '\0');
+            // sizes come from the file: the name must lie inside it (checked in 64 bits, so a
+            // nameSize of 0xFFFFFFFF cannot wrap the buffer size), and so must the grid
+            if (!F.holds(pos, nameSize) || !F.holds(pos + nameSize, 0)) return NVQ_ERR_TRUNCATED;
+            std::vector<char> nm(size_t(nameSize) + 1, '\0');
             if (nameSize && !F.read(pos, nm.data(), nameSize)) return NVQ_ERR_TRUNCATED;
             me.name.assign(nm.data());
             pos += nameSize;
@@ -176,6 +195,7 @@ int nvqb_read_nvdb(const char* path, int32_t index, const char* name, void** out
                 if (codec != 0) return NVQ_ERR_UNSUPPORTED_TYPE; // ZIP / BLOSC segments are not decoded here
                 return take(F, pos, me.gridSize, out, bytes);
             }
+            if (!F.holds(pos, me.fileSize)) return NVQ_ERR_TRUNCATED;
             pos += me.fileSize;
         }
     }

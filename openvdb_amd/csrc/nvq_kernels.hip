@@ -46,16 +46,19 @@ nvq::GridLayout nvq::gridLayout(const nvq_grid* g)
 }
 
 // The device buffer's root bbox may have changed (statistics recomputed on the device): re-read
-// it into the handle's view and info.  Waits for the device first, as queries may use the view.
-int nvq::gridRefreshBBox(nvq_grid* g)
+// it into the handle's view and info, on stream s, and wait for s.  The caller's in-place update
+// already requires that no query on other streams reads the grid meanwhile.
+int nvq::gridRefreshBBox(nvq_grid* g, hipStream_t s)
 {
     int32_t box[6];
-    if (hipDeviceSynchronize() != hipSuccess ||
-        hipMemcpy(box, static_cast<const uint8_t*>(g->dbuf) + kGridSize + g->nodeOff[3] + kRootBBox, sizeof(box),
-                  hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpyAsync(box, static_cast<const uint8_t*>(g->dbuf) + kGridSize + g->nodeOff[3] + kRootBBox, sizeof(box),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return NVQ_ERR_HIP;
     for (int i = 0; i < 6; ++i) g->view.bbox[i] = g->info.index_bbox[i] = box[i];
-    if (hipMemcpy(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice) != hipSuccess) return NVQ_ERR_HIP;
+    if (hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return NVQ_ERR_HIP;
     return NVQ_OK;
 }
 
@@ -336,6 +339,7 @@ const char* nvq_status_string(int status)
     case NVQ_ERR_TRUNCATED: return "grid buffer is truncated or has out-of-range node offsets";
     case NVQ_ERR_HIP: return "HIP runtime error";
     case NVQ_ERR_NO_DEVICE: return "no usable GPU device";
+    case NVQ_ERR_OVERFLOW: return "a value exceeded the fixed-point range of the gradient accumulator (saturated)";
     default: return "unknown status";
     }
 }
@@ -565,10 +569,9 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
     if (image_width && n % image_width) return NVQ_ERR_INVALID_ARG;
     if (n == 0 || (!dL_dradiance && !dL_dtransmittance)) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
-    return fogMarchBackward(grid->view, eye, dir, n, image_width, dt, sigma_t, radiance, transmittance, dL_dradiance,
-                            dL_dtransmittance, grad_accum, fixed_shift, S(stream)) == 0
-               ? NVQ_OK
-               : NVQ_ERR_HIP;
+    const int st = fogMarchBackward(grid->view, eye, dir, n, image_width, dt, sigma_t, radiance, transmittance,
+                                    dL_dradiance, dL_dtransmittance, grad_accum, fixed_shift, S(stream));
+    return st == 0 ? NVQ_OK : (st == -2 ? NVQ_ERR_OVERFLOW : NVQ_ERR_HIP);
 }
 
 int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream)
@@ -578,13 +581,16 @@ int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shif
     return fixedToFloat(accum, count, fixed_shift, out, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
 }
 
-int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
-                    const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
-                    float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal,
-                    void* stream)
+} // extern "C"
+
+// Validate the arguments of a multi-grid query and lay out its grids' tiles (shared by
+// nvq_multi_query and nvq_multi_query_noalloc).  Reads host memory only.
+int nvq::makeMultiArgs(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                       const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
+                       float dt, MultiArgs& A)
 {
     if (!host_grids || !host_ray_offsets || k == 0 || k > kMaxGrids || !(dt > 0.0f)) return NVQ_ERR_INVALID_ARG;
-    MultiArgs A{};
+    A = MultiArgs{};
     A.k = k;
     uint64_t tiles = 0;
     for (uint32_t j = 0; j < k; ++j) {
@@ -615,7 +621,19 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
     }
     if (tiles >= (uint64_t(1) << 31)) return NVQ_ERR_INVALID_ARG;
     A.totalTiles = uint32_t(tiles);
-    if (tiles == 0) return NVQ_OK;
+    return NVQ_OK;
+}
+
+extern "C" {
+
+int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                    const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
+                    float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal,
+                    void* stream)
+{
+    MultiArgs A;
+    const int st = makeMultiArgs(host_grids, k, host_ray_offsets, host_image_width, host_ops, eye, dir, dt, A);
+    if (st != NVQ_OK || A.totalTiles == 0) return st;
     return multiQuery(A, eye, dir, dt, sigma_t, hit, ijk, out_a, out_b, normal, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
 }
 

@@ -388,7 +388,7 @@ int nvq_update_grid_stats_mode(nvq_grid* grid, int mode, void* stream)
     const bool launched = hipGetLastError() == hipSuccess;
     (void)hipFreeAsync(acc, s);
     int rc = (launched && hipStreamSynchronize(s) == hipSuccess) ? NVQ_OK : NVQ_ERR_HIP;
-    if (rc == NVQ_OK) rc = gridRefreshBBox(grid);
+    if (rc == NVQ_OK) rc = gridRefreshBBox(grid, s);
     return rc;
 }
 

@@ -51,6 +51,7 @@ class DeviceGrid:
                                          ctypes.c_void_p(self.device_bytes.data_ptr()), s, ctypes.byref(handle)),
               "nvq_grid_upload_into")
         self.handle = handle
+        self._device = self.device_bytes.device
         torch.cuda.synchronize()  # the copy is asynchronous: keep the host bytes alive until it is done
         info = GridInfo()
         check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
@@ -87,6 +88,7 @@ class DeviceGrid:
         g = cls.__new__(cls)
         g.handle = handle
         g.device_bytes = None
+        g._device = torch.device("cuda", torch.cuda.current_device())
         info = GridInfo()
         check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
         g.info = info
@@ -117,6 +119,7 @@ class DeviceGrid:
         g = cls.__new__(cls)
         g.handle = handle
         g.device_bytes = None
+        g._device = torch.device("cuda", torch.cuda.current_device())
         info = GridInfo()
         check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
         g.info = info
@@ -147,6 +150,7 @@ class DeviceGrid:
         g = cls.__new__(cls)
         g.handle = handle
         g.device_bytes = None
+        g._device = torch.device("cuda", torch.cuda.current_device())
         info = GridInfo()
         check(lib().nvq_grid_get_info(handle, ctypes.byref(info)), "nvq_grid_get_info")
         g.info = info
@@ -232,6 +236,11 @@ class DeviceGrid:
         check(lib().nvq_grid_set_channel(self.handle, ctypes.c_void_p(channel.data_ptr()), channel.numel()),
               "nvq_grid_set_channel")
         self._channel = channel
+
+    @property
+    def device(self):
+        """The torch device whose memory holds the grid; queries take tensors on this device."""
+        return self._device
 
     @property
     def grid_type(self):

@@ -33,9 +33,17 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _points(x, dtype, name):
-    if not x.is_cuda:
+def _on_grid(x, grid, name):
+    """x must be a CUDA tensor on the grid's device: the kernels dereference its pointer there."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
         raise ValueError(f"{name} must be a CUDA tensor")
+    dev = getattr(grid, "device", None)
+    if dev is not None and x.device != dev:
+        raise ValueError(f"{name} is on {x.device}, the grid on {dev}")
+
+
+def _points(x, dtype, name, grid=None):
+    _on_grid(x, grid, name)
     if x.dtype != dtype:
         raise ValueError(f"{name} must be {dtype}")
     if x.dim() != 2 or x.shape[1] != 3:
@@ -43,8 +51,34 @@ def _points(x, dtype, name):
     return x.contiguous()
 
 
+def _out(x, grid, name, shape, dtype=torch.float32):
+    """An output the kernels write in place: a contiguous tensor of exactly this dtype, shape and
+    device (a narrower dtype would be written past its end)."""
+    if not isinstance(x, torch.Tensor) or x.dtype != dtype:
+        raise ValueError(f"{name} must be a {dtype} tensor, not {getattr(x, 'dtype', type(x))}")
+    if tuple(x.shape) != tuple(shape) or not x.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous tensor of shape {list(shape)}")
+    _on_grid(x, grid, name)
+
+
+def _rays(grid, eye, dir):
+    eye = _points(eye, torch.float32, "eye", grid)
+    dir = _points(dir, torch.float32, "dir", grid)
+    if eye.shape[0] != dir.shape[0]:
+        raise ValueError("eye and dir must hold the same number of rays")
+    return eye, dir
+
+
+def _per_ray(x, grid, name, n):
+    """A per-ray float32 input, kept alive by the caller's reference for the length of the call."""
+    _on_grid(x, grid, name)
+    if x.numel() != n:
+        raise ValueError(f"{name} must hold {n} values")
+    return x.reshape(-1).contiguous().float()
+
+
 def get_value(grid, ijk, want_dim=False):
-    ijk = _points(ijk, torch.int32, "ijk")
+    ijk = _points(ijk, torch.int32, "ijk", grid)
     n = ijk.shape[0]
     val = torch.empty(n, dtype=torch.float32, device=ijk.device)
     act = torch.empty(n, dtype=torch.uint8, device=ijk.device)
@@ -54,7 +88,7 @@ def get_value(grid, ijk, want_dim=False):
 
 
 def sample(grid, xyz, world=False, nn=True, val=True, grad_mode=GRAD_FD6):
-    xyz = _points(xyz, torch.float32, "xyz")
+    xyz = _points(xyz, torch.float32, "xyz", grid)
     n = xyz.shape[0]
     dev = xyz.device
     o_nn = torch.empty(n, dtype=torch.float32, device=dev) if nn else None
@@ -68,18 +102,17 @@ def sample(grid, xyz, world=False, nn=True, val=True, grad_mode=GRAD_FD6):
 def sample_from_voxels(points, grid, values, gradients=None):
     """Same arguments and semantics as the reference op: world-space ``points[N,3]``,
     trilinear ``values[N]`` and optional FD6 ``gradients[N,3]`` written in place."""
-    points = _points(points, torch.float32, "points")
+    points = _points(points, torch.float32, "points", grid)
     n = points.shape[0]
-    if values.shape[0] != n or not values.is_contiguous():
-        raise ValueError("values must be a contiguous [N] tensor")
-    if gradients is not None and (tuple(gradients.shape) != (n, 3) or not gradients.is_contiguous()):
-        raise ValueError("gradients must be a contiguous [N, 3] tensor")
+    _out(values, grid, "values", (n,))
+    if gradients is not None:
+        _out(gradients, grid, "gradients", (n, 3))
     check(lib().nvq_sample(grid.handle, _ptr(points), n, 1, None, _ptr(values), _ptr(gradients), GRAD_FD6,
                            _stream()), "nvq_sample")
 
 
 def grad_stencil(grid, ijk):
-    ijk = _points(ijk, torch.int32, "ijk")
+    ijk = _points(ijk, torch.int32, "ijk", grid)
     n = ijk.shape[0]
     dev = ijk.device
     g = torch.empty((n, 3), dtype=torch.float32, device=dev)
@@ -92,8 +125,7 @@ def grad_stencil(grid, ijk):
 
 
 def zero_crossing(grid, eye, dir, image_width=0):
-    eye = _points(eye, torch.float32, "eye")
-    dir = _points(dir, torch.float32, "dir")
+    eye, dir = _rays(grid, eye, dir)
     n = eye.shape[0]
     dev = eye.device
     hit = torch.empty(n, dtype=torch.uint8, device=dev)
@@ -107,8 +139,7 @@ def zero_crossing(grid, eye, dir, image_width=0):
 
 
 def iso_crossing(grid, eye, dir, iso, image_width=0):
-    eye = _points(eye, torch.float32, "eye")
-    dir = _points(dir, torch.float32, "dir")
+    eye, dir = _rays(grid, eye, dir)
     n = eye.shape[0]
     dev = eye.device
     hit = torch.empty(n, dtype=torch.uint8, device=dev)
@@ -137,8 +168,7 @@ def render_iso_surface(grid, camera, persistent=True, device="cuda"):
 
 
 def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):
-    eye = _points(eye, torch.float32, "eye")
-    dir = _points(dir, torch.float32, "dir")
+    eye, dir = _rays(grid, eye, dir)
     n = eye.shape[0]
     L = torch.empty(n, dtype=torch.float32, device=eye.device)
     T = torch.empty(n, dtype=torch.float32, device=eye.device)
@@ -157,8 +187,9 @@ def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.0
     zeroCrossing, fog volumes the march; index grids need ``ops``).  Returns
     (hit, ijk, a, b, normal): zeroCrossing gives hit/ijk/t/v/normal, the march gives radiance in
     ``a`` and transmittance in ``b``."""
-    eye = _points(eye, torch.float32, "eye")
-    dir = _points(dir, torch.float32, "dir")
+    eye, dir = _rays(grids[0] if len(grids) else None, eye, dir)
+    for g in grids[1:]:
+        _on_grid(eye, g, "eye")
     k = len(grids)
     if len(offsets) != k + 1:
         raise ValueError("offsets must have len(grids) + 1 entries")
@@ -182,23 +213,30 @@ def fog_march_backward(grid, eye, dir, radiance, grad_radiance, transmittance=No
                        sigma_t=0.05, image_width=0, accum=None, fixed_shift=40):
     """Adjoint of ``fog_march``: adds dLoss/dvalue (int64 fixed point, 2^-fixed_shift) into
     ``accum`` [leafCount*512] and returns it (allocated and zeroed when None)."""
-    eye = _points(eye, torch.float32, "eye")
-    dir = _points(dir, torch.float32, "dir")
+    eye, dir = _rays(grid, eye, dir)
     n = eye.shape[0]
     if accum is None:
         accum = torch.zeros(grid.grad_size, dtype=torch.int64, device=eye.device)
-    gr = None if grad_radiance is None else grad_radiance.contiguous().float()
-    gt = None if grad_transmittance is None else grad_transmittance.contiguous().float()
-    check(lib().nvq_fog_march_bwd(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t,
-                                  _ptr(radiance.contiguous()), _ptr(None if transmittance is None else transmittance.contiguous()),
+    else:
+        _out(accum.view(-1) if accum.is_contiguous() else accum, grid, "accum", (grid.grad_size,), torch.int64)
+    # float32 copies bound to locals, so each stays alive (and distinct) until the call returns
+    L = _per_ray(radiance, grid, "radiance", n)
+    T = None if transmittance is None else _per_ray(transmittance, grid, "transmittance", n)
+    gr = None if grad_radiance is None else _per_ray(grad_radiance, grid, "grad_radiance", n)
+    gt = None if grad_transmittance is None else _per_ray(grad_transmittance, grid, "grad_transmittance", n)
+    check(lib().nvq_fog_march_bwd(grid.handle, _ptr(eye), _ptr(dir), n, image_width, dt, sigma_t, _ptr(L), _ptr(T),
                                   _ptr(gr), _ptr(gt), _ptr(accum), fixed_shift, _stream()), "nvq_fog_march_bwd")
     return accum
 
 
 def fixed_to_float(accum, fixed_shift=40, out=None):
+    if not isinstance(accum, torch.Tensor) or not accum.is_cuda or accum.dtype != torch.int64 or \
+            not accum.is_contiguous():
+        raise ValueError("accum must be a contiguous int64 CUDA tensor")
     if out is None:
         out = torch.empty(accum.shape, dtype=torch.float32, device=accum.device)
-    elif out.numel() != accum.numel() or out.dtype != torch.float32 or not out.is_contiguous():
+    elif not out.is_cuda or out.device != accum.device or out.numel() != accum.numel() or \
+            out.dtype != torch.float32 or not out.is_contiguous():
         raise ValueError("out must be a contiguous float32 tensor with accum's element count")
     check(lib().nvq_grad_fixed_to_float(_ptr(accum), accum.numel(), fixed_shift, _ptr(out), _stream()),
           "nvq_grad_fixed_to_float")
