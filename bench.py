@@ -100,6 +100,23 @@ def fog_surface(torch, nvq, grid, gen, m):
     return torch.cat(out)[:m]
 
 
+def c2_points(torch, nvq, torus, fog, n, seed=1234):
+    """The point-query batches of the headline workload, generated on the GPU from one seeded
+    generator: 2^k points for the torus, then for the fog (half uniform, half near the surface)."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    p_torus = make_points(torch, nvq, torus, n, gen, torus_surface(torch, gen, 1 << 20))
+    p_fog = make_points(torch, nvq, fog, n, gen, fog_surface(torch, nvq, fog, gen, 1 << 20))
+    return p_torus, p_fog, gen
+
+
+def c2_fog_grid(nvq, size=512):
+    """The headline fog: fBm (6 octaves, base frequency 1/64, seed 7, amplitude 1) thresholded at 0.15."""
+    s = size - 1
+    return nvq.tools.create_fog_fbm([0, 0, 0, s, s, s], octaves=6, base_freq=1.0 / 64.0, seed=7, amplitude=1.0,
+                                    threshold=0.15)
+
+
 def leaf_bytes(grid):
     sizes = {1: 2144, 13: 352, 14: 608, 15: 1120}
     return grid.leaf_count * sizes.get(grid.info.grid_type, 2144)
@@ -158,10 +175,12 @@ def shell_cameras(nvq, count, res, centre, radius, seed, fov=45.0):
     return np.concatenate(eyes), np.concatenate(dirs)
 
 
-def diff_march(torch, nvq, steps):
-    """Differentiable march: forward time vs forward + L2-loss adjoint time (one step each)."""
+def diff_march(torch, nvq, steps, cpu=True):
+    """Differentiable march: forward time vs forward + L2-loss adjoint time (one step each); the
+    rooflines of both kernels; with `cpu`, the gradient error and CPU timing on one camera."""
     box = [0, 0, 0, 255, 255, 255]
-    fp = nvq.DeviceGrid(nvq.tools.create_fog_fbm(box, seed=7))
+    fp_np = nvq.tools.create_fog_fbm(box, seed=7)
+    fp = nvq.DeviceGrid(fp_np)
     ft = nvq.DeviceGrid(nvq.tools.create_fog_fbm(box, seed=11))
     e, d = shell_cameras(nvq, 64, 256, (128.0, 128.0, 128.0), 600.0, seed=2024)
     E, D = torch.from_numpy(e).cuda(), torch.from_numpy(d).cuda()
@@ -185,9 +204,46 @@ def diff_march(torch, nvq, steps):
     wb, pb = timed_launches(torch, [fwd_bwd], steps, 2)
     kb = kernel_ms("k_fog_march_bwd")[0] / steps
     L, _ = fwd()
-    return {"rays": n, "leaves": fp.leaf_count, "fwd_ms": 1e3 * wf / steps, "fwd_bwd_ms": 1e3 * wb / steps,
-            "ratio": wb / wf, "fwd_rays_per_s": n * steps / wf, "fwd_kernel_ms": kf, "bwd_kernel_ms": kb,
-            "loss": float(((L - target) ** 2).sum().item()), "grad_nonzero": int((accum != 0).sum().item())}
+    nz = int((accum != 0).sum().item())
+    leaves = int((accum.view(-1, 512) != 0).any(1).sum().item())
+    U = leaves * 2144
+    out = {"rays": n, "leaves": fp.leaf_count, "fwd_ms": 1e3 * wf / steps, "fwd_bwd_ms": 1e3 * wb / steps,
+           "ratio": wb / wf, "fwd_rays_per_s": n * steps / wf, "fwd_kernel_ms": kf, "bwd_kernel_ms": kb,
+           "loss": float(((L - target) ** 2).sum().item()), "grad_nonzero": nz, "leaves_touched": leaves,
+           "roofline": roofline_entry("k_fog_march_bwd", n * (24 + 4 + 4) + 8 * nz + U, kb,
+                                      "per ray 24 B eye/dir + 4 B radiance + 4 B dL/dradiance; 8 B per non-zero "
+                                      "int64 accumulator; the float values of the leaves the rays touch (2144 B each)"),
+           "fwd_roofline": roofline_entry("k_fog_march", n * (24 + 8) + U, kf,
+                                          "per ray 24 B eye/dir + 8 B radiance/transmittance; the leaves the rays "
+                                          "touch (2144 B each)")}
+    if cpu:
+        out["cpu_baseline"], out["grad_error"] = c4_cpu_leg(torch, nvq, fp_np, fp, E, D, target)
+    return out
+
+
+def c4_cpu_leg(torch, nvq, fp_np, fp, E, D, target, m=256 * 256):
+    """Camera 0 of the differentiable march (256^2 rays): the GPU adjoint against the oracle's FP64
+    adjoint over every leaf value (the gradient error of the metric), and the oracle's
+    single-thread FP64 forward + adjoint timed as the CPU baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle
+    E0, D0 = E[:m].contiguous(), D[:m].contiguous()
+    L0, _ = nvq.ops.fog_march(fp, E0, D0, 0.5, 0.05, image_width=256)
+    gL = (2.0 * (L0 - target[:m])).contiguous()
+    acc = nvq.ops.fog_march_backward(fp, E0, D0, L0, gL, dt=0.5, sigma_t=0.05, image_width=256)
+    g = nvq.ops.fixed_to_float(acc).cpu().numpy().astype(np.float64)
+    w = gL.cpu().numpy().astype(np.float64)
+    o = Oracle()
+    t0 = time.perf_counter()
+    og = o.fog_march_adjoint(fp_np, E0.cpu().numpy(), D0.cpu().numpy(), 0.5, 0.05, w)
+    t = time.perf_counter() - t0
+    err = {"grad_rel_err": float(np.linalg.norm(g - og) / np.linalg.norm(og)),
+           "grad_max_err_rel_to_max": float(np.abs(g - og).max() / np.abs(og).max()),
+           "values": int(og.size), "nonzero": int((og != 0).sum()),
+           "against": "FP64 oracle adjoint (or_fog_march_adjoint), camera 0 of 64, L2 loss weights 2(L - target)"}
+    base = {"value": m / t, "unit": "rays/s (forward + adjoint)", "cores": 1, "kind": "port",
+            "sample": "camera 0 of the 64 (256^2 rays): the oracle's FP64 re-march + adjoint, one thread"}
+    return base, err
 
 
 def multi_grid_set(nvq):
@@ -210,7 +266,7 @@ def multi_grid_set(nvq):
         return specs, list(ex.map(build, specs))
 
 
-def multi_query_bench(torch, nvq, steps, total_log2=24):
+def multi_query_bench(torch, nvq, steps, total_log2=24, cpu=True):
     from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
     specs, grids = multi_grid_set(nvq)
@@ -238,16 +294,17 @@ def multi_query_bench(torch, nvq, steps, total_log2=24):
         eyes.append(e)
         dirs.append(d)
         offs.append(offs[-1] + side * side)
-    E = torch.from_numpy(np.concatenate(eyes)).cuda()
-    D = torch.from_numpy(np.concatenate(dirs)).cuda()
+    e_np, d_np = np.concatenate(eyes), np.concatenate(dirs)
+    E = torch.from_numpy(e_np).cuda()
+    D = torch.from_numpy(d_np).cuda()
     n = E.shape[0]
+    fog_rays = sum(s * s for (k, _), s in zip(specs, sides) if k == "fog")
     out = {"rays": n, "grid_rays": [s * s for s in sides], "grids": [f"{k}:{p}" for k, p in specs], "codecs": {}}
     for codec in ("fp16", "fp8", "fpn"):
         t0 = time.time()
         with ThreadPoolExecutor(8) as ex:
             q = list(ex.map(lambda g: nvq.tools.quantize(g, codec, 1e-3), grids))
         dg = [nvq.DeviceGrid(x) for x in q]
-        del q
         log(f"C5 {codec}: quantised and uploaded in {time.time() - t0:.1f}s")
         res = {}
 
@@ -256,13 +313,104 @@ def multi_query_bench(torch, nvq, steps, total_log2=24):
 
         w, p = timed_launches(torch, [run], steps, 1)
         hit = res["r"][0]
-        out["codecs"][codec] = {"rays_per_s": n * steps / w, "ms": 1e3 * w / steps,
-                                "kernel_ms": kernel_ms("k_multi_query")[0] / steps,
-                                "bytes": int(sum(g.nbytes for g in dg)), "hits": int(hit.sum().item())}
+        kms = kernel_ms("k_multi_query")[0] / steps
+        gbytes = int(sum(g.nbytes for g in dg))
+        # zeroCrossing rays: 24 B in, hit 1 + ijk 12 + t 4 + v 4 + normal 12 out; march rays: 24 B in, 8 B out
+        alg = (n - fog_rays) * (24 + 33) + fog_rays * (24 + 8) + gbytes
+        c = {"rays_per_s": n * steps / w, "ms": 1e3 * w / steps, "kernel_ms": kms, "bytes": gbytes,
+             "hits": int(hit.sum().item()),
+             "roofline": roofline_entry("k_multi_query", alg, kms,
+                                        "per zeroCrossing ray 24 B in + 33 B out, per march ray 24 B in + 8 B out, "
+                                        "plus every byte of the 16 grids (an upper bound of the footprint)")}
+        if cpu:
+            c["cpu_baseline"] = c5_cpu_baseline(specs, q, e_np, d_np, offs)
+            c["errors"] = c5_errors(torch, nvq, specs, grids, q, dg, e_np, d_np, offs, res["r"], codec)
+        out["codecs"][codec] = c
         for g in dg:
             g.close()
-        log(f"C5 {codec}: {out['codecs'][codec]['rays_per_s']/1e6:.1f} M rays/s")
+        del q
+        log(f"C5 {codec}: {c['rays_per_s']/1e6:.1f} M rays/s")
     return out
+
+
+def c5_cpu_baseline(specs, quant, e_np, d_np, offs, stride=16):
+    """The reference host code on the quantised grids (zeroCrossing for level sets, the march for
+    fogs) over every `stride`-th ray of each grid, on the host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Reference, have_reference
+    if not have_reference():
+        return None
+    ref = Reference()
+    nt = host_threads()
+    sel = [(int(offs[j]), int(offs[j + 1])) for j in range(len(specs))]
+
+    def run():
+        for (kind, _), q, (lo, hi) in zip(specs, quant, sel):
+            e, d = e_np[lo:hi:stride], d_np[lo:hi:stride]
+            if kind == "fog":
+                ref.fog_march(q, e, d, 0.5, 0.05, nthreads=nt, fp32=True, fp64=False)
+            else:
+                ref.zero_crossing(q, e, d, nthreads=nt)
+    rays = sum(len(range(lo, hi, stride)) for lo, hi in sel)
+    t = median_time(run, runs=3, warm=1)
+    return {"value": rays / t, "unit": "rays/s", "cores": nt, "kind": "reference",
+            "sample": f"every {stride}th ray of each of the 16 grids ({rays} rays); ReadAccessor + zeroCrossing "
+                      f"(level sets) or the FP32 march over the reference HDDA and sampler (fogs), g++ -O2, "
+                      f"median of 3 runs after 1 warm-up; {nt} threads"}
+
+
+def c5_errors(torch, nvq, specs, sources, quant, dgrids, e_np, d_np, offs, gpu_out, codec, npts=1 << 14, stride=8):
+    """Per grid, against the FP64 oracle: (i) arithmetic error, the GPU's FP32 decode + sample vs an
+    FP64 evaluation of the same quantised codes; (ii) codec error, the FP64 evaluation of the
+    quantised grid vs that of its FP32 source.  Point samples (uniform in the padded bbox) and the
+    workload's own rays (every `stride`-th): relative radiance errors for fogs, hit-voxel and hit-time
+    differences for level sets."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle
+    o = Oracle()
+    hit, ijk, a = (t.cpu().numpy() for t in gpu_out[:3])
+    pts, gpu_val = [], []
+    for j, dg in enumerate(dgrids):
+        b = dg.index_bbox
+        lo, hi = np.array(b[:3], np.float64) - 2, np.array(b[3:], np.float64) + 3
+        x = (np.random.default_rng(900 + j).random((npts, 3)) * (hi - lo) + lo).astype(np.float32)
+        _, v, _ = nvq.ops.sample(dg, torch.from_numpy(x).cuda(), nn=False, grad_mode=0)
+        pts.append(x)
+        gpu_val.append(v.cpu().numpy().astype(np.float64))
+
+    def one(j):
+        (kind, p), src, q = specs[j], sources[j], quant[j]
+        vq, _ = o.sample_f64(q, pts[j])
+        vs, _ = o.sample_f64(src, pts[j])
+        r = {"grid": f"{kind}:{p}", "sample_arith_max_abs": float(np.abs(gpu_val[j] - vq).max()),
+             "sample_codec_max_abs": float(np.abs(vq - vs).max())}
+        lo, hi = int(offs[j]), int(offs[j + 1])
+        e, d = e_np[lo:hi:stride], d_np[lo:hi:stride]
+        if kind == "fog":
+            Lq, _, _ = o.fog_march(q, e, d, 0.5, 0.05)
+            Ls, _, _ = o.fog_march(src, e, d, 0.5, 0.05)
+            lit = Ls > 1e-3
+            g = a[lo:hi:stride].astype(np.float64)
+            r["ray_arith_max_rel"] = float(np.max(np.abs(g - Lq)[lit] / Lq[lit])) if lit.any() else 0.0
+            r["ray_codec_max_rel"] = float(np.max(np.abs(Lq - Ls)[lit] / Ls[lit])) if lit.any() else 0.0
+            r["rays_checked"] = int(lit.sum())
+        else:
+            oh, oijk, ot, _, _ = o.zero_crossing(q, e, d)
+            sh, sijk, st, _, _ = o.zero_crossing(src, e, d)
+            gh, gijk = hit[lo:hi:stride], ijk[lo:hi:stride]
+            r["ray_arith_mismatches"] = int((gh != oh).sum() + (np.any(gijk != oijk, axis=1) & (oh == 1)).sum())
+            both = (oh == 1) & (sh == 1)
+            r["ray_codec_moved_hits"] = int((np.any(oijk != sijk, axis=1) & both).sum() + (oh != sh).sum())
+            r["ray_codec_max_dt"] = float(np.abs(ot[both] - st[both]).max()) if both.any() else 0.0
+            r["rays_checked"] = int(len(oh))
+        return r
+
+    with ThreadPoolExecutor(8) as ex:
+        rows = list(ex.map(one, range(len(specs))))
+    log(f"C5 {codec} errors: sample (i) max {max(x['sample_arith_max_abs'] for x in rows):.2e}, "
+        f"(ii) max {max(x['sample_codec_max_abs'] for x in rows):.2e}")
+    return rows
 
 
 def fog_voxels(torch, nvq, fog, size, gen):
@@ -432,6 +580,40 @@ def cpu_baseline(grids_np, pts_np, log2n):
                       f"median of 5 runs after 1 warm-up; {threads} threads"}
 
 
+def host_threads():
+    """Host threads for the reference CPU legs: every core of the box, capped at 16 (the GPU box's
+    CPU share for one GPU)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def median_time(fn, runs=5, warm=1):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def roofline_entry(kernel, alg_bytes, avg_ms, note):
+    """HBM roofline of one kernel: algorithmic bytes per launch over its average launch time."""
+    gbs = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": int(alg_bytes),
+            "kernel_avg_ms": avg_ms, "bytes": note}
+
+
+def touched_leaves(torch, nvq, grid, E, D, width, dt=0.5, sigma=0.05):
+    """Leaves whose values a march reads: those the adjoint (dL/dradiance = 1) writes to."""
+    L, _ = nvq.ops.fog_march(grid, E, D, dt, sigma, image_width=width)
+    acc = nvq.ops.fog_march_backward(grid, E, D, L, torch.ones_like(L), dt=dt, sigma_t=sigma, image_width=width)
+    leaves = int((acc.view(-1, 512) != 0).any(1).sum().item())
+    del acc
+    return leaves
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "sample_fd6_traffic.json")
     if not os.path.exists(path):
@@ -453,18 +635,13 @@ def main():
 
     t0 = time.time()
     torus_np = nvq.tools.create_level_set_torus(300.0, 80.0, (0, 0, 0), 1.0, 3.0)
-    fs = args.fog_size - 1
-    fog_np = nvq.tools.create_fog_fbm([0, 0, 0, fs, fs, fs], octaves=6, base_freq=1.0 / 64.0, seed=7, amplitude=1.0,
-                                      threshold=0.15)
+    fog_np = c2_fog_grid(nvq, args.fog_size)
     log(f"grids built in {time.time() - t0:.1f}s: torus {torus_np.nbytes/1e6:.1f} MB, fog {fog_np.nbytes/1e6:.1f} MB")
     torus = nvq.DeviceGrid(torus_np)
     fog = nvq.DeviceGrid(fog_np)
 
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(1234)
     n = 1 << args.log2_points
-    p_torus = make_points(torch, nvq, torus, n, gen, torus_surface(torch, gen, 1 << 20))
-    p_fog = make_points(torch, nvq, fog, n, gen, fog_surface(torch, nvq, fog, gen, 1 << 20))
+    p_torus, p_fog, gen = c2_points(torch, nvq, torus, fog, n)
     outs = []
     for _ in range(2):
         outs.append((torch.empty(n, device="cuda"), torch.empty(n, device="cuda"), torch.empty((n, 3), device="cuda")))
@@ -515,9 +692,10 @@ def main():
 
     if not args.no_extra:
         # C1: level-set ray hit, 512x512 camera at (0,0,400) FOV 45, sphere R=100
-        sph = nvq.DeviceGrid(nvq.tools.create_level_set_sphere(100.0, (0, 0, 0), 1.0, 3.0))
-        e, d = nvq.tools.camera_rays(512, 512, (0.0, 0.0, 400.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0)
-        e, d = torch.from_numpy(e).cuda(), torch.from_numpy(d).cuda()
+        sph_np = nvq.tools.create_level_set_sphere(100.0, (0, 0, 0), 1.0, 3.0)
+        sph = nvq.DeviceGrid(sph_np)
+        e_np, d_np = nvq.tools.camera_rays(512, 512, (0.0, 0.0, 400.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0)
+        e, d = torch.from_numpy(e_np).cuda(), torch.from_numpy(d_np).cuda()
         hit = torch.empty(512 * 512, dtype=torch.uint8, device="cuda")
         ijk = torch.empty((512 * 512, 3), dtype=torch.int32, device="cuda")
         tt = torch.empty(512 * 512, device="cuda")
@@ -529,14 +707,29 @@ def main():
             lib.nvq_zero_crossing(sph.handle, P(e), P(d), 512 * 512, 512, P(hit), P(ijk), P(tt), P(vv), P(nr),
                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
         w1, p1 = timed_launches(torch, [c1], 50, 5)
-        extra["c1_rays_per_s"] = 512 * 512 * 50 / w1
-        extra["c1_kernel_ms"] = kernel_ms("k_zero_crossing")[0] / 50
-        extra["c1_hits"] = int(hit.sum().item())
+        k1 = kernel_ms("k_zero_crossing")[0] / 50
+        hit_leaves = int(torch.unique(torch.div(ijk[hit.bool()], 8, rounding_mode="floor"), dim=0).shape[0])
+        c1r = {"rays_per_s": 512 * 512 * 50 / w1, "kernel_ms": k1, "hits": int(hit.sum().item()),
+               "roofline": roofline_entry("k_zero_crossing", 512 * 512 * (24 + 33) + hit_leaves * 2144, k1,
+                                          "per ray 24 B eye/dir in, hit 1 + ijk 12 + t 4 + v 4 + normal 12 B out; "
+                                          "the leaves holding hit voxels (2144 B each, a lower bound of the "
+                                          "footprint)")}
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            from pyoracle import Reference, have_reference
+            if have_reference():
+                ref, nt = Reference(), host_threads()
+                t = median_time(lambda: ref.zero_crossing(sph_np, e_np, d_np, nthreads=nt))
+                c1r["cpu_baseline"] = {"value": 512 * 512 / t, "unit": "rays/s", "cores": nt, "kind": "reference",
+                                       "sample": f"the whole 512^2 camera; Ray::worldToIndexF + math::zeroCrossing "
+                                                 f"+ trilinear-gradient normal, g++ -O2, median of 5 runs after 1 "
+                                                 f"warm-up; {nt} threads"}
+        extra["c1"] = c1r
         # C3: fog march, 1024x1024 rays, dt 0.5, sigma 0.05, camera outside the bbox looking at its centre
         c = args.fog_size / 2.0
-        e3, d3 = nvq.tools.camera_rays(1024, 1024, (c + 0.35 * args.fog_size, c + 0.45 * args.fog_size,
-                                                     c + 1.6 * args.fog_size), (c, c, c), (0.0, 1.0, 0.0), 45.0)
-        e3, d3 = torch.from_numpy(e3).cuda(), torch.from_numpy(d3).cuda()
+        e3_np, d3_np = nvq.tools.camera_rays(1024, 1024, (c + 0.35 * args.fog_size, c + 0.45 * args.fog_size,
+                                                           c + 1.6 * args.fog_size), (c, c, c), (0.0, 1.0, 0.0), 45.0)
+        e3, d3 = torch.from_numpy(e3_np).cuda(), torch.from_numpy(d3_np).cuda()
         L = torch.empty(1024 * 1024, device="cuda")
         T = torch.empty(1024 * 1024, device="cuda")
 
@@ -544,11 +737,28 @@ def main():
             lib.nvq_fog_march(fog.handle, P(e3), P(d3), 1024 * 1024, 1024, 0.5, 0.05, P(L), P(T),
                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
         w3, p3 = timed_launches(torch, [c3], 10, 2)
-        extra["c3_rays_per_s"] = 1024 * 1024 * 10 / w3
-        extra["c3_kernel_ms"] = kernel_ms("k_fog_march")[0] / 10
-        extra["c3_mean_transmittance"] = float(T.mean().item())
-        log(f"C1 {extra['c1_rays_per_s']/1e9:.2f} G rays/s ({extra['c1_hits']} hits); "
-            f"C3 {extra['c3_rays_per_s']/1e6:.1f} M rays/s")
+        k3 = kernel_ms("k_fog_march")[0] / 10
+        mean_t = float(T.mean().item())
+        leaves3 = touched_leaves(torch, nvq, fog, e3, d3, 1024)
+        c3r = {"rays_per_s": 1024 * 1024 * 10 / w3, "kernel_ms": k3, "mean_transmittance": mean_t,
+               "leaves_touched": leaves3,
+               "roofline": roofline_entry("k_fog_march", 1024 * 1024 * (24 + 8) + leaves3 * 2144, k3,
+                                          "per ray 24 B eye/dir in, 8 B radiance/transmittance out; the leaves the "
+                                          "rays read (2144 B each)")}
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            from pyoracle import Reference, have_reference
+            if have_reference():
+                ref, nt = Reference(), host_threads()
+                es, ds = e3_np[::16].copy(), d3_np[::16].copy()
+                t = median_time(lambda: ref.fog_march(fog_np, es, ds, 0.5, 0.05, nthreads=nt, fp32=True, fp64=False),
+                                runs=3)
+                c3r["cpu_baseline"] = {"value": len(es) / t, "unit": "rays/s", "cores": nt, "kind": "reference",
+                                       "sample": f"every 16th ray of the 1024^2 camera ({len(es)} rays); the FP32 "
+                                                 f"march over the reference HDDA, ReadAccessor and trilinear sampler "
+                                                 f"(oracle/ref_wrap.cpp), g++ -O2, median of 3; {nt} threads"}
+        extra["c3"] = c3r
+        log(f"C1 {c1r['rays_per_s']/1e9:.2f} G rays/s ({c1r['hits']} hits); C3 {c3r['rays_per_s']/1e6:.1f} M rays/s")
         # iso-surface example (examples/ex_raytrace_iso_surface): createLevelSetSphere(100, (-20,0,0)),
         # 4096^2 pixels, iso = voxel size, RenderOp camera; persistent and tiled launches
         iso_np = nvq.tools.create_level_set_sphere(100.0, (-20.0, 0.0, 0.0), 1.0, 3.0)
@@ -570,11 +780,12 @@ def main():
         log(f"iso-surface 4096^2: persistent {extra['iso_render_persistent']['rays_per_s']/1e9:.2f} G rays/s, "
             f"tiled {extra['iso_render_tiled']['rays_per_s']/1e9:.2f} G rays/s")
         if not args.no_diff:
-            extra["c4"] = diff_march(torch, nvq, 5)
+            extra["c4"] = diff_march(torch, nvq, 5, cpu=not args.no_cpu)
+            ge = extra["c4"].get("grad_error", {}).get("grad_rel_err")
             log(f"C4: fwd {extra['c4']['fwd_ms']:.2f} ms, fwd+bwd {extra['c4']['fwd_bwd_ms']:.2f} ms, "
-                f"ratio {extra['c4']['ratio']:.2f}")
+                f"ratio {extra['c4']['ratio']:.2f}, gradient error vs FP64 {ge}")
         if not args.no_multi:
-            extra["c5"] = multi_query_bench(torch, nvq, 3)
+            extra["c5"] = multi_query_bench(torch, nvq, 3, cpu=not args.no_cpu)
         if not args.no_build:
             vox = fog_voxels(torch, nvq, fog, args.fog_size, gen)
             extra["voxels_to_grid"] = build_bench(torch, nvq, vox, 5, 2)
