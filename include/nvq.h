@@ -81,9 +81,24 @@ int nvq_grid_upload_into(const void* host_buf, uint64_t bytes, void* device_dst,
  * device before switching; every query on an index grid without a channel returns
  * NVQ_ERR_INVALID_ARG. */
 int nvq_grid_set_channel(nvq_grid* grid, const float* channel, uint64_t count);
+/* How the ray queries (nvq_zero_crossing, nvq_iso_crossing, nvq_render_iso_surface, nvq_fog_march and
+ * its adjoint, nvq_multi_query) map world rays into index space.  NVQ_RAY_MAP_F32 (the default):
+ * Ray::worldToIndexF (math/Ray.h:258-268), the FP32 Map.  NVQ_RAY_MAP_F64: Ray::applyInverseMap
+ * (math/Ray.h:223-230), the FP64 Map (NanoVDB.h:1507-1510, :1530), for eyes far from the origin of a
+ * grid with a large translation or a non-power-of-two voxel size, where the FP32 map loses the
+ * voxel.  The traversal stays FP32.  Waits for the device before switching. */
+enum nvq_ray_map { NVQ_RAY_MAP_F32 = 0, NVQ_RAY_MAP_F64 = 1 };
+int nvq_grid_set_ray_map(nvq_grid* grid, int mode);
 /* Free the grid handle and, for nvq_grid_upload, the device copy (waits for the device). */
 int nvq_grid_free(nvq_grid* grid);
 int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info);
+/* Copy the grid's kernel-argument view (nvq::GridView of nvq_device_impl.h: device pointers to
+ * the root, the first leaf and, for index grids, the channel set at the time of the call, plus
+ * the Map and root bbox) into host memory `host_view` of exactly `view_bytes` = sizeof(GridView),
+ * else NVQ_ERR_INVALID_ARG.  Client kernels take it through nvq::getDeviceGrid (nvq_device.h), the
+ * counterpart of GridHandle::deviceGrid<T>() (GridHandle.h:159-170); it stays valid until the grid
+ * is freed. */
+int nvq_grid_get_view(const nvq_grid* grid, void* host_view, uint64_t view_bytes);
 
 /* Batched ReadAccessor::getValue / isActive / getDim (NanoVDB.h:5370-5377, 5431-5447)
  * for n integer voxels ijk[n][3].  Any of val / active / dim may be NULL.

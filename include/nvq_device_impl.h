@@ -49,6 +49,8 @@ struct GridView {
     double voxelSize[3];
     uint32_t leafCount;
     uint32_t leafSize;    // 0 for FpN (variable)
+    uint32_t rayMapF64;   // ray queries map world rays with the FP64 Ray::applyInverseMap
+    uint32_t pad;
 };
 
 __device__ __forceinline__ uint64_t ld_u64(const uint8_t* p) { return *reinterpret_cast<const uint64_t*>(p); }
@@ -511,20 +513,35 @@ __device__ __forceinline__ Ray makeRay(const float e[3], const float d[3], float
     return r;
 }
 
-// Ray::worldToIndexF: Map::applyInverseMapF / applyInverseJacobianF (fmaf form, math/Math.h:1704-1709)
+// World ray -> index ray.  Ray::worldToIndexF (math/Ray.h:258-268): Map::applyInverseMapF /
+// applyInverseJacobianF (fmaf form, math/Math.h:1704-1709), t1 left at FLT_MAX.  With
+// g.rayMapF64, Ray::applyInverseMap (math/Ray.h:223-230): the FP64 Map::applyInverseMap /
+// applyInverseJacobian (float(xyz - vecD), then a double fma chain rounded to float,
+// NanoVDB.h:1507-1510, math/Math.h:1718-1723) and both times scaled by the length.
 __device__ __forceinline__ Ray worldToIndexRay(const GridView& g, const float e[3], const float d[3])
 {
-    const float t[3] = {e[0] - g.vecF[0], e[1] - g.vecF[1], e[2] - g.vecF[2]};
     float eye[3], dir[3];
+    if (g.rayMapF64) {
+        const float t[3] = {float(double(e[0]) - g.vecD[0]), float(double(e[1]) - g.vecD[1]),
+                            float(double(e[2]) - g.vecD[2])};
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        eye[r] = __builtin_fmaf(t[0], g.invMatF[3 * r], __builtin_fmaf(t[1], g.invMatF[3 * r + 1], t[2] * g.invMatF[3 * r + 2]));
-        dir[r] = __builtin_fmaf(d[0], g.invMatF[3 * r], __builtin_fmaf(d[1], g.invMatF[3 * r + 1], d[2] * g.invMatF[3 * r + 2]));
+        for (int r = 0; r < 3; ++r) {
+            const double* m = g.invMatD + 3 * r;
+            eye[r] = float(__builtin_fma(double(t[0]), m[0], __builtin_fma(double(t[1]), m[1], double(t[2]) * m[2])));
+            dir[r] = float(__builtin_fma(double(d[0]), m[0], __builtin_fma(double(d[1]), m[1], double(d[2]) * m[2])));
+        }
+    } else {
+        const float t[3] = {e[0] - g.vecF[0], e[1] - g.vecF[1], e[2] - g.vecF[2]};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            eye[r] = __builtin_fmaf(t[0], g.invMatF[3 * r], __builtin_fmaf(t[1], g.invMatF[3 * r + 1], t[2] * g.invMatF[3 * r + 2]));
+            dir[r] = __builtin_fmaf(d[0], g.invMatF[3 * r], __builtin_fmaf(d[1], g.invMatF[3 * r + 1], d[2] * g.invMatF[3 * r + 2]));
+        }
     }
     const float length = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
     const float invLength = 1.0f / length;
     const float nd[3] = {invLength * dir[0], invLength * dir[1], invLength * dir[2]};
-    return makeRay(eye, nd, length * 1.0e-5f, kFltMax);
+    return makeRay(eye, nd, length * 1.0e-5f, g.rayMapF64 ? length * kFltMax : kFltMax);
 }
 
 // Ray::clip(CoordBBox) with the +1 max padding
@@ -640,6 +657,49 @@ struct HDDA {
         return advance(nz, dz, vz, sz);
     }
 };
+
+// ------------------------------------------------------------------ isoCrossing / zeroCrossing
+// math/HDDA.h:225-248 (isoCrossing) over the device accessor on an index-space ray.  zeroCrossing
+// (math/HDDA.h:189-214) is the same loop with iso = 0: value - 0.0f is the value, bit for bit.
+// Returns the first voxel whose value - iso changes sign against the value at the entry voxel.
+template<uint32_t GT>
+__device__ __forceinline__ bool isoCrossingCore(Accessor<GT>& acc, const GridView& g, Ray& ray, float iso,
+                                                int32_t c[3], float& v, float& t)
+{
+    if (!rayClip(ray, g.bbox) || double(ray.t1) > 1e20) return false;
+    const float Delta = 1.0001f;
+    float p[3];
+    rayAt(ray, ray.t0, p);
+    c[0] = ifloor(p[0]);
+    c[1] = ifloor(p[1]);
+    c[2] = ifloor(p[2]);
+    const Probe p0 = acc.probe(c[0], c[1], c[2]);
+    HDDA h;
+    h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
+    const float v0 = p0.value - iso;
+    while (h.stepOnce()) {
+        rayAt(ray, h.t0 + Delta, p);
+        c[0] = ifloor(p[0]);
+        c[1] = ifloor(p[1]);
+        c[2] = ifloor(p[2]);
+        const Probe q = acc.probe(c[0], c[1], c[2]);
+        h.update(ray, int32_t(q.dim));
+        if (h.dim > 1 || !q.active) continue;
+        while (h.stepOnce()) { // in the narrow band
+            const Probe r = acc.probe(h.vx, h.vy, h.vz);
+            if (!r.active) break;
+            v = r.value - iso;
+            if (v * v0 < 0) {
+                c[0] = h.vx;
+                c[1] = h.vy;
+                c[2] = h.vz;
+                t = h.t0;
+                return true;
+            }
+        }
+    }
+    return false;
+}
 
 // ------------------------------------------------------------------ fog-march step structure
 // HDDA over cells of size >= 8 (the leaf level, as TreeMarcher, math/HDDA.h:450-536, with

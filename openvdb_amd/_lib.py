@@ -53,6 +53,8 @@ SIGNATURES = {
     "nvq_grid_free": ([_vp], _i),
     "nvq_grid_set_channel": ([_vp, _vp, _u64], _i),
     "nvq_grid_get_info": ([_vp, ctypes.POINTER(GridInfo)], _i),
+    "nvq_grid_get_view": ([_vp, _vp, _u64], _i),
+    "nvq_grid_set_ray_map": ([_vp, _i], _i),
     "nvq_get_value": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp], _i),
     "nvq_sample": ([_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _vp], _i),
     "nvq_grad_stencil": ([_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp], _i),

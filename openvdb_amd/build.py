@@ -59,7 +59,24 @@ def build(verbose=False, incremental=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs + ["-pthread"], verbose)
     os.replace(tmp, LIB)
+    build_api_test(verbose)
     return LIB
+
+
+# Client kernels of the device-API tests: written against include/nvq_device.h alone (plus the
+# C ABI of libnvq.so for nvq_grid_get_view), as a NanoVDB user's own kernels would be.
+API_TEST_SRC = os.path.join(ROOT, "tests", "device_api", "api_kernels.hip")
+API_TEST_LIB = os.path.join(ROOT, "tests", "device_api", "libnvq_api_test.so")
+
+
+def build_api_test(verbose=False):
+    if not os.path.exists(API_TEST_SRC):
+        return None
+    tmp = API_TEST_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", API_TEST_SRC, "-o", tmp, "-L", HERE, "-l:libnvq.so",
+                                "-Wl,-rpath,$ORIGIN/../../openvdb_amd"], verbose)
+    os.replace(tmp, API_TEST_LIB)
+    return API_TEST_LIB
 
 
 if __name__ == "__main__":

@@ -465,10 +465,28 @@ int nvq_grid_set_channel(nvq_grid* grid, const float* channel, uint64_t count)
     return NVQ_OK;
 }
 
+int nvq_grid_set_ray_map(nvq_grid* grid, int mode)
+{
+    if (!grid || (mode != NVQ_RAY_MAP_F32 && mode != NVQ_RAY_MAP_F64)) return NVQ_ERR_INVALID_ARG;
+    if (grid->view.rayMapF64 == uint32_t(mode)) return NVQ_OK;
+    // queued work may still read the current view: wait for the device, then swap
+    if (hipDeviceSynchronize() != hipSuccess) return NVQ_ERR_HIP;
+    grid->view.rayMapF64 = uint32_t(mode);
+    if (hipMemcpy(grid->dview, &grid->view, sizeof(GridView), hipMemcpyHostToDevice) != hipSuccess) return NVQ_ERR_HIP;
+    return NVQ_OK;
+}
+
 int nvq_grid_get_info(const nvq_grid* grid, nvq_grid_info* info)
 {
     if (!grid || !info) return NVQ_ERR_INVALID_ARG;
     *info = grid->info;
+    return NVQ_OK;
+}
+
+int nvq_grid_get_view(const nvq_grid* grid, void* host_view, uint64_t view_bytes)
+{
+    if (!grid || !host_view || view_bytes != sizeof(GridView)) return NVQ_ERR_INVALID_ARG;
+    std::memcpy(host_view, &grid->view, sizeof(GridView));
     return NVQ_OK;
 }
 

@@ -4,49 +4,6 @@
 
 namespace nvq {
 
-// ------------------------------------------------------------------ isoCrossing / zeroCrossing
-// math/HDDA.h:225-248 (isoCrossing) over the device accessor on an index-space ray.  zeroCrossing
-// (math/HDDA.h:189-214) is the same loop with iso = 0: value - 0.0f is the value, bit for bit.
-// Returns the first voxel whose value - iso changes sign against the value at the entry voxel.
-template<uint32_t GT>
-__device__ __forceinline__ bool isoCrossingCore(Accessor<GT>& acc, const GridView& g, Ray& ray, float iso,
-                                                int32_t c[3], float& v, float& t)
-{
-    if (!rayClip(ray, g.bbox) || double(ray.t1) > 1e20) return false;
-    const float Delta = 1.0001f;
-    float p[3];
-    rayAt(ray, ray.t0, p);
-    c[0] = ifloor(p[0]);
-    c[1] = ifloor(p[1]);
-    c[2] = ifloor(p[2]);
-    const Probe p0 = acc.probe(c[0], c[1], c[2]);
-    HDDA h;
-    h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
-    const float v0 = p0.value - iso;
-    while (h.stepOnce()) {
-        rayAt(ray, h.t0 + Delta, p);
-        c[0] = ifloor(p[0]);
-        c[1] = ifloor(p[1]);
-        c[2] = ifloor(p[2]);
-        const Probe q = acc.probe(c[0], c[1], c[2]);
-        h.update(ray, int32_t(q.dim));
-        if (h.dim > 1 || !q.active) continue;
-        while (h.stepOnce()) { // in the narrow band
-            const Probe r = acc.probe(h.vx, h.vy, h.vz);
-            if (!r.active) break;
-            v = r.value - iso;
-            if (v * v0 < 0) {
-                c[0] = h.vx;
-                c[1] = h.vy;
-                c[2] = h.vz;
-                t = h.t0;
-                return true;
-            }
-        }
-    }
-    return false;
-}
-
 // Batched ray query: Ray::worldToIndexF (math/Ray.h:258-268), isoCrossing, and the normal from
 // the trilinear gradient at ray(t), Grid::indexToWorldGradF (Map::applyIJTF,
 // math/Math.h:1758-1763), normalised.  v is the value at the hit voxel minus iso.

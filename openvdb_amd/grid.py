@@ -237,6 +237,15 @@ class DeviceGrid:
               "nvq_grid_set_channel")
         self._channel = channel
 
+    def set_ray_map(self, mode):
+        """How ray queries map world rays to index space: "f32" (Ray::worldToIndexF, the default)
+        or "f64" (Ray::applyInverseMap, the FP64 Map, for eyes far from the origin of grids with a
+        large translation or a non-power-of-two voxel size)."""
+        modes = {"f32": 0, "f64": 1}
+        if mode not in modes:
+            raise ValueError("ray map must be 'f32' or 'f64'")
+        check(lib().nvq_grid_set_ray_map(self.handle, modes[mode]), "nvq_grid_set_ray_map")
+
     @property
     def device(self):
         """The torch device whose memory holds the grid; queries take tensors on this device."""

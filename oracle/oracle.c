@@ -193,6 +193,8 @@ typedef struct {
     const uint8_t* leaf; /* leaf node holding the voxel, or NULL */
     uint32_t n;          /* voxel offset in that leaf */
     uint64_t index;      /* index grids: the channel index the value comes from */
+    const uint8_t* lower; /* lower / upper node the lookup passed through, or NULL */
+    const uint8_t* upper;
 } Look;
 
 static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
@@ -201,6 +203,8 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
     r.leaf = NULL;
     r.n = 0;
     r.index = 0;
+    r.lower = NULL;
+    r.upper = NULL;
     const int idx = g->type == GT_ONINDEX;
     const uint64_t key = ((uint64_t)((uint32_t)z >> 12)) | ((uint64_t)((uint32_t)y >> 12) << 21) |
                          ((uint64_t)((uint32_t)x >> 12) << 42);
@@ -225,6 +229,7 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
         return r;
     }
     const uint8_t* up = g->root + child;
+    r.upper = up;
     const int upSkip = (int)(rd_u64(up + UP_FLAGS) & 1u);
     const uint32_t nu = ((((uint32_t)x & 4095u) >> 7) << 10) | ((((uint32_t)y & 4095u) >> 7) << 5) | (((uint32_t)z & 4095u) >> 7);
     if (!mask_on(up + UP_CMASK, nu)) {
@@ -235,6 +240,7 @@ static Look lookup(const G* g, int32_t x, int32_t y, int32_t z)
         return r;
     }
     const uint8_t* lo = up + rd_i64(up + UP_TABLE + 8 * nu);
+    r.lower = lo;
     const int loSkip = (int)(rd_u64(lo + LO_FLAGS) & 1u);
     const uint32_t nl = ((((uint32_t)x & 127u) >> 3) << 8) | ((((uint32_t)y & 127u) >> 3) << 4) | (((uint32_t)z & 127u) >> 3);
     if (!mask_on(lo + LO_CMASK, nl)) {
@@ -501,6 +507,26 @@ static Ray world_to_index_ray(const G* g, const float e[3], const float d[3])
     return make_ray(eye, nd, length * t0, t1);
 }
 
+/* Ray::applyInverseMap(map), math/Ray.h:223-230: eye through the FP64 Map::applyInverseMap
+ * (translation subtracted in FP64 and rounded to float, then the double fma chain rounded to
+ * float, NanoVDB.h:1507-1510) and the direction through the FP64 applyInverseJacobian (:1530);
+ * the direction is normalised in FP32 and both times are scaled by the length (t1 = FLT_MAX times
+ * a length above 1 is inf, as in the reference). */
+static Ray world_to_index_ray_f64(const G* g, const float e[3], const float d[3])
+{
+    const uint8_t* m = g->buf + GRID_MAP;
+    double M[9];
+    for (int a = 0; a < 9; ++a) M[a] = rd_f64(m + MAP_INVMATD + 8 * a);
+    float eye[3], dir[3];
+    world_to_index_f64map(g, e, eye);
+    for (int r = 0; r < 3; ++r)
+        dir[r] = (float)fma((double)d[0], M[3 * r], fma((double)d[1], M[3 * r + 1], (double)d[2] * M[3 * r + 2]));
+    const float length = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const float invLength = 1.0f / length;
+    const float nd[3] = {invLength * dir[0], invLength * dir[1], invLength * dir[2]};
+    return make_ray(eye, nd, length * 1.0e-5f, length * FLT_MAX);
+}
+
 static int ray_clip(Ray* r, const int32_t* bbox)
 {
     float t0 = r->t0, t1 = r->t1;
@@ -666,12 +692,19 @@ void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
 void or_iso_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, uint8_t* hit,
                      int32_t* ijk, float* t, float* v, float* normal)
 {
+    or_iso_crossing_map(buf, eye, dir, n, iso, 0, hit, ijk, t, v, normal);
+}
+
+void or_iso_crossing_map(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, int f64map,
+                         uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal)
+{
     const G g = open_grid(buf);
     const uint8_t* m = g.buf + GRID_MAP;
     float IM[9];
     for (int a = 0; a < 9; ++a) IM[a] = rd_f32(m + MAP_INVMATF + 4 * a);
     for (uint64_t i = 0; i < n; ++i) {
-        Ray r = world_to_index_ray(&g, eye + 3 * i, dir + 3 * i);
+        Ray r = f64map ? world_to_index_ray_f64(&g, eye + 3 * i, dir + 3 * i)
+                       : world_to_index_ray(&g, eye + 3 * i, dir + 3 * i);
         int32_t c[3] = {0, 0, 0};
         float vv = 0.0f, tt = 0.0f;
         const int h = iso_crossing(&g, &r, iso, c, &vv, &tt);
@@ -884,4 +917,250 @@ void or_fog_march_adjoint(const void* buf, const float* eye, const float* dir, u
     free(m.a);
     free(m.Tk);
     free(m.st);
+}
+
+/* ------------------------------------------------------------------ node spans and first hits
+ * TreeMarcher<NodeT, RayT, AccT>::init / step, math/HDDA.h:450-536.  level 0 = LeafNode (edge 8,
+ * children 1), 1 = lower InternalNode (128, 8), 2 = upper InternalNode (4096, 128).  The HDDA
+ * steps at max(getDim, edge); a span starts where the lookup at the span's start lands in a
+ * child of NodeT (getDim <= child edge), its node is the NodeT that lookup passed through
+ * (ReadAccessor::getNode<NodeT>, the node cached by that getDim), and it ends at the next HDDA
+ * crossing (clamped to the clipped t1).  Spans with t0 >= t1 are skipped. */
+static const uint8_t* look_node(const Look* l, int level)
+{
+    return level == 0 ? l->leaf : (level == 1 ? l->lower : l->upper);
+}
+
+static uint32_t tree_march_ray(const G* g, const float* e, const float* d, int level, uint32_t max_spans, float* T0,
+                               float* T1, int64_t* node)
+{
+    const uint32_t DIM = level == 0 ? 8u : (level == 1 ? 128u : 4096u);
+    const uint32_t CHILD = level == 0 ? 1u : (level == 1 ? 8u : 128u);
+    Ray R = world_to_index_ray(g, e, d);
+    if (!ray_clip(&R, g->bbox)) return 0;
+    const float Eps = 0.000001f, Delta = 0.01f;
+    const float t0 = R.t0 + Eps, t1 = R.t1 - Eps;
+    if (t0 > t1) return 0;
+    float p[3];
+    ray_at(&R, t0, p);
+    Look l = lookup(g, ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+    HDDA h;
+    hdda_init(&h, &R, t0, t1, (int)(l.dim > DIM ? l.dim : DIM));
+    float mT0 = l.dim <= CHILD ? h.t0 : -1.0f;
+    const uint8_t* cached = look_node(&l, level);
+    uint32_t count = 0;
+    for (;;) {
+        const float s0 = mT0;
+        const uint8_t* current = cached;
+        const int valid = hdda_step(&h);
+        ray_at(&R, h.t0 + Delta, p);
+        l = lookup(g, ifloor(p[0]), ifloor(p[1]), ifloor(p[2]));
+        cached = look_node(&l, level);
+        hdda_update(&h, &R, (int)(l.dim > DIM ? l.dim : DIM));
+        mT0 = l.dim <= CHILD ? h.t0 : -1.0f;
+        if (s0 >= 0.0f) {
+            const float s1 = fminf(t1, h.t0);
+            if (s0 < s1 && current) {
+                if (count < max_spans) {
+                    T0[count] = s0;
+                    T1[count] = s1;
+                    node[count] = (int64_t)(current - g->buf);
+                }
+                ++count;
+            }
+        }
+        if (!valid) break;
+    }
+    return count;
+}
+
+void or_tree_march(const void* buf, const float* eye, const float* dir, uint64_t n, int level, uint32_t max_spans,
+                   uint32_t* count, float* t0, float* t1, int64_t* node)
+{
+    const G g = open_grid(buf);
+    for (uint64_t i = 0; i < n; ++i)
+        count[i] = tree_march_ray(&g, eye + 3 * i, dir + 3 * i, level, max_spans, t0 + i * max_spans,
+                                  t1 + i * max_spans, node + i * max_spans);
+}
+
+/* firstActive(ray, acc, ijk, t), math/HDDA.h:429-444: clip to the root bbox, then HDDA steps with
+ * a 1.0001 nudge until the lookup at RoundDown(ray(t)) is active (a voxel or a tile). */
+static int first_active(const G* g, Ray* ray, int32_t ijk[3], float* t)
+{
+    if (!ray_clip(ray, g->bbox) || (double)ray->t1 > 1e20) return 0;
+    const float Delta = 1.0001f;
+    float p[3];
+    *t = ray->t0;
+    ray_at(ray, ray->t0, p);
+    for (int a = 0; a < 3; ++a) ijk[a] = ifloor(p[a]);
+    Look l = lookup(g, ijk[0], ijk[1], ijk[2]);
+    HDDA h;
+    hdda_init(&h, ray, ray->t0, ray->t1, (int)l.dim);
+    while (!l.active) {
+        if (!hdda_step(&h)) return 0;
+        *t = h.t0 + Delta;
+        ray_at(ray, *t, p);
+        for (int a = 0; a < 3; ++a) ijk[a] = ifloor(p[a]);
+        l = lookup(g, ijk[0], ijk[1], ijk[2]);
+        hdda_update(&h, ray, (int)l.dim);
+    }
+    return 1;
+}
+
+void or_first_active(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
+                     float* t)
+{
+    const G g = open_grid(buf);
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r = world_to_index_ray(&g, eye + 3 * i, dir + 3 * i);
+        int32_t c[3] = {0, 0, 0};
+        float tt = 0.0f;
+        const int h = first_active(&g, &r, c, &tt);
+        hit[i] = (uint8_t)h;
+        for (int a = 0; a < 3; ++a) ijk[3 * i + a] = h ? c[a] : 0;
+        t[i] = h ? tt : 0.0f;
+    }
+}
+
+/* DDA<RayT, Coord, Dim>, math/HDDA.h:258-379: voxel = RoundDown(ray(t0)) & ~(Dim-1); a step moves
+ * the voxel by +-Dim and the axis' next time by |1/dir| (one voxel's time, as the reference
+ * writes it), with the same forward-stepping fix as the HDDA. */
+void or_dda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,
+                  float* nexts, int32_t* voxels, uint8_t* ok)
+{
+    const Ray r = make_ray(eye, dir, t0, t1);
+    float T0 = r.t0, next[3], delta[3], pos[3];
+    int32_t voxel[3], step[3];
+    ray_at(&r, T0, pos);
+    for (int a = 0; a < 3; ++a) {
+        voxel[a] = ifloor(pos[a]) & ~(dim - 1);
+        delta[a] = 0.0f;
+        if (r.dir[a] == 0.0f) {
+            next[a] = FLT_MAX;
+            step[a] = 0;
+        } else if (r.inv[a] > 0.0f) {
+            step[a] = dim;
+            next[a] = T0 + ((float)(voxel[a] + dim) - pos[a]) * r.inv[a];
+            delta[a] = r.inv[a];
+        } else {
+            step[a] = -dim;
+            next[a] = T0 + ((float)voxel[a] - pos[a]) * r.inv[a];
+            delta[a] = -r.inv[a];
+        }
+    }
+    for (int s = 0; s <= nsteps; ++s) {
+        if (s == 0) {
+            ok[0] = 1;
+        } else {
+            const int a = min_index(next);
+            if (next[a] <= T0) next[a] += T0 - 0.999999f * next[a] + 1.0e-6f;
+            T0 = next[a];
+            next[a] += delta[a];
+            voxel[a] += step[a];
+            ok[s] = (uint8_t)(T0 <= r.t1);
+        }
+        times[s] = T0;
+        nexts[s] = fminf(r.t1, fminf(next[0], fminf(next[1], next[2])));
+        for (int a = 0; a < 3; ++a) voxels[3 * s + a] = voxel[a];
+    }
+}
+
+/* ------------------------------------------------------------------ BoxStencil, math/Stencils.h:306-432
+ * moveTo(Vec3f) fills the 8 corners of the cell at RoundDown(xyz) in the order 000, 001, 011, 010,
+ * 100, 101, 111, 110; interpolation nests z, y, x as A + (B - A)*w; gradient returns the partials
+ * mapped by the FP64 Map::applyIJT (matMultT(mInvMatD, g), math/Math.h:1766-1776) and rounded to
+ * float.  Float grids. */
+void or_box_stencil(const void* buf, const float* xyz, uint64_t n, float* interp, float* grad)
+{
+    const G g = open_grid(buf);
+    const uint8_t* m = g.buf + GRID_MAP;
+    double IM[9];
+    for (int a = 0; a < 9; ++a) IM[a] = rd_f64(m + MAP_INVMATD + 8 * a);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float* p = xyz + 3 * i;
+        const int32_t c[3] = {ifloor(p[0]), ifloor(p[1]), ifloor(p[2])};
+        static const int off[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0},
+                                      {1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}};
+        float V[8];
+        for (int k = 0; k < 8; ++k) V[k] = get_value(&g, c[0] + off[k][0], c[1] + off[k][1], c[2] + off[k][2]);
+        const float u = p[0] - (float)c[0], v = p[1] - (float)c[1], w = p[2] - (float)c[2];
+        if (interp) {
+            float A = V[0] + (V[1] - V[0]) * w;
+            float B = V[3] + (V[2] - V[3]) * w;
+            const float C = A + (B - A) * v;
+            A = V[4] + (V[5] - V[4]) * w;
+            B = V[7] + (V[6] - V[7]) * w;
+            const float D = A + (B - A) * v;
+            interp[i] = C + (D - C) * u;
+        }
+        if (grad) {
+            float D[4] = {V[1] - V[0], V[2] - V[3], V[5] - V[4], V[6] - V[7]};
+            float A = D[0] + (D[1] - D[0]) * v;
+            float B = D[2] + (D[3] - D[2]) * v;
+            float gi[3];
+            gi[2] = A + (B - A) * u;
+            D[0] = V[0] + D[0] * w;
+            D[1] = V[3] + D[1] * w;
+            D[2] = V[4] + D[2] * w;
+            D[3] = V[7] + D[3] * w;
+            A = D[0] + (D[1] - D[0]) * v;
+            B = D[2] + (D[3] - D[2]) * v;
+            gi[0] = B - A;
+            A = D[1] - D[0];
+            B = D[3] - D[2];
+            gi[1] = A + (B - A) * u;
+            for (int a = 0; a < 3; ++a)
+                grad[3 * i + a] =
+                    (float)fma((double)gi[0], IM[a], fma((double)gi[1], IM[3 + a], (double)gi[2] * IM[6 + a]));
+        }
+    }
+}
+
+/* GradStencil extras at integer voxels (math/Stencils.h:512-518, 544-554): the first-order upwind
+ * gradient along V[i] (V > 0 takes the backward difference) times 2*mInv2Dx, and the
+ * closest-point transform ijk - d*(central differences), d = float(v0 * 0.5 * mInvDx2).  Float
+ * grids. */
+void or_grad_stencil_ext(const void* buf, const int32_t* ijk, uint64_t n, const float* V, float* upwind, float* cpt)
+{
+    const G g = open_grid(buf);
+    const GradC gc = grad_consts(&g);
+    for (uint64_t i = 0; i < n; ++i) {
+        const int32_t a = ijk[3 * i], b = ijk[3 * i + 1], c = ijk[3 * i + 2];
+        float v[7];
+        v[0] = get_value(&g, a, b, c);
+        v[1] = get_value(&g, a - 1, b, c);
+        v[2] = get_value(&g, a + 1, b, c);
+        v[3] = get_value(&g, a, b - 1, c);
+        v[4] = get_value(&g, a, b + 1, c);
+        v[5] = get_value(&g, a, b, c - 1);
+        v[6] = get_value(&g, a, b, c + 1);
+        if (upwind) {
+            const float* d = V + 3 * i;
+            const float s[3] = {d[0] > 0 ? v[0] - v[1] : v[2] - v[0], d[1] > 0 ? v[0] - v[3] : v[4] - v[0],
+                                d[2] > 0 ? v[0] - v[5] : v[6] - v[0]};
+            for (int k = 0; k < 3; ++k) upwind[3 * i + k] = (s[k] * 2.0f) * gc.inv2dx;
+        }
+        if (cpt) {
+            const float d = (float)(v[0] * 0.5 * gc.invdx2);
+            cpt[3 * i] = (float)a - d * (v[2] - v[1]);
+            cpt[3 * i + 1] = (float)b - d * (v[4] - v[3]);
+            cpt[3 * i + 2] = (float)c - d * (v[6] - v[5]);
+        }
+    }
+}
+
+/* TrilinearSampler::zeroCrossing of the cell at Floor(xyz), math/SampleFromVoxels.h:274-288 */
+void or_sample_zero_crossing(const void* buf, const float* xyz, uint64_t n, uint8_t* zc)
+{
+    const G g = open_grid(buf);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float* p = xyz + 3 * i;
+        float s[2][2][2];
+        stencil8(&g, ifloor(p[0]), ifloor(p[1]), ifloor(p[2]), s);
+        const int less = s[0][0][0] < 0.0f;
+        zc[i] = (uint8_t)((less ^ (s[0][0][1] < 0.0f)) || (less ^ (s[0][1][1] < 0.0f)) ||
+                          (less ^ (s[0][1][0] < 0.0f)) || (less ^ (s[1][0][0] < 0.0f)) ||
+                          (less ^ (s[1][0][1] < 0.0f)) || (less ^ (s[1][1][1] < 0.0f)) ||
+                          (less ^ (s[1][1][0] < 0.0f)));
+    }
 }

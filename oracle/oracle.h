@@ -44,6 +44,9 @@ void or_iso_crossing(const void* buf, const float* eye, const float* dir, uint64
                      int32_t* ijk, float* t, float* v, float* normal);
 void or_iso_render(const void* buf, int32_t width, int32_t height, float iso, float dx, float wbbox_dim_z,
                    float tan_half_fov, const float origin[3], float* image, uint8_t* hit, float* t);
+/* the same with the ray mapped by the FP64 Ray::applyInverseMap when f64map != 0 */
+void or_iso_crossing_map(const void* buf, const float* eye, const float* dir, uint64_t n, float iso, int f64map,
+                         uint8_t* hit, int32_t* ijk, float* t, float* v, float* normal);
 void or_zero_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
                       float* t, float* v, float* normal);
 
@@ -83,6 +86,24 @@ int or_update_grid_stats_mode(void* buf, int mode);
 /* tools/cuda/SignedFloodFill.cuh signedFloodFill<float> restated, in place on a host float grid
  * buffer.  0 ok, -4 not a float grid, -3 a root tile the fill must write is missing. */
 int or_signed_flood_fill(void* buf);
+
+/* TreeMarcher<NodeT>::init/step for world rays (worldToIndexF): level 0 leaf, 1 lower, 2 upper
+ * nodes.  count[i] = the ray's number of spans; the first max_spans are written at
+ * [i*max_spans ...): t0, t1 and the node's byte offset from the start of the buffer. */
+void or_tree_march(const void* buf, const float* eye, const float* dir, uint64_t n, int level, uint32_t max_spans,
+                   uint32_t* count, float* t0, float* t1, int64_t* node);
+/* firstActive for world rays (worldToIndexF): hit, ijk, t (zeros on a miss) */
+void or_first_active(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
+                     float* t);
+/* DDA<Ray, Coord, dim> trace in the format of or_hdda_trace */
+void or_dda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,
+                  float* nexts, int32_t* voxels, uint8_t* ok);
+/* BoxStencil at index-space xyz: interpolation and the world-space (FP64 applyIJT) gradient */
+void or_box_stencil(const void* buf, const float* xyz, uint64_t n, float* interp, float* grad);
+/* GradStencil upwind gradient along V[n][3] and closest-point transform at integer voxels */
+void or_grad_stencil_ext(const void* buf, const int32_t* ijk, uint64_t n, const float* V, float* upwind, float* cpt);
+/* TrilinearSampler::zeroCrossing of the cell at Floor(xyz) */
+void or_sample_zero_crossing(const void* buf, const float* xyz, uint64_t n, uint8_t* zc);
 
 /* Known-answer helpers */
 void or_hdda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,

@@ -35,7 +35,81 @@ def aligned_empty(nbytes):
     return raw[off:off + nbytes]
 
 
-class Oracle:
+class _SpanQueries:
+    """Node spans, first hits, DDA traces and the stencil extras, bound to the same C signature in
+    liboracle.so (prefix "or_") and libnanoref.so (prefix "ref_")."""
+
+    def _bind_span_queries(self, prefix):
+        L = self.L
+        self._pre = prefix
+        f = lambda name: getattr(L, prefix + name)  # noqa: E731
+        f("tree_march").argtypes = [_vp, _vp, _vp, _u64, _i, ctypes.c_uint32, _vp, _vp, _vp, _vp]
+        f("first_active").argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp]
+        f("dda_trace").argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
+        f("box_stencil").argtypes = [_vp, _vp, _u64, _vp, _vp]
+        f("grad_stencil_ext").argtypes = [_vp, _vp, _u64, _vp, _vp, _vp]
+        f("sample_zero_crossing").argtypes = [_vp, _vp, _u64, _vp]
+
+    def _fn(self, name):
+        return getattr(self.L, self._pre + name)
+
+    def tree_march(self, buf, eye, dir, level, max_spans=256):
+        """TreeMarcher spans of world rays: (count[n], t0[n,max], t1[n,max], node byte offset[n,max])."""
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        cnt = np.zeros(n, np.uint32)
+        t0 = np.zeros((n, max_spans), np.float32)
+        t1 = np.zeros((n, max_spans), np.float32)
+        node = np.zeros((n, max_spans), np.int64)
+        self._fn("tree_march")(_p(buf), _p(eye), _p(dir), n, level, max_spans, _p(cnt), _p(t0), _p(t1), _p(node))
+        return cnt, t0, t1, node
+
+    def first_active(self, buf, eye, dir):
+        eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        n = eye.shape[0]
+        hit = np.empty(n, np.uint8)
+        ijk = np.empty((n, 3), np.int32)
+        t = np.empty(n, np.float32)
+        self._fn("first_active")(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t))
+        return hit, ijk, t
+
+    def dda_trace(self, eye, dir, t0, t1, dim, nsteps):
+        eye = np.asarray(eye, np.float32)
+        dir = np.asarray(dir, np.float32)
+        times = np.empty(nsteps + 1, np.float32)
+        nexts = np.empty(nsteps + 1, np.float32)
+        vox = np.empty((nsteps + 1, 3), np.int32)
+        ok = np.empty(nsteps + 1, np.uint8)
+        self._fn("dda_trace")(_p(eye), _p(dir), t0, t1, dim, nsteps, _p(times), _p(nexts), _p(vox), _p(ok))
+        return times, nexts, vox, ok
+
+    def box_stencil(self, buf, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        interp = np.empty(n, np.float32)
+        grad = np.empty((n, 3), np.float32)
+        self._fn("box_stencil")(_p(buf), _p(xyz), n, _p(interp), _p(grad))
+        return interp, grad
+
+    def grad_stencil_ext(self, buf, ijk, V):
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+        n = ijk.shape[0]
+        up = np.empty((n, 3), np.float32)
+        cpt = np.empty((n, 3), np.float32)
+        self._fn("grad_stencil_ext")(_p(buf), _p(ijk), n, _p(V), _p(up), _p(cpt))
+        return up, cpt
+
+    def sample_zero_crossing(self, buf, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        zc = np.empty(xyz.shape[0], np.uint8)
+        self._fn("sample_zero_crossing")(_p(buf), _p(xyz), xyz.shape[0], _p(zc))
+        return zc
+
+
+class Oracle(_SpanQueries):
     def __init__(self, path=ORACLE_SO):
         L = ctypes.CDLL(path)
         self.L = L
@@ -52,6 +126,7 @@ class Oracle:
         L.or_value_count.argtypes = [_vp]
         L.or_value_count.restype = _u64
         L.or_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _vp]
+        L.or_iso_crossing_map.argtypes = [_vp, _vp, _vp, _u64, _f, _i, _vp, _vp, _vp, _vp, _vp]
         L.or_iso_render.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f, _f, _f, _f, _vp, _vp, _vp, _vp]
         L.or_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
         L.or_fog_march_adjoint.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
@@ -69,6 +144,7 @@ class Oracle:
         L.or_points_to_grid_density.argtypes = [_vp, ctypes.c_int, _u64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_uint32, ctypes.c_char_p, ctypes.POINTER(_vp),
                                                 ctypes.POINTER(_u64), ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_uint32)]
+        self._bind_span_queries("or_")
 
     def validate(self, buf):
         return self.L.or_validate(_p(buf), buf.nbytes)
@@ -112,7 +188,8 @@ class Oracle:
         self.L.or_grad_stencil(_p(buf), _p(ijk), n, _p(g), _p(ns), _p(lap), _p(zc))
         return g, ns, lap, zc
 
-    def zero_crossing(self, buf, eye, dir):
+    def zero_crossing(self, buf, eye, dir, f64map=False):
+        """f64map: the index-space ray comes from the FP64 Ray::applyInverseMap instead of worldToIndexF."""
         eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
         dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
         n = eye.shape[0]
@@ -121,7 +198,8 @@ class Oracle:
         t = np.empty(n, np.float32)
         v = np.empty(n, np.float32)
         nrm = np.empty((n, 3), np.float32)
-        self.L.or_zero_crossing(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm))
+        self.L.or_iso_crossing_map(_p(buf), _p(eye), _p(dir), n, 0.0, int(f64map), _p(hit), _p(ijk), _p(t), _p(v),
+                                   _p(nrm))
         return hit, ijk, t, v, nrm
 
     def iso_crossing(self, buf, eye, dir, iso):
@@ -253,7 +331,7 @@ class Oracle:
         return bool(hit), float(out[0]), float(out[1])
 
 
-class Reference:
+class Reference(_SpanQueries):
     """The reference NanoVDB host code (compiled from the reference headers)."""
 
     FLOAT, FP4, FP8, FP16, FPN = 0, 1, 2, 3, 4
@@ -281,6 +359,7 @@ class Reference:
         L.ref_sample.argtypes = [_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _i]
         L.ref_grad_stencil.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _vp]
         L.ref_zero_crossing.argtypes = [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _i]
+        L.ref_zero_crossing_map.argtypes = [_vp, _vp, _vp, _u64, _i, _vp, _vp, _vp, _vp, _vp, _i]
         L.ref_iso_crossing.argtypes = [_vp, _vp, _vp, _u64, _f, _vp, _vp, _vp, _vp, _i]
         L.ref_write_nvdb.argtypes = [ctypes.c_char_p, _vp, _vp, _i, _i, _i]
         L.ref_index_grid.argtypes = [_vp, ctypes.c_uint32, _i, _i, ctypes.POINTER(_u64)]
@@ -297,6 +376,7 @@ class Reference:
         L.ref_index_values.argtypes = [_vp, _vp, _u64, _vp, _vp]
         L.ref_update_grid_stats.argtypes = [_vp, ctypes.c_int]
         L.ref_point_voxels.argtypes = [_vp, _vp, _u64, _vp, _vp]
+        self._bind_span_queries("ref_")
 
     def hw_threads(self):
         return int(self.L.ref_hw_threads())
@@ -373,7 +453,7 @@ class Reference:
         self.L.ref_grad_stencil(_p(buf), _p(ijk), n, _p(g), _p(ns), _p(lap), _p(zc))
         return g, ns, lap, zc
 
-    def zero_crossing(self, buf, eye, dir, nthreads=1):
+    def zero_crossing(self, buf, eye, dir, nthreads=1, f64map=False):
         eye = np.ascontiguousarray(eye, np.float32).reshape(-1, 3)
         dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
         n = eye.shape[0]
@@ -382,7 +462,8 @@ class Reference:
         t = np.empty(n, np.float32)
         v = np.empty(n, np.float32)
         nrm = np.empty((n, 3), np.float32)
-        self.L.ref_zero_crossing(_p(buf), _p(eye), _p(dir), n, _p(hit), _p(ijk), _p(t), _p(v), _p(nrm), nthreads)
+        self.L.ref_zero_crossing_map(_p(buf), _p(eye), _p(dir), n, int(f64map), _p(hit), _p(ijk), _p(t), _p(v),
+                                     _p(nrm), nthreads)
         return hit, ijk, t, v, nrm
 
     def iso_crossing(self, buf, eye, dir, iso, nthreads=1):

@@ -36,6 +36,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using namespace nanovdb;
@@ -340,8 +341,18 @@ int ref_grad_stencil(const void* buf, const int32_t* ijk, uint64_t n, float* gra
 
 // World-space rays -> worldToIndexF -> zeroCrossing; normal = world-space trilinear
 // gradient at the hit position ray(t), mapped with indexToWorldGradF and normalised.
+int ref_zero_crossing_map(const void* buf, const float* eye, const float* dir, uint64_t n, int f64map, uint8_t* hit,
+                          int32_t* ijk, float* t, float* v, float* normal, int nthreads);
 int ref_zero_crossing(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
                       float* t, float* v, float* normal, int nthreads)
+{
+    return ref_zero_crossing_map(buf, eye, dir, n, 0, hit, ijk, t, v, normal, nthreads);
+}
+
+// The same, with the index-space ray from Ray::applyInverseMap(grid->map()) (the FP64 map,
+// math/Ray.h:223-230) when f64map != 0.
+int ref_zero_crossing_map(const void* buf, const float* eye, const float* dir, uint64_t n, int f64map, uint8_t* hit,
+                          int32_t* ijk, float* t, float* v, float* normal, int nthreads)
 {
     return dispatch(buf, [&](auto* grid) {
         parallelFor(n, nthreads, [&](uint64_t b, uint64_t e) {
@@ -350,7 +361,7 @@ int ref_zero_crossing(const void* buf, const float* eye, const float* dir, uint6
             for (uint64_t i = b; i < e; ++i) {
                 math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
                                       Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
-                math::Ray<float> iRay = wRay.worldToIndexF(*grid);
+                math::Ray<float> iRay = f64map ? wRay.applyInverseMap(grid->map()) : wRay.worldToIndexF(*grid);
                 Coord c(0);
                 float vv = 0.0f, tt = 0.0f;
                 const bool h = math::zeroCrossing(iRay, acc, c, vv, tt);
@@ -604,6 +615,144 @@ int ref_ray_gen(int w, int h, float wBBoxDimZ, float cx, float cy, float cz, flo
         }
     }
     return 0;
+}
+
+// TreeMarcher<NodeT, Ray, Acc>::init/step (math/HDDA.h:450-536) for world rays mapped with
+// worldToIndexF: level 0 LeafNode, 1 lower, 2 upper.  count[i] spans; the first max_spans are
+// written at [i*max_spans ...) as t0, t1 and the node's byte offset from the buffer start.
+int ref_tree_march(const void* buf, const float* eye, const float* dir, uint64_t n, int level, uint32_t max_spans,
+                   uint32_t* count, float* t0, float* t1, int64_t* node)
+{
+    return dispatch(buf, [&](auto* grid) {
+        using GridT = std::remove_pointer_t<decltype(grid)>;
+        using BuildT = typename GridT::BuildType;
+        if constexpr (std::is_same_v<BuildT, ValueOnIndex>) {
+            return -4;
+        } else {
+            auto run = [&](auto tag) {
+                using NodeT = typename decltype(tag)::type;
+                for (uint64_t i = 0; i < n; ++i) {
+                    auto acc = grid->getAccessor();
+                    math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
+                                          Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
+                    math::TreeMarcher<NodeT, math::Ray<float>, decltype(acc)> marcher(acc);
+                    uint32_t c = 0;
+                    if (marcher.init(wRay.worldToIndexF(*grid))) {
+                        const NodeT* nd = nullptr;
+                        float a = 0.0f, b = 0.0f;
+                        while (marcher.step(&nd, a, b)) {
+                            if (c < max_spans) {
+                                t0[i * max_spans + c] = a;
+                                t1[i * max_spans + c] = b;
+                                node[i * max_spans + c] =
+                                    int64_t(reinterpret_cast<const uint8_t*>(nd) - reinterpret_cast<const uint8_t*>(buf));
+                            }
+                            ++c;
+                        }
+                    }
+                    count[i] = c;
+                }
+                return 0;
+            };
+            struct Leaf { using type = typename GridT::LeafNodeType; };
+            struct Lower { using type = NanoLower<BuildT>; };
+            struct Upper { using type = NanoUpper<BuildT>; };
+            if (level == 0) return run(Leaf{});
+            if (level == 1) return run(Lower{});
+            return run(Upper{});
+        }
+    });
+}
+
+// math::firstActive (math/HDDA.h:429-444) for world rays mapped with worldToIndexF; zeros on a miss
+int ref_first_active(const void* buf, const float* eye, const float* dir, uint64_t n, uint8_t* hit, int32_t* ijk,
+                     float* t)
+{
+    return dispatch(buf, [&](auto* grid) {
+        auto acc = makeAcc(grid);
+        for (uint64_t i = 0; i < n; ++i) {
+            math::Ray<float> wRay(Vec3f(eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]),
+                                  Vec3f(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
+            math::Ray<float> iRay = wRay.worldToIndexF(*grid);
+            Coord c(0);
+            float tt = 0.0f;
+            const bool h = math::firstActive(iRay, acc, c, tt);
+            hit[i] = h ? 1 : 0;
+            for (int a = 0; a < 3; ++a) ijk[3 * i + a] = h ? c[a] : 0;
+            t[i] = h ? tt : 0.0f;
+        }
+        return 0;
+    });
+}
+
+// DDA<Ray, Coord, Dim> known-answer traces (math/HDDA.h:258-379), format of ref_hdda_trace; dim 1, 8, 128
+int ref_dda_trace(const float* eye, const float* dir, float t0, float t1, int dim, int nsteps, float* times,
+                  float* nexts, int32_t* voxels, uint8_t* ok)
+{
+    math::Ray<float> ray(Vec3f(eye[0], eye[1], eye[2]), Vec3f(dir[0], dir[1], dir[2]), t0, t1);
+    auto run = [&](auto dda) {
+        for (int s = 0; s <= nsteps; ++s) {
+            if (s > 0) ok[s] = dda.step() ? 1 : 0;
+            else ok[0] = 1;
+            times[s] = dda.time();
+            nexts[s] = dda.next();
+            for (int a = 0; a < 3; ++a) voxels[3 * s + a] = dda.voxel()[a];
+        }
+        return 0;
+    };
+    if (dim == 1) return run(math::DDA<math::Ray<float>, Coord, 1>(ray));
+    if (dim == 8) return run(math::DDA<math::Ray<float>, Coord, 8>(ray));
+    if (dim == 128) return run(math::DDA<math::Ray<float>, Coord, 128>(ray));
+    return -1;
+}
+
+// BoxStencil<NanoGrid<float>> (math/Stencils.h:306-432): moveTo(RoundDown(xyz)), interpolation(xyz),
+// gradient(xyz).  BaseStencil::moveTo(const Vec3<RealT>&) (Stencils.h:146-152) does not compile
+// (it calls RoundDown without its CoordT argument), so the cell is moved to explicitly.
+int ref_box_stencil(const void* buf, const float* xyz, uint64_t n, float* interp, float* grad)
+{
+    const auto* grid = asGrid<float>(buf);
+    math::BoxStencil<NanoGrid<float>> st(*grid);
+    for (uint64_t i = 0; i < n; ++i) {
+        const Vec3f p(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        st.moveTo(math::RoundDown<Coord>(p));
+        if (interp) interp[i] = st.interpolation(p);
+        if (grad) {
+            const Vec3f g = st.gradient(p);
+            for (int a = 0; a < 3; ++a) grad[3 * i + a] = g[a];
+        }
+    }
+    return 0;
+}
+
+// GradStencil::gradient(V) (upwind) and cpt() at integer voxels (math/Stencils.h:512-518, 544-554)
+int ref_grad_stencil_ext(const void* buf, const int32_t* ijk, uint64_t n, const float* V, float* upwind, float* cpt)
+{
+    const auto* grid = asGrid<float>(buf);
+    math::GradStencil<NanoGrid<float>> st(*grid);
+    for (uint64_t i = 0; i < n; ++i) {
+        st.moveTo(Coord(ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]));
+        if (upwind) {
+            const Vec3f g = st.gradient(Vec3f(V[3 * i], V[3 * i + 1], V[3 * i + 2]));
+            for (int a = 0; a < 3; ++a) upwind[3 * i + a] = g[a];
+        }
+        if (cpt) {
+            const Vec3f c = st.cpt();
+            for (int a = 0; a < 3; ++a) cpt[3 * i + a] = c[a];
+        }
+    }
+    return 0;
+}
+
+// SampleFromVoxels<Acc,1,true>::zeroCrossing(xyz) (math/SampleFromVoxels.h:394-399)
+int ref_sample_zero_crossing(const void* buf, const float* xyz, uint64_t n, uint8_t* zc)
+{
+    return dispatch(buf, [&](auto* grid) {
+        auto acc = makeAcc(grid);
+        auto s = math::createSampler<1>(acc);
+        for (uint64_t i = 0; i < n; ++i) zc[i] = s.zeroCrossing(Vec3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2])) ? 1 : 0;
+        return 0;
+    });
 }
 
 // HDDA known-answer traces: init(ray(eye,dir,t0,t1), dim) then nsteps of step(),
