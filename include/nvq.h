@@ -201,6 +201,19 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
                       float dt, float sigma_t, const float* radiance, const float* transmittance,
                       const float* dL_dradiance, const float* dL_dtransmittance, int64_t* grad_accum, int fixed_shift,
                       void* stream);
+/* Adjoint of nvq_sample (no counterpart in the reference; it differentiates the forward of
+ * nanovdb.tools.cuda.sampleFromVoxels, python/cuda/PySampleFromVoxels.cu:16-52, and
+ * SampleFromVoxels<Acc,0|1>, math/SampleFromVoxels.h:149-156, 423-476): given the same points
+ * and dLoss/d(nearest)[n], dLoss/d(value)[n] and dLoss/d(gradient)[n][3] of grad_mode (each may
+ * be NULL), ADDS dLoss/dvalue into grad_accum with the layout, fixed point and range rules of
+ * nvq_fog_march_bwd.  Float grids: only leaf values receive gradients (tile and background values
+ * are constants); index grids: every channel entry a query reads.  Each query's contribution to
+ * one value is the FP32 sum of its terms (value, gradient, nearest, in that order) converted once
+ * to fixed point, so the result is bit-identical for every schedule and batch size.  Float and
+ * NVQ_ONINDEX grids only.  Waits for `stream`. */
+int nvq_sample_bwd(const nvq_grid* grid, const float* xyz, uint64_t n, int world, const float* dL_dnn,
+                   const float* dL_dval, const float* dL_dgrad, int grad_mode, int64_t* grad_accum, int fixed_shift,
+                   void* stream);
 /* out[i] = float(double(accum[i]) * 2^-fixed_shift) */
 int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream);
 

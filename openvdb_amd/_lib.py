@@ -65,6 +65,7 @@ SIGNATURES = {
     "nvq_multi_query": ([_vp, _u32, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march_bwd": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
     "nvq_grad_fixed_to_float": ([_vp, _u64, _i, _vp, _vp], _i),
+    "nvq_sample_bwd": ([_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _vp, _i, _vp], _i),
     "nvq_voxels_to_grid": ([_vp, _u64, _u32, _d, ctypes.c_char_p, _vp, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_voxels_to_grid_fp": ([_vp, _i, _u64, _u32, _d, ctypes.c_char_p, _vp, _vp, ctypes.POINTER(_vp)], _i),
     "nvq_points_to_grid": ([_vp, _u64, _d, _u32, ctypes.c_char_p, _vp, ctypes.POINTER(_vp)], _i),

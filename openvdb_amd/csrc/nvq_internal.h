@@ -65,6 +65,13 @@ inline uint32_t launchBlocks(const Sched& s) { return s.perXcd * 8u; }
 bool sampleBinnedSupported(const GridView& g, uint64_t n);
 int sampleBinned(const GridView& g, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                  int mode, hipStream_t s);
+// Adjoint of the point queries into an int64 fixed-point accumulator (binned for large batches,
+// lane per query otherwise; bit-identical).  0, -2 allocation, -3 type, -4 HIP error,
+// -6 a contribution saturated or was not finite.  Waits for s.
+int sampleBackward(const GridView& g, const float* xyz, uint64_t n, int world, const float* gnn, const float* gval,
+                   const float* ggrad, int mode, int64_t* accum, int shift, hipStream_t s);
+// NVQ_SAMPLE_DIRECT=1 in the environment forces the lane-per-query kernels (read once).
+bool forceDirectSampling();
 
 // Persistent multi-grid ray queries (nvq_multi.hip)
 constexpr uint32_t kMaxGrids = 32;

@@ -325,6 +325,12 @@ int validate(const uint8_t* b, uint64_t bytes)
 } // namespace
 
 // ======================================================================= C ABI
+bool nvq::forceDirectSampling()
+{
+    static const bool force = std::getenv("NVQ_SAMPLE_DIRECT") && std::getenv("NVQ_SAMPLE_DIRECT")[0] == '1';
+    return force;
+}
+
 extern "C" {
 
 const char* nvq_status_string(int status)
@@ -505,8 +511,7 @@ int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, fl
     if (n == 0) return NVQ_OK;
     // Large batches go through spatial binning + LDS bricks (bit-identical results);
     // NVQ_SAMPLE_DIRECT=1 forces the lane-per-query kernel.
-    static const bool forceDirect = std::getenv("NVQ_SAMPLE_DIRECT") && std::getenv("NVQ_SAMPLE_DIRECT")[0] == '1';
-    if (!forceDirect && sampleBinnedSupported(grid->view, n)) {
+    if (!forceDirectSampling() && sampleBinnedSupported(grid->view, n)) {
         const int st = sampleBinned(grid->view, xyz, n, world, nn, val, grad, grad_mode, S(stream));
         if (st == 0) return NVQ_OK;
         if (st == -2) return NVQ_ERR_HIP; // scratch allocation failed
@@ -590,6 +595,26 @@ int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, 
     const int st = fogMarchBackward(grid->view, eye, dir, n, image_width, dt, sigma_t, radiance, transmittance,
                                     dL_dradiance, dL_dtransmittance, grad_accum, fixed_shift, S(stream));
     return st == 0 ? NVQ_OK : (st == -2 ? NVQ_ERR_OVERFLOW : NVQ_ERR_HIP);
+}
+
+int nvq_sample_bwd(const nvq_grid* grid, const float* xyz, uint64_t n, int world, const float* dL_dnn,
+                   const float* dL_dval, const float* dL_dgrad, int grad_mode, int64_t* grad_accum, int fixed_shift,
+                   void* stream)
+{
+    if (!grid || !grad_accum || (n && !xyz) || fixed_shift < 0 || fixed_shift > 62 ||
+        (dL_dgrad && (grad_mode < 1 || grad_mode > 3)))
+        return NVQ_ERR_INVALID_ARG;
+    if (grid->view.type != GT_FLOAT && grid->view.type != GT_ONINDEX) return NVQ_ERR_UNSUPPORTED_TYPE;
+    if (!queryable(grid)) return NVQ_ERR_INVALID_ARG;
+    if (n == 0 || (!dL_dnn && !dL_dval && !dL_dgrad)) return NVQ_OK;
+    const int st = sampleBackward(grid->view, xyz, n, world, dL_dnn, dL_dval, dL_dgrad, grad_mode, grad_accum,
+                                  fixed_shift, S(stream));
+    switch (st) {
+    case 0: return NVQ_OK;
+    case -3: return NVQ_ERR_UNSUPPORTED_TYPE;
+    case -6: return NVQ_ERR_OVERFLOW;
+    default: return NVQ_ERR_HIP;
+    }
 }
 
 int nvq_grad_fixed_to_float(const int64_t* accum, uint64_t count, int fixed_shift, float* out, void* stream)

@@ -12,6 +12,8 @@ Each function mirrors one reference call shape, applied to a batch:
 * ``render_iso_surface`` the iso-surface example's RenderOp (examples/ex_raytrace_iso_surface/common.h)
 * ``fog_march``        emission-absorption march with HDDA node skipping
 * ``multi_query``      one persistent launch over many grids (zeroCrossing / fog march per grid)
+* ``sample_backward``  adjoint of ``sample`` (dLoss/dvalue, deterministic int64 fixed point);
+                       ``SampleFromVoxels`` / ``sample_differentiable`` wrap it for autograd
 
 All tensors must be CUDA tensors on the grid's device; work is queued on the
 current torch stream.
@@ -241,6 +243,56 @@ def fixed_to_float(accum, fixed_shift=40, out=None):
     check(lib().nvq_grad_fixed_to_float(_ptr(accum), accum.numel(), fixed_shift, _ptr(out), _stream()),
           "nvq_grad_fixed_to_float")
     return out
+
+
+def sample_backward(grid, xyz, grad_nn=None, grad_val=None, grad_grad=None, grad_mode=GRAD_FD6, world=False,
+                    accum=None, fixed_shift=40):
+    """Adjoint of ``sample``: adds dLoss/dvalue (int64 fixed point, 2^-fixed_shift) for the given
+    dLoss/d(nearest) [N], dLoss/d(value) [N] and dLoss/d(gradient) [N, 3] into ``accum``
+    [grid.grad_size] and returns it (allocated and zeroed when None).  Float grids receive
+    gradients on leaf values only; index grids on every channel entry a query reads."""
+    xyz = _points(xyz, torch.float32, "xyz", grid)
+    n = xyz.shape[0]
+    if accum is None:
+        accum = torch.zeros(grid.grad_size, dtype=torch.int64, device=xyz.device)
+    else:
+        _out(accum, grid, "accum", (grid.grad_size,), torch.int64)
+    gn = None if grad_nn is None else _per_ray(grad_nn, grid, "grad_nn", n)
+    gv = None if grad_val is None else _per_ray(grad_val, grid, "grad_val", n)
+    gg = None if grad_grad is None else _per_ray(grad_grad, grid, "grad_grad", 3 * n)
+    if gg is not None and grad_mode not in (GRAD_FD6, GRAD_STENCIL, GRAD_TRILINEAR):
+        raise ValueError("grad_grad needs grad_mode 1, 2 or 3")
+    check(lib().nvq_sample_bwd(grid.handle, _ptr(xyz), n, int(world), _ptr(gn), _ptr(gv), _ptr(gg),
+                               int(grad_mode or 0), _ptr(accum), fixed_shift, _stream()), "nvq_sample_bwd")
+    return accum
+
+
+class SampleFromVoxels(torch.autograd.Function):
+    """Differentiable point queries: (nearest, value, gradient) = sample(values) at fixed points.
+
+    ``values`` is the tensor the grid reads in place (``grid.leaf_values()`` for a Float grid, the
+    channel for an index grid); backward returns dLoss/dvalues of its shape, reduced
+    deterministically (int64 fixed point).  Points are not differentiated."""
+
+    @staticmethod
+    def forward(ctx, values, grid, xyz, world, grad_mode, fixed_shift):
+        nn, val, g = sample(grid, xyz, world=world, nn=True, val=True, grad_mode=grad_mode)
+        ctx.save_for_backward(xyz)
+        ctx.grid, ctx.world, ctx.mode, ctx.shift, ctx.shape = grid, world, grad_mode, fixed_shift, values.shape
+        return nn, val, g
+
+    @staticmethod
+    def backward(ctx, gnn, gval, ggrad):
+        (xyz,) = ctx.saved_tensors
+        accum = sample_backward(ctx.grid, xyz, gnn, gval, ggrad if ctx.mode else None, ctx.mode, ctx.world,
+                                fixed_shift=ctx.shift)
+        return fixed_to_float(accum, ctx.shift).view(ctx.shape), None, None, None, None, None
+
+
+def sample_differentiable(grid, xyz, world=False, grad_mode=GRAD_FD6, fixed_shift=40):
+    """``sample`` with autograd through the grid's values (``leaf_values()`` or the channel)."""
+    values = grid.channel_values() if grid.grid_type == "OnIndex" else grid.leaf_values()
+    return SampleFromVoxels.apply(values, grid, xyz, world, grad_mode, fixed_shift)
 
 
 class FogMarch(torch.autograd.Function):

@@ -417,6 +417,271 @@ void or_sample_f64(const void* buf, const float* xyz, uint64_t n, double* val, d
     }
 }
 
+/* ------------------------------------------------------------------ adjoint of the point queries
+ * The reference sampler has no backward: parity against the reference is unpinned by
+ * construction.  Two restatements of dLoss/dvalue of or_sample's outputs:
+ *  - or_sample_bwd_fixed: the FP32 / int64 fixed-point contract the device adjoint implements
+ *    (per query and voxel, the FP32 sum in term order of ((s*A)*B)*C, converted once with
+ *    round-to-nearest-even and saturation at +-2^62, integer-summed with wrap-around);
+ *  - or_sample_bwd_f64: the exact derivative in FP64 (weights from FP64 fractions, FD6 cells at
+ *    floor(p) +- 1), pinned by central differences of or_sample_loss_f64, an FP64 forward of the
+ *    same outputs with one value perturbed.
+ * Keys: Float grids leafIndex*512 + n for voxels in leaves (others receive nothing), index grids
+ * the channel index every lookup reads.  Forward semantics: PySampleFromVoxels.cu:42-47 (FD6),
+ * math/SampleFromVoxels.h:149-156 (nearest), :232-271 (trilinear gradient), :423-476
+ * (trilinear), math/Stencils.h:502-507 (GradStencil). */
+typedef struct {
+    int32_t o[3];
+    float s;
+    float A[2], B[2], C[2];
+    int single;
+} ATerm;
+
+static int64_t adj_key(const G* g, const Look* l)
+{
+    if (g->type == GT_ONINDEX) return (int64_t)l->index;
+    if (!l->leaf) return -1;
+    return (int64_t)((uint64_t)(l->leaf - g->leaf0) / g->leafSize) * 512 + l->n;
+}
+
+static void adj_cell(ATerm* t, int32_t x, int32_t y, int32_t z, float s, float a0, float a1, float b0, float b1,
+                     float c0, float c1)
+{
+    t->o[0] = x; t->o[1] = y; t->o[2] = z;
+    t->s = s;
+    t->A[0] = a0; t->A[1] = a1; t->B[0] = b0; t->B[1] = b1; t->C[0] = c0; t->C[1] = c1;
+    t->single = 0;
+}
+
+static void adj_single(ATerm* t, int32_t x, int32_t y, int32_t z, float s)
+{
+    t->o[0] = x; t->o[1] = y; t->o[2] = z;
+    t->s = s;
+    t->single = 1;
+}
+
+static int adj_terms(const float p[3], float gn, float gv, const float gg[3], int mode, const float fd[3],
+                     float gs_inv2dx, ATerm T[8])
+{
+    int nt = 0;
+    const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+    const int32_t i0 = (int32_t)fx, j0 = (int32_t)fy, k0 = (int32_t)fz;
+    const float u = p[0] - fx, v = p[1] - fy, w = p[2] - fz;
+    if (gv != 0.0f) adj_cell(&T[nt++], i0, j0, k0, gv, 1.0f - u, u, 1.0f - v, v, 1.0f - w, w);
+    if (mode == 1) {
+        for (int t = 0; t < 6; ++t) {
+            const int ax = t >> 1, plus = (t & 1) == 0;
+            float q[3];
+            for (int b = 0; b < 3; ++b) {
+                const float d = b == ax ? 1.0f : 0.0f;
+                q[b] = plus ? p[b] + d : p[b] - d;
+            }
+            const float s0 = gg[ax] * fd[ax];
+            const float s = plus ? s0 : -s0;
+            if (s == 0.0f) continue;
+            const float gx = floorf(q[0]), gy = floorf(q[1]), gz = floorf(q[2]);
+            const float a = q[0] - gx, b = q[1] - gy, c = q[2] - gz;
+            adj_cell(&T[nt++], (int32_t)gx, (int32_t)gy, (int32_t)gz, s, 1.0f - a, a, 1.0f - b, b, 1.0f - c, c);
+        }
+    } else if (mode == 2) {
+        for (int ax = 0; ax < 3; ++ax) {
+            const float s = gs_inv2dx * gg[ax];
+            if (s == 0.0f) continue;
+            const int32_t dx = ax == 0, dy = ax == 1, dz = ax == 2;
+            adj_single(&T[nt++], i0 + dx, j0 + dy, k0 + dz, s);
+            adj_single(&T[nt++], i0 - dx, j0 - dy, k0 - dz, -s);
+        }
+    } else if (mode == 3) {
+        if (gg[0] != 0.0f) adj_cell(&T[nt++], i0, j0, k0, gg[0], -1.0f, 1.0f, 1.0f - v, v, 1.0f - w, w);
+        if (gg[1] != 0.0f) adj_cell(&T[nt++], i0, j0, k0, gg[1], 1.0f - u, u, -1.0f, 1.0f, 1.0f - w, w);
+        if (gg[2] != 0.0f) adj_cell(&T[nt++], i0, j0, k0, gg[2], 1.0f - u, u, 1.0f - v, v, -1.0f, 1.0f);
+    }
+    if (gn != 0.0f) adj_single(&T[nt++], (int32_t)rintf(p[0]), (int32_t)rintf(p[1]), (int32_t)rintf(p[2]), gn);
+    return nt;
+}
+
+static inline int adj_corner_is(const ATerm* t, int c, int32_t x, int32_t y, int32_t z)
+{
+    return t->o[0] + (c >> 2) == x && t->o[1] + ((c >> 1) & 1) == y && t->o[2] + (c & 1) == z;
+}
+
+void or_sample_bwd_fixed(const void* buf, const float* xyz, uint64_t n, int world, const float* gnn,
+                         const float* gval, const float* ggrad, int grad_mode, int shift, int64_t* accum,
+                         int* overflow)
+{
+    const G g = open_grid(buf);
+    float fd[3];
+    for (int a = 0; a < 3; ++a) fd[a] = (float)(0.5 / rd_f64(g.buf + GRID_VOXELSIZE + 8 * a));
+    const float gs = grad_consts(&g).inv2dx;
+    const float scale = ldexpf(1.0f, shift);
+    const float lim = 4611686018427387904.0f; /* 2^62 */
+    int over = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        float p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        if (world) {
+            float q[3];
+            world_to_index_f64map(&g, p, q);
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+        }
+        const float gg[3] = {ggrad ? ggrad[3 * i] : 0.0f, ggrad ? ggrad[3 * i + 1] : 0.0f,
+                             ggrad ? ggrad[3 * i + 2] : 0.0f};
+        ATerm T[8];
+        const int nt = adj_terms(p, gnn ? gnn[i] : 0.0f, gval ? gval[i] : 0.0f, gg, ggrad ? grad_mode : 0, fd, gs, T);
+        for (int t = 0; t < nt; ++t) {
+            const int nc = T[t].single ? 1 : 8;
+            for (int c = 0; c < nc; ++c) {
+                const int32_t X = T[t].o[0] + (c >> 2), Y = T[t].o[1] + ((c >> 1) & 1), Z = T[t].o[2] + (c & 1);
+                int seen = 0;
+                for (int t2 = 0; t2 < t && !seen; ++t2)
+                    for (int c2 = 0; c2 < (T[t2].single ? 1 : 8); ++c2)
+                        if (adj_corner_is(&T[t2], c2, X, Y, Z)) { seen = 1; break; }
+                if (seen) continue;
+                float sum = 0.0f;
+                for (int t2 = t; t2 < nt; ++t2)
+                    for (int c2 = 0; c2 < (T[t2].single ? 1 : 8); ++c2)
+                        if (adj_corner_is(&T[t2], c2, X, Y, Z))
+                            sum += T[t2].single ? T[t2].s
+                                                : ((T[t2].s * T[t2].A[c2 >> 2]) * T[t2].B[(c2 >> 1) & 1]) * T[t2].C[c2 & 1];
+                const float x = sum * scale;
+                long long q;
+                if (fabsf(x) < lim) q = llrintf(x);
+                else {
+                    over = 1;
+                    q = (x != x) ? 0 : (x > 0.0f ? (1ll << 62) : -(1ll << 62));
+                }
+                if (q == 0) continue;
+                const Look l = lookup(&g, X, Y, Z);
+                const int64_t key = adj_key(&g, &l);
+                if (key >= 0) accum[key] = (int64_t)((uint64_t)accum[key] + (uint64_t)q);
+            }
+        }
+    }
+    if (overflow) *overflow = over;
+}
+
+/* FP64: the value a lookup returns, plus delta where its key is `key` */
+static double adj_value_f64(const G* g, int32_t x, int32_t y, int32_t z, int64_t key, double delta)
+{
+    const Look l = lookup(g, x, y, z);
+    double v = l.leaf ? leaf_value_f64(g, l.leaf, l.n) : (double)l.value;
+    if (key >= 0 && adj_key(g, &l) == key) v += delta;
+    return v;
+}
+
+static double trilinear_f64d(const G* g, double x, double y, double z, int64_t key, double delta, double grad[3])
+{
+    const double fx = floor(x), fy = floor(y), fz = floor(z);
+    const double u = x - fx, v = y - fy, w = z - fz;
+    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
+    double s[2][2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) s[a][b][c] = adj_value_f64(g, i + a, j + b, k + c, key, delta);
+    if (grad) {
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c) {
+                    const double wa = a ? u : 1.0 - u, wb = b ? v : 1.0 - v, wc = c ? w : 1.0 - w;
+                    gx += (a ? 1.0 : -1.0) * wb * wc * s[a][b][c];
+                    gy += wa * (b ? 1.0 : -1.0) * wc * s[a][b][c];
+                    gz += wa * wb * (c ? 1.0 : -1.0) * s[a][b][c];
+                }
+        grad[0] = gx; grad[1] = gy; grad[2] = gz;
+    }
+    double r = 0.0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) r += (a ? u : 1.0 - u) * (b ? v : 1.0 - v) * (c ? w : 1.0 - w) * s[a][b][c];
+    return r;
+}
+
+/* Loss = sum_i gnn_i nn_i + gval_i val_i + ggrad_i . grad_i with every output in FP64 and the
+ * value at `key` raised by delta (key -1: none).  Index-space points (no map). */
+double or_sample_loss_f64(const void* buf, const float* xyz, uint64_t n, const double* gnn, const double* gval,
+                          const double* ggrad, int grad_mode, int64_t key, double delta)
+{
+    const G g = open_grid(buf);
+    double fd[3];
+    for (int a = 0; a < 3; ++a) fd[a] = 0.5 / rd_f64(g.buf + GRID_VOXELSIZE + 8 * a);
+    const double gs = 0.5 / rd_f64(g.buf + GRID_VOXELSIZE);
+    double L = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        if (gnn) L += gnn[i] * adj_value_f64(&g, (int32_t)rint(p[0]), (int32_t)rint(p[1]), (int32_t)rint(p[2]), key, delta);
+        if (gval) L += gval[i] * trilinear_f64d(&g, p[0], p[1], p[2], key, delta, NULL);
+        if (!ggrad) continue;
+        double o[3] = {0.0, 0.0, 0.0};
+        if (grad_mode == 1) {
+            for (int ax = 0; ax < 3; ++ax) {
+                double qp[3] = {p[0], p[1], p[2]}, qm[3] = {p[0], p[1], p[2]};
+                qp[ax] += 1.0;
+                qm[ax] -= 1.0;
+                o[ax] = (trilinear_f64d(&g, qp[0], qp[1], qp[2], key, delta, NULL) -
+                         trilinear_f64d(&g, qm[0], qm[1], qm[2], key, delta, NULL)) * fd[ax];
+            }
+        } else if (grad_mode == 2) {
+            const int32_t a = (int32_t)floor(p[0]), b = (int32_t)floor(p[1]), c = (int32_t)floor(p[2]);
+            o[0] = gs * (adj_value_f64(&g, a + 1, b, c, key, delta) - adj_value_f64(&g, a - 1, b, c, key, delta));
+            o[1] = gs * (adj_value_f64(&g, a, b + 1, c, key, delta) - adj_value_f64(&g, a, b - 1, c, key, delta));
+            o[2] = gs * (adj_value_f64(&g, a, b, c + 1, key, delta) - adj_value_f64(&g, a, b, c - 1, key, delta));
+        } else if (grad_mode == 3) {
+            trilinear_f64d(&g, p[0], p[1], p[2], key, delta, o);
+        }
+        L += ggrad[3 * i] * o[0] + ggrad[3 * i + 1] * o[1] + ggrad[3 * i + 2] * o[2];
+    }
+    return L;
+}
+
+/* Exact FP64 derivative of or_sample_loss_f64 in every value, added into grad (keys as above). */
+void or_sample_bwd_f64(const void* buf, const float* xyz, uint64_t n, const double* gnn, const double* gval,
+                       const double* ggrad, int grad_mode, double* grad)
+{
+    const G g = open_grid(buf);
+    double fd[3];
+    for (int a = 0; a < 3; ++a) fd[a] = 0.5 / rd_f64(g.buf + GRID_VOXELSIZE + 8 * a);
+    const double gs = 0.5 / rd_f64(g.buf + GRID_VOXELSIZE);
+#define ADJ_ADD(X, Y, Z, W)                                                                                            \
+    do {                                                                                                               \
+        const Look l_ = lookup(&g, (X), (Y), (Z));                                                                     \
+        const int64_t k_ = adj_key(&g, &l_);                                                                           \
+        if (k_ >= 0) grad[k_] += (W);                                                                                  \
+    } while (0)
+    for (uint64_t i = 0; i < n; ++i) {
+        const double p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        const double f[3] = {floor(p[0]), floor(p[1]), floor(p[2])};
+        const int32_t c0[3] = {(int32_t)f[0], (int32_t)f[1], (int32_t)f[2]};
+        const double u[3] = {p[0] - f[0], p[1] - f[1], p[2] - f[2]};
+        if (gnn && gnn[i] != 0.0) ADJ_ADD((int32_t)rint(p[0]), (int32_t)rint(p[1]), (int32_t)rint(p[2]), gnn[i]);
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c) {
+                    const double wa = a ? u[0] : 1.0 - u[0], wb = b ? u[1] : 1.0 - u[1], wc = c ? u[2] : 1.0 - u[2];
+                    const int32_t X = c0[0] + a, Y = c0[1] + b, Z = c0[2] + c;
+                    if (gval && gval[i] != 0.0) ADJ_ADD(X, Y, Z, gval[i] * wa * wb * wc);
+                    if (!ggrad) continue;
+                    const double* d = ggrad + 3 * i;
+                    if (grad_mode == 3)
+                        ADJ_ADD(X, Y, Z, d[0] * (a ? 1.0 : -1.0) * wb * wc + d[1] * wa * (b ? 1.0 : -1.0) * wc +
+                                              d[2] * wa * wb * (c ? 1.0 : -1.0));
+                    if (grad_mode == 1)
+                        /* sample at p +- e_ax: cell c0 +- e_ax, same fractions */
+                        for (int ax = 0; ax < 3; ++ax) {
+                            const double wgt = d[ax] * fd[ax] * wa * wb * wc;
+                            ADJ_ADD(X + (ax == 0), Y + (ax == 1), Z + (ax == 2), wgt);
+                            ADJ_ADD(X - (ax == 0), Y - (ax == 1), Z - (ax == 2), -wgt);
+                        }
+                }
+        if (ggrad && grad_mode == 2)
+            for (int ax = 0; ax < 3; ++ax) {
+                const double s = gs * ggrad[3 * i + ax];
+                ADJ_ADD(c0[0] + (ax == 0), c0[1] + (ax == 1), c0[2] + (ax == 2), s);
+                ADJ_ADD(c0[0] - (ax == 0), c0[1] - (ax == 1), c0[2] - (ax == 2), -s);
+            }
+    }
+#undef ADJ_ADD
+}
+
 /* GradStencil, math/Stencils.h:455-576; GodunovsNormSqrd :62-80 */
 static inline float sqf(float x) { return x * x; }
 void or_grad_stencil(const void* buf, const int32_t* ijk, uint64_t n, float* grad, float* normsq, float* lap,

@@ -34,6 +34,20 @@ void or_sample(const void* buf, const float* xyz, uint64_t n, int world, float* 
  * the same FP32 positions; used to split arithmetic error from codec error. */
 void or_sample_f64(const void* buf, const float* xyz, uint64_t n, double* val, double* grad_fd6);
 
+/* Adjoint of or_sample (no reference implementation exists).  or_sample_bwd_fixed adds the
+ * FP32 / int64 fixed-point adjoint (2^-shift resolution) into accum and sets *overflow when a
+ * contribution saturated or was not finite; or_sample_bwd_f64 adds the exact FP64 derivative
+ * into grad; or_sample_loss_f64 is the FP64 forward loss sum(gnn*nn + gval*val + ggrad.grad)
+ * with the value whose key is `key` raised by delta (index-space points).  Keys: Float grids
+ * leafIndex*512 + n (leaf voxels only), index grids the channel index. */
+void or_sample_bwd_fixed(const void* buf, const float* xyz, uint64_t n, int world, const float* gnn,
+                         const float* gval, const float* ggrad, int grad_mode, int shift, int64_t* accum,
+                         int* overflow);
+void or_sample_bwd_f64(const void* buf, const float* xyz, uint64_t n, const double* gnn, const double* gval,
+                       const double* ggrad, int grad_mode, double* grad);
+double or_sample_loss_f64(const void* buf, const float* xyz, uint64_t n, const double* gnn, const double* gval,
+                          const double* ggrad, int grad_mode, int64_t key, double delta);
+
 /* GradStencil at integer voxels: gradient, normSqGrad (Godunov), laplacian, zeroCrossing */
 void or_grad_stencil(const void* buf, const int32_t* ijk, uint64_t n, float* grad, float* normsq, float* lap,
                      uint8_t* zc);

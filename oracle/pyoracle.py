@@ -130,6 +130,10 @@ class Oracle(_SpanQueries):
         L.or_iso_render.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f, _f, _f, _f, _vp, _vp, _vp, _vp]
         L.or_fog_march.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
         L.or_fog_march_adjoint.argtypes = [_vp, _vp, _vp, _u64, _f, _f, _vp, _vp, _vp]
+        L.or_sample_bwd_fixed.argtypes = [_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
+        L.or_sample_bwd_f64.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _i, _vp]
+        L.or_sample_loss_f64.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp, _i, ctypes.c_int64, _d]
+        L.or_sample_loss_f64.restype = _d
         L.or_hdda_trace.argtypes = [_vp, _vp, _f, _f, _i, _i, _vp, _vp, _vp, _vp]
         L.or_ray_clip.argtypes = [_vp, _vp, _f, _f, _vp, _vp]
         L.or_voxels_to_grid.argtypes = [_vp, _u64, ctypes.c_uint32, _d, ctypes.c_char_p, ctypes.POINTER(_vp),
@@ -245,6 +249,44 @@ class Oracle(_SpanQueries):
         grad = np.zeros(int(self.L.or_value_count(_p(buf))) if is_index else self.leaf_count(buf) * 512, np.float64)
         self.L.or_fog_march_adjoint(_p(buf), _p(eye), _p(dir), n, dt, sigma, _p(dL), _p(dT), _p(grad))
         return grad
+
+    def grad_size(self, buf):
+        """Length of an adjoint accumulator: value count (index grids) or leafCount*512."""
+        is_index = int(np.frombuffer(buf[636:640].tobytes(), np.uint32)[0]) == 20
+        return int(self.L.or_value_count(_p(buf))) if is_index else self.leaf_count(buf) * 512
+
+    def sample_bwd_fixed(self, buf, xyz, gnn=None, gval=None, ggrad=None, grad_mode=1, world=False, shift=40,
+                         accum=None):
+        """FP32 / int64 fixed-point adjoint of sample(): (accum int64[grad_size], overflow flag)."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        f = lambda a, k: None if a is None else np.ascontiguousarray(a, np.float32).reshape(n * k)
+        gnn, gval, ggrad = f(gnn, 1), f(gval, 1), f(ggrad, 3)
+        if accum is None:
+            accum = np.zeros(self.grad_size(buf), np.int64)
+        over = ctypes.c_int(0)
+        self.L.or_sample_bwd_fixed(_p(buf), _p(xyz), n, int(world), _p(gnn), _p(gval), _p(ggrad), int(grad_mode),
+                                   int(shift), _p(accum), ctypes.byref(over))
+        return accum, bool(over.value)
+
+    def sample_bwd_f64(self, buf, xyz, gnn=None, gval=None, ggrad=None, grad_mode=1):
+        """Exact FP64 adjoint of sample() (index-space points): float64[grad_size]."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        f = lambda a, k: None if a is None else np.ascontiguousarray(a, np.float64).reshape(n * k)
+        grad = np.zeros(self.grad_size(buf), np.float64)
+        self.L.or_sample_bwd_f64(_p(buf), _p(xyz), n, _p(f(gnn, 1)), _p(f(gval, 1)), _p(f(ggrad, 3)), int(grad_mode),
+                                 _p(grad))
+        return grad
+
+    def sample_loss_f64(self, buf, xyz, gnn=None, gval=None, ggrad=None, grad_mode=1, key=-1, delta=0.0):
+        """FP64 loss sum(gnn*nn + gval*val + ggrad.grad) with value `key` raised by delta."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        f = lambda a, k: None if a is None else np.ascontiguousarray(a, np.float64).reshape(n * k)
+        a, b, c = f(gnn, 1), f(gval, 1), f(ggrad, 3)
+        return float(self.L.or_sample_loss_f64(_p(buf), _p(xyz), n, _p(a), _p(b), _p(c), int(grad_mode), int(key),
+                                               float(delta)))
 
     def voxels_to_grid(self, ijk, grid_type=20, voxel_size=1.0, name=None, want_index=False):
         """voxelsToGrid<float | ValueOnIndex | ValueIndex> restated (grid_type 1, 20, 19): (buffer,
