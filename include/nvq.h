@@ -6,8 +6,8 @@
  * hipStream_t passed as void* (NULL = the default stream).  Every call is
  * stream-ordered and asynchronous.  Every call returns NVQ_OK (0) or a negative
  * NVQ_ERR_* status.  Nothing throws across the ABI.  Apart from the opt-in
- * kernel profiler (nvq_profile_*), the library holds no global mutable state:
- * calls on distinct streams are thread-safe.  The caller
+ * kernel profiler (nvq_profile_*) and the work-queue array of nvq_multi_query_noalloc,
+ * the library holds no global mutable state: calls on distinct streams are thread-safe.  The caller
  * owns every buffer and keeps the grid alive until work queued on it has
  * finished.
  *
@@ -183,6 +183,20 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
                     const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,
                     float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a, float* out_b, float* normal,
                     void* stream);
+
+/* nvq_multi_query for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments, limits and results (bit for bit) as nvq_multi_query; the only scratch, the work
+ * queue's counters, lives in a fixed 2 KiB array that the library holds on each device until the
+ * process ends, so the call allocates and frees nothing (no hipMalloc, no stream-ordered pool).
+ * The scratch does not grow with the batch: any launch nvq_multi_query accepts fits (fewer than
+ * 2^31 ray tiles, else NVQ_ERR_INVALID_ARG).  The counters are cleared by a kernel at the start of
+ * each call, so they keep the last call's contents in between; calls that overlap on one device
+ * share them and race: serialise them (one stream, or events).  Does not wait. */
+int nvq_multi_query_noalloc(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                            const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye,
+                            const float* dir, float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a,
+                            float* out_b, float* normal, void* stream);
 
 /* Adjoint of nvq_fog_march (no counterpart in the reference): dLoss/dvalue for every leaf value
  * of a Float grid, or every channel value of an index grid (NVQ_ONINDEX), given the forward

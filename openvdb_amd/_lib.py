@@ -63,6 +63,7 @@ SIGNATURES = {
     "nvq_render_iso_surface": ([_vp, ctypes.POINTER(IsoCamera), _i, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp], _i),
     "nvq_multi_query": ([_vp, _u32, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "nvq_multi_query_noalloc": ([_vp, _u32, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "nvq_fog_march_bwd": ([_vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
     "nvq_grad_fixed_to_float": ([_vp, _u64, _i, _vp, _vp], _i),
     "nvq_sample_bwd": ([_vp, _vp, _u64, _i, _vp, _vp, _vp, _i, _vp, _i, _vp], _i),

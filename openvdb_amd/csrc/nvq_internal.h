@@ -90,6 +90,9 @@ struct MultiArgs {
 };
 int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
                int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
+// The same with the queue counters in the library's per-device array: no allocation.
+int multiQueryNoAlloc(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
+                      int32_t* ijk, float* a, float* b, float* normal, hipStream_t s);
 // Argument checks and tile layout of a multi-grid query (nvq_kernels.hip); NVQ_OK or a status.
 int makeMultiArgs(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
                   const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye, const float* dir,

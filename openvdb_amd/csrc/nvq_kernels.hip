@@ -680,4 +680,16 @@ int nvq_multi_query(const nvq_grid* const* host_grids, uint32_t k, const uint64_
     return multiQuery(A, eye, dir, dt, sigma_t, hit, ijk, out_a, out_b, normal, S(stream)) == 0 ? NVQ_OK : NVQ_ERR_HIP;
 }
 
+int nvq_multi_query_noalloc(const nvq_grid* const* host_grids, uint32_t k, const uint64_t* host_ray_offsets,
+                            const uint32_t* host_image_width, const uint32_t* host_ops, const float* eye,
+                            const float* dir, float dt, float sigma_t, uint8_t* hit, int32_t* ijk, float* out_a,
+                            float* out_b, float* normal, void* stream)
+{
+    MultiArgs A;
+    const int st = makeMultiArgs(host_grids, k, host_ray_offsets, host_image_width, host_ops, eye, dir, dt, A);
+    if (st != NVQ_OK || A.totalTiles == 0) return st;
+    return multiQueryNoAlloc(A, eye, dir, dt, sigma_t, hit, ijk, out_a, out_b, normal, S(stream)) == 0 ? NVQ_OK
+                                                                                                        : NVQ_ERR_HIP;
+}
+
 } // extern "C"

@@ -12,6 +12,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
+// Work-queue counters of nvq_multi_query_noalloc (kQueueBands counters, kQueueStride words apart:
+// 2 KiB).  Each device holds its own copy until the process ends; host code reaches it only
+// through the device address hipGetSymbolAddress returns for that device.
+static __device__ __align__(256) uint32_t scratch[nvq::kQueueBands * nvq::kQueueStride];
+
 namespace nvq {
 namespace {
 
@@ -54,7 +61,48 @@ __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float
     });
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex gScratchLock;
+uint32_t* gScratch[kMaxDevices]; // device address of `scratch` on each device, set on first use
+
+// Device address of `scratch` on device `dev`, which must be current: looked up once per device.
+int scratchOn(int dev, uint32_t** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lock(gScratchLock);
+    if (!gScratch[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        gScratch[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = gScratch[dev];
+    return 0;
+}
+
+uint32_t queryBlocks(const MultiArgs& A, int cus)
+{
+    const uint64_t waves = A.totalTiles; // one wave per tile is the most that can be busy
+    uint32_t blocks = uint32_t(cus) * 8u;
+    if (uint64_t(blocks) * (kBlock / 64) > waves) blocks = uint32_t((waves + (kBlock / 64) - 1) / (kBlock / 64));
+    return blocks == 0 ? 1 : blocks;
+}
+
 } // namespace
+
+int multiQueryNoAlloc(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
+                      int32_t* ijk, float* a, float* b, float* normal, hipStream_t s)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    uint32_t* counter = nullptr;
+    if (scratchOn(dev, &counter) != 0) return -2;
+    // the counters keep the previous call's values: clear them with a kernel (no memset node)
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
+    hipLaunchKernelGGL(k_multi_query, dim3(queryBlocks(A, cus)), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter,
+                       hit, ijk, a, b, normal);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
 
 int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt, float sigma, uint8_t* hit,
                int32_t* ijk, float* a, float* b, float* normal, hipStream_t s)
@@ -65,13 +113,9 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     uint32_t* counter = nullptr;
     if (hipMallocAsync(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
-    const uint64_t waves = A.totalTiles; // one wave per tile is the most that can be busy
-    uint32_t blocks = uint32_t(cus) * 8u;
-    if (uint64_t(blocks) * (kBlock / 64) > waves) blocks = uint32_t((waves + (kBlock / 64) - 1) / (kBlock / 64));
-    if (blocks == 0) blocks = 1;
     ProfScope prof("k_multi_query", s);
-    hipLaunchKernelGGL(k_multi_query, dim3(blocks), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter, hit, ijk, a,
-                       b, normal);
+    hipLaunchKernelGGL(k_multi_query, dim3(queryBlocks(A, cus)), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter,
+                       hit, ijk, a, b, normal);
     const bool ok = hipGetLastError() == hipSuccess;
     if (hipFreeAsync(counter, s) != hipSuccess) return -3;
     return ok ? 0 : -4;

@@ -182,13 +182,14 @@ def fog_march(grid, eye, dir, dt=0.5, sigma_t=0.05, image_width=0):
 OP_ZERO_CROSSING, OP_FOG_MARCH = 0, 1
 
 
-def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.05, ops=None):
+def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.05, ops=None, noalloc=False):
     """One persistent launch over ``grids`` (<= 32, any codecs).  The rays of grid j are
     ``eye/dir[offsets[j]:offsets[j+1]]``; ``image_widths[j]`` (0 = linear batch) makes them a
     row-major image.  ``ops[j]`` is OP_ZERO_CROSSING or OP_FOG_MARCH (None: level sets run
     zeroCrossing, fog volumes the march; index grids need ``ops``).  Returns
     (hit, ijk, a, b, normal): zeroCrossing gives hit/ijk/t/v/normal, the march gives radiance in
-    ``a`` and transmittance in ``b``."""
+    ``a`` and transmittance in ``b``.  ``noalloc`` calls nvq_multi_query_noalloc, whose work
+    queue lives in the library's per-device array (the library allocates nothing)."""
     eye, dir = _rays(grids[0] if len(grids) else None, eye, dir)
     for g in grids[1:]:
         _on_grid(eye, g, "eye")
@@ -206,8 +207,9 @@ def multi_query(grids, eye, dir, offsets, image_widths=None, dt=0.5, sigma_t=0.0
     a = torch.zeros(n, dtype=torch.float32, device=dev)
     b = torch.zeros(n, dtype=torch.float32, device=dev)
     nrm = torch.zeros((n, 3), dtype=torch.float32, device=dev)
-    check(lib().nvq_multi_query(hs, k, offs, widths, opa, _ptr(eye), _ptr(dir), dt, sigma_t, _ptr(hit), _ptr(ijk),
-                                _ptr(a), _ptr(b), _ptr(nrm), _stream()), "nvq_multi_query")
+    fn = lib().nvq_multi_query_noalloc if noalloc else lib().nvq_multi_query
+    check(fn(hs, k, offs, widths, opa, _ptr(eye), _ptr(dir), dt, sigma_t, _ptr(hit), _ptr(ijk), _ptr(a), _ptr(b),
+             _ptr(nrm), _stream()), "nvq_multi_query_noalloc" if noalloc else "nvq_multi_query")
     return hit, ijk, a, b, nrm
 
 
