@@ -614,6 +614,26 @@ def touched_leaves(torch, nvq, grid, E, D, width, dt=0.5, sigma=0.05):
     return leaves
 
 
+def binned_count(torch, nvq, grid, pts, chunk=1 << 22):
+    """Queries of a batch that touch a leaf in the 8^3 slots of [cell-1, cell+2]^3: the ones the
+    binned path sorts and stages (the others are answered by its classify pass).  Counted with
+    getDim (1 inside a leaf) at the slot origins."""
+    m = 0
+    for s in range(0, pts.shape[0], chunk):
+        c = torch.floor(pts[s:s + chunk]).to(torch.int32)
+        lo, hi = torch.div(c - 1, 8, rounding_mode="floor"), torch.div(c + 2, 8, rounding_mode="floor")
+        hit = torch.zeros(c.shape[0], dtype=torch.bool, device=c.device)
+        for a in (0, 1):
+            for b in (0, 1):
+                for e in (0, 1):
+                    sl = torch.stack([hi[:, 0] if a else lo[:, 0], hi[:, 1] if b else lo[:, 1],
+                                      hi[:, 2] if e else lo[:, 2]], 1) * 8
+                    _, _, dim = nvq.ops.get_value(grid, sl.contiguous(), want_dim=True)
+                    hit |= dim == 1
+        m += int(hit.sum().item())
+    return m
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "sample_fd6_traffic.json")
     if not os.path.exists(path):
@@ -673,20 +693,45 @@ def main():
     ms_step = 1e3 * wall / args.steps
     value = 2 * n * args.steps / wall
     call_ms = [p / args.steps for p in per]
-    b_io = n * (12 + 4 + 4 + 12)
-    b_alg = [b_io + leaf_bytes(torus), b_io + leaf_bytes(fog)]
-    # dominant kernel of the binned query: k_bin_sample, one launch per grid per step
-    dom = "k_bin_sample" if n >= (1 << 20) and os.environ.get("NVQ_SAMPLE_DIRECT") != "1" else "k_sample"
-    dom_ms, dom_n = kernel_ms(dom)
-    dom_avg = dom_ms / max(dom_n, 1)
-    achieved = (b_alg[0] + b_alg[1]) * args.steps / (1e-3 * dom_ms) / 1e9 if dom_ms > 0 else 0.0
-    traffic = load_traffic()
-    log(f"C2: {ms_step:.3f} ms/step, calls {call_ms[0]:.3f} / {call_ms[1]:.3f} ms, {dom} {dom_avg:.3f} ms avg over "
-        f"{dom_n} launches, {value/1e9:.2f} G samples/s, {achieved:.0f} GB/s algorithmic in {dom}")
+    # Algorithmic bytes per query: 12 B point in, 4 B nearest + 4 B value + 12 B gradient out; per
+    # call also every leaf once (2^26 queries touch them all).  The binned path splits a call into
+    # k_bin_classify (all n queries; marks the m that touch a leaf), the radix sort of those m,
+    # k_bin_sample (those m, into records) and k_bin_finish (all n outputs: the m records copied,
+    # the others answered from the slot table).
+    leaves = [leaf_bytes(torus), leaf_bytes(fog)]
+    b_call = [n * 32 + leaves[0], n * 32 + leaves[1]]
+    binned = os.environ.get("NVQ_SAMPLE_DIRECT") != "1"
+    stages = {}
+    for k in (["k_bin_classify", "bin_sort", "k_bin_sample", "k_bin_finish"] if binned else ["k_sample"]):
+        ms, cnt = kernel_ms(k)
+        stages[k] = {"avg_ms": ms / max(cnt, 1), "launches": cnt}
+    m = [binned_count(torch, nvq, torus, p_torus), binned_count(torch, nvq, fog, p_fog)] if binned else [n, n]
+    per_launch = {  # algorithmic bytes one launch processes, mean of the two grids
+        "k_bin_classify": 12 * n,
+        "k_bin_sample": sum(32 * mi + lb for mi, lb in zip(m, leaves)) / 2,
+        "k_bin_finish": sum(20 * n + 12 * (n - mi) for mi in m) / 2,
+        "k_sample": sum(b_call) / 2,
+    }
+    dom = max((k for k in stages if k in per_launch), key=lambda k: stages[k]["avg_ms"])
+    dom_avg = stages[dom]["avg_ms"]
+    achieved = per_launch[dom] / (1e-3 * dom_avg) / 1e9 if dom_avg > 0 else 0.0
+    call_gbs = sum(b_call) / (1e-3 * sum(call_ms)) / 1e9
+    traffic = load_traffic() if dom == "k_bin_sample" else None
+    log(f"C2: {ms_step:.3f} ms/step, calls {call_ms[0]:.3f} / {call_ms[1]:.3f} ms, stages "
+        + ", ".join(f"{k} {v['avg_ms']:.3f}" for k, v in stages.items())
+        + f" ms; binned {m[0]/n:.2f} / {m[1]/n:.2f} of the queries; {value/1e9:.2f} G samples/s; {dom} "
+        f"{achieved:.0f} GB/s algorithmic; call {call_gbs:.0f} GB/s")
 
     extra = {"c2_call_ms": {"torus": round(call_ms[0], 4), "fog": round(call_ms[1], 4)},
-             "c2_dominant_kernel": {"name": dom, "avg_ms": dom_avg, "launches": dom_n},
-             "c2_alg_bytes_per_launch": {"torus": b_alg[0], "fog": b_alg[1]},
+             "c2_stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
+             "c2_binned_queries": {"torus": m[0], "fog": m[1]},
+             "c2_dominant_kernel": {"name": dom, "avg_ms": dom_avg, "launches": stages[dom]["launches"],
+                                    "alg_bytes_per_launch": int(per_launch[dom])},
+             "c2_alg_bytes_per_call": {"torus": b_call[0], "fog": b_call[1]},
+             "c2_call_roofline": {"bound": "hbm", "achieved": call_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": call_gbs / HBM_PEAK_GBS,
+                                  "bytes": "32 B per query (12 in, 20 out) + every leaf once, over the wall time "
+                                           "of the whole nvq_sample call (all its kernels)"},
              "c2_samples_per_s": {"torus": n / (1e-3 * call_ms[0]), "fog": n / (1e-3 * call_ms[1])},
              "fog_grid": {"bytes": int(fog_np.nbytes), "leaves": fog.leaf_count}}
 

@@ -1,25 +1,28 @@
-// Large-batch point sampling: spatial binning + LDS-staged voxel bricks.
+// Large-batch point sampling: spatial binning + LDS-staged voxel bricks, and its adjoint.
 //
-// Random query points touch a dozen distinct cache lines each over a grid that
-// does not fit in L2, so a lane-per-query kernel is bound by line traffic from
-// the last-level cache.  For large batches the queries are first sorted into
-// bins of 16^3 voxels (over the grid's active bounding box grown by 2 voxels;
-// points outside are clamped into the border bins):
-//   k_bin_keys     : bin key per point and the point itself {index-space xyz (after the FP64
-//                    world->index map when asked), original index}
-//   radix sort     : (key, point) pairs, rocPRIM/hipCUB DeviceRadixSort; the points travel
-//                    with their keys, so the per-bin pass reads them coalesced instead of
-//                    gathering 12 bytes per 128-byte cache line
+// Random query points touch a dozen distinct cache lines each over a grid that does not fit in
+// L2, so a lane-per-query kernel is bound by line traffic from the last-level cache.  For large
+// batches (forward):
+//   k_slot_table   : every 8^3 slot of the binned region: leaf bit (a 1-bit map) and the tile or
+//                    background value of slots without a leaf
+//   k_bin_classify : per query, whether a slot of its [cell-1, cell+2]^3 block holds a leaf (map
+//                    staged in LDS); marks those queries per chunk of 2048 and counts them
+//   scan + compact : the marked queries' bin keys (16^3 bins over the grid's active bbox grown by
+//                    2 voxels; points outside are clamped into the border bins) and points
+//                    {index-space xyz, original index}, in chunk order; the call waits once, to
+//                    size the sort
+//   radix sort     : (key, point) pairs of the marked queries (rocPRIM/hipCUB DeviceRadixSort)
 //   k_bin_bounds   : [begin, end) of every non-empty bin in the sorted order
-//   k_bin_sample   : one workgroup per bin stages the 20^3 brick of voxel values
-//                    around the bin in LDS (ReadAccessor::getValue semantics, so
-//                    results are bit-identical) and evaluates the bin's queries
-//                    from LDS; each query's results go to its original index as
-//                    one aligned 32-byte record {nn, val, grad xyz}, a whole memory
-//                    sector, instead of three partial-sector writes to the outputs.
-//   k_sample_list  : queries whose stencil leaves the brick (border bins) run
-//                    through the lane accessor, into the same records.
-//   k_unpack       : records -> the requested outputs, coalesced.
+//   k_bin_sample   : one workgroup per bin stages the 20^3 brick of voxel values around the bin in
+//                    LDS (ReadAccessor::getValue semantics, so results are bit-identical) and
+//                    evaluates the bin's queries from it; each query's results go to its original
+//                    index as one 64-byte record, written by the four lanes of a quad
+//   k_sample_list  : queries whose stencil leaves the brick run through the lane accessor
+//   k_bin_finish   : the outputs in caller order: marked queries copy their record, the others
+//                    (their block has no leaf) are evaluated from the slot table's values
+// On MI355X a random store is cheap only when one instruction writes a whole aligned 64-byte block
+// (2^26 random 32-byte records: 2.9 ms; 64-byte records written by lane quads: 1.3 ms), which is
+// why records are 64 bytes and the compaction writes in item/wave/lane order.
 #include "nvq_device.h"
 #include "nvq_internal.h"
 #include "nvq_profile.h"
@@ -54,17 +57,37 @@ struct alignas(16) Pt {
     uint32_t id;   // original position in the batch
 };
 
-struct alignas(32) Rec {
-    float4 a; // nn, val, grad x, grad y
-    float4 b; // grad z, unused
+// One query's results at its original index: 64 bytes although 20 are used.  On MI355X a random
+// write is cheap only when one store instruction covers a whole aligned 64-byte block: 2^26 random
+// records take 1.3 ms written that way, against 2.9 ms for 32-byte records (a partial block is
+// read, merged and written back) and 2.8 ms for 64-byte records one lane writes with four stores.
+// So the four lanes of a quad write the four 16-byte quarters of one record per instruction
+// (storeRecQuad).  k_unpack reads only the first 32 bytes.
+struct alignas(64) Rec {
+    float4 a;    // nn, val, grad x, grad y
+    float4 b;    // grad z, unused
+    float4 c, d; // unused
 };
 
-__device__ __forceinline__ void storeRec(Rec* __restrict__ rec, uint32_t id, float nn, float val, const float g[3])
+// Wave-wide record store: every lane of the wave must call it (inactive queries pass valid =
+// false).  Four instructions; in instruction j the quad's lanes h = 0..3 write quarter h of the
+// record of the quad's lane j.
+__device__ __forceinline__ void storeRecQuad(Rec* __restrict__ rec, bool valid, uint32_t id, float nn, float val,
+                                             const float g[3])
 {
-    Rec r;
-    r.a = make_float4(nn, val, g[0], g[1]);
-    r.b = make_float4(g[2], 0.0f, 0.0f, 0.0f);
-    rec[id] = r;
+    const int lane = int(threadIdx.x & 63u), h = lane & 3, quad = lane & ~3;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int src = quad + j;
+        const int ok = __shfl(int(valid), src, 64);
+        const uint32_t d = __shfl(id, src, 64);
+        const float a = __shfl(nn, src, 64), b = __shfl(val, src, 64), c = __shfl(g[0], src, 64),
+                    e = __shfl(g[1], src, 64), f = __shfl(g[2], src, 64);
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (h == 0) v = make_float4(a, b, c, e);
+        if (h == 1) v.x = f;
+        if (ok) reinterpret_cast<float4*>(rec + d)[h] = v;
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t m, uint32_t v)
@@ -109,6 +132,290 @@ __global__ __launch_bounds__(kThreads) void k_bin_keys(GridView g, BinLayout L, 
         t.z = p[2];
         t.id = i;
         pts[i] = t;
+    }
+}
+
+// The 8^3 slots that [cell-1, cell+2]^3 overlaps (1 or 2 per axis), with the value every voxel
+// of a slot without a leaf reads (its tile or background value, ReadAccessor::getValue semantics).
+struct SlotBlock {
+    int32_t s0[3], s1[3];
+    Cell v; // v.v[a][b][e]: slot (s0 + (a,b,e)); unused where s1 == s0
+};
+
+// Every 8^3 slot of a box covering the binned region: whether a leaf holds it, and otherwise the
+// value all its voxels read (tile or background, ReadAccessor::getValue semantics).  Built per
+// call by one tree walk per slot; the classify pass then reads it with independent loads instead
+// of walking the tree per query.
+struct SlotTable {
+    int32_t lo[3];   // first slot per axis
+    uint32_t n[3];   // slots per axis
+    const float* value;
+    const uint32_t* leaf; // one bit per slot (bit i%32 of word i/32): a leaf holds the slot
+};
+
+// Bitmaps of at most this many 32-bit words are staged in LDS by the classify pass (256K slots, the
+// C2 grids need 6-8K words); larger ones are read from global memory.
+constexpr uint32_t kLdsLeafWords = 8192;
+
+// One thread per slot; the bitmap words are assembled with a wave ballot (64 slots per wave, so
+// every word is written whole by one lane).  The launch covers whole waves: count rounded up to 64.
+template<uint32_t GT>
+__global__ __launch_bounds__(kThreads) void k_slot_table(GridView g, SlotTable T, float* __restrict__ value,
+                                                         uint32_t* __restrict__ leaf)
+{
+    Accessor<GT> acc(g);
+    const uint64_t count = uint64_t(T.n[0]) * T.n[1] * T.n[2];
+    const uint64_t padded = (count + 63) & ~uint64_t(63);
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < padded; i += uint64_t(gridDim.x) * kThreads) {
+        bool isLeaf = false;
+        if (i < count) {
+            const uint32_t z = uint32_t(i % T.n[2]), y = uint32_t((i / T.n[2]) % T.n[1]),
+                           x = uint32_t(i / (uint64_t(T.n[2]) * T.n[1]));
+            const Probe pr = acc.probe((T.lo[0] + int32_t(x)) << 3, (T.lo[1] + int32_t(y)) << 3,
+                                       (T.lo[2] + int32_t(z)) << 3);
+            value[i] = pr.value;
+            isLeaf = pr.leaf != nullptr;
+        }
+        const uint64_t b = __ballot(isLeaf);
+        const uint32_t lane = threadIdx.x & 63u;
+        if (lane == 0) leaf[i >> 5] = uint32_t(b);
+        if (lane == 32) leaf[i >> 5] = uint32_t(b >> 32);
+    }
+}
+
+// True when a slot of the block around p's cell holds a leaf or lies outside the table (the query
+// then goes through the sort).  `bits` is the leaf bitmap (in LDS or global memory).
+__device__ __forceinline__ bool touchesLeaf(const SlotTable& T, const uint32_t* bits, const float p[3], SlotBlock& B)
+{
+    uint32_t r0[3], r1[3];
+    bool outside = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t c = int32_t(floorf(p[a]));
+        B.s0[a] = (c - 1) >> 3;
+        B.s1[a] = (c + 2) >> 3;
+        r0[a] = uint32_t(B.s0[a] - T.lo[a]);
+        r1[a] = uint32_t(B.s1[a] - T.lo[a]);
+        outside = outside || r0[a] >= T.n[a] || r1[a] >= T.n[a];
+    }
+    if (outside) return true;
+    const bool two[3] = {r1[0] != r0[0], r1[1] != r0[1], r1[2] != r0[2]};
+    bool leaf = false;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if ((a && !two[0]) || (b && !two[1]) || (e && !two[2])) continue; // a repeated slot
+                const uint64_t i = (uint64_t(a ? r1[0] : r0[0]) * T.n[1] + (b ? r1[1] : r0[1])) * T.n[2] + (e ? r1[2] : r0[2]);
+                leaf = leaf || ((bits[i >> 5] >> (i & 31u)) & 1u);
+            }
+    return leaf;
+}
+
+// The slot values of the block around p's cell (a query touchesLeaf found inside the table and
+// without leaves): distinct slots are loaded, repeated ones copied.
+__device__ __forceinline__ void loadSlots(const SlotTable& T, const float p[3], SlotBlock& B)
+{
+    uint32_t r[2][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t c = int32_t(floorf(p[a]));
+        B.s0[a] = (c - 1) >> 3;
+        B.s1[a] = (c + 2) >> 3;
+        r[0][a] = uint32_t(B.s0[a] - T.lo[a]);
+        r[1][a] = uint32_t(B.s1[a] - T.lo[a]);
+    }
+    const bool two[3] = {r[1][0] != r[0][0], r[1][1] != r[0][1], r[1][2] != r[0][2]};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int sa = two[0] ? a : 0, sb = two[1] ? b : 0, se = two[2] ? e : 0;
+                if ((a && !two[0]) || (b && !two[1]) || (e && !two[2])) {
+                    B.v.v[a][b][e] = pick(B.v, sa, sb, se);
+                    continue;
+                }
+                B.v.v[a][b][e] = T.value[(uint64_t(r[a][0]) * T.n[1] + r[b][1]) * T.n[2] + r[e][2]];
+            }
+}
+
+// Value of voxel (x, y, z) of the block (no leaf in it): the value of its slot
+__device__ __forceinline__ float slotValue(const SlotBlock& B, int32_t x, int32_t y, int32_t z)
+{
+    return pick(B.v, (x >> 3) != B.s0[0], (y >> 3) != B.s0[1], (z >> 3) != B.s0[2]);
+}
+
+// The query's outputs from the slot values (bit-identical to sampleEval: same cells, fractions and
+// operation order).  Requires the FD6 sample cells to be the neighbouring ones (adjointInBlock).
+__device__ __forceinline__ SampleOut sampleSlots(const SlotBlock& B, const float p[3], bool wantNN, bool wantVal,
+                                                 int mode, const SampleConsts& k)
+{
+    SampleOut r{0.0f, 0.0f, {0.0f, 0.0f, 0.0f}};
+    const float V = B.v.v[0][0][0];
+    bool uniform = isfinite(V) && __float_as_uint(V) != 0x80000000u;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) uniform = uniform && __float_as_uint(B.v.v[a][b][e]) == __float_as_uint(V);
+    if (uniform) {
+        // every value the query reads is V (finite, not -0): lerps give V, differences +0
+        r.nn = V;
+        r.val = V;
+        return r;
+    }
+    auto cellAt = [&](int32_t i, int32_t j, int32_t l, Cell& C) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) C.v[a][b][e] = slotValue(B, i + a, j + b, l + e);
+    };
+    auto tri = [&](float qx, float qy, float qz) {
+        const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+        Cell C;
+        cellAt(int32_t(fx), int32_t(fy), int32_t(fz), C);
+        return trilerp(C, qx - fx, qy - fy, qz - fz);
+    };
+    const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+    const int32_t i0 = int32_t(fx), j0 = int32_t(fy), k0 = int32_t(fz);
+    if (wantNN) r.nn = slotValue(B, int32_t(rintf(p[0])), int32_t(rintf(p[1])), int32_t(rintf(p[2])));
+    if (wantVal) r.val = tri(p[0], p[1], p[2]);
+    if (mode == 1) {
+        r.g[0] = (tri(p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f) - tri(p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f)) * k.fdx;
+        r.g[1] = (tri(p[0] + 0.0f, p[1] + 1.0f, p[2] + 0.0f) - tri(p[0] - 0.0f, p[1] - 1.0f, p[2] - 0.0f)) * k.fdy;
+        r.g[2] = (tri(p[0] + 0.0f, p[1] + 0.0f, p[2] + 1.0f) - tri(p[0] - 0.0f, p[1] - 0.0f, p[2] - 1.0f)) * k.fdz;
+    } else if (mode == 2) {
+        r.g[0] = k.gsInv2dx * (slotValue(B, i0 + 1, j0, k0) - slotValue(B, i0 - 1, j0, k0));
+        r.g[1] = k.gsInv2dx * (slotValue(B, i0, j0 + 1, k0) - slotValue(B, i0, j0 - 1, k0));
+        r.g[2] = k.gsInv2dx * (slotValue(B, i0, j0, k0 + 1) - slotValue(B, i0, j0, k0 - 1));
+    } else if (mode == 3) {
+        Cell C;
+        cellAt(i0, j0, k0, C);
+        trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, r.g);
+    }
+    return r;
+}
+
+constexpr int kClassifyItems = 8; // points per thread per block iteration
+
+constexpr uint32_t kChunk = kThreads * kClassifyItems; // queries per block iteration
+
+// Exclusive prefix over the block of one count per thread (waves in order, lanes in order); `total`
+// receives the block's sum.  Uses `waveSum`, which the caller must not reuse before a barrier.
+__device__ __forceinline__ uint32_t blockPrefix(uint32_t cnt, uint32_t* waveSum, uint32_t& total)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d, 64);
+        if (lane >= uint32_t(d)) incl += v;
+    }
+    if (lane == 63) waveSum[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        off += uint32_t(w) < wave ? waveSum[w] : 0u;
+        total += waveSum[w];
+    }
+    return off + incl - cnt;
+}
+
+// Classify: chunk c holds queries c*kChunk + j*kThreads + t (item j of thread t), so loads are
+// coalesced.  Queries that touch a leaf (or whose FD6 samples round into a farther cell, or that
+// lie outside the slot table) are marked in flags[c*kThreads + t] (bit j) and counted per chunk in
+// counts[c]; k_bin_compact places them after a scan of the counts (no atomics) in chunk, item,
+// wave and lane order.  The others are answered from their slot values by k_bin_finish.
+__global__ __launch_bounds__(kThreads) void k_bin_classify(GridView g, SlotTable T, const float* __restrict__ xyz,
+                                                           uint32_t n, uint32_t chunks, int world, int mode,
+                                                           uint8_t* __restrict__ flagsOut, uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t waveSum[kThreads / 64];
+    __shared__ uint32_t ldsLeaf[kLdsLeafWords];
+    const uint64_t words = (uint64_t(T.n[0]) * T.n[1] * T.n[2] + 31) / 32;
+    const uint32_t* bits = T.leaf;
+    if (words <= kLdsLeafWords) {
+        for (uint32_t w = threadIdx.x; w < uint32_t(words); w += kThreads) ldsLeaf[w] = T.leaf[w];
+        __syncthreads();
+        bits = ldsLeaf;
+    }
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint64_t c0 = uint64_t(c) * kChunk;
+        uint32_t flags = 0;
+        for (int j = 0; j < kClassifyItems; ++j) {
+            const uint64_t i = c0 + uint64_t(j) * kThreads + threadIdx.x;
+            if (i >= n) continue;
+            float p[3];
+            loadPoint(g, xyz, i, world, p);
+            SlotBlock B;
+            // FD6 samples that round into a farther cell read outside the block's slots: such
+            // queries go through the sort with the others
+            if (touchesLeaf(T, bits, p, B) || !adjointInBlock(p, mode)) flags |= 1u << j;
+        }
+        flagsOut[uint64_t(c) * kThreads + threadIdx.x] = uint8_t(flags);
+        uint32_t total;
+        (void)blockPrefix(__popc(flags), waveSum, total);
+        if (threadIdx.x == 0) counts[c] = total;
+        __syncthreads(); // waveSum is reused by the next chunk
+    }
+}
+
+// Compaction of the marked queries: chunk c's go to offsets[c] on, ordered by item, wave and lane,
+// so each store instruction writes one contiguous run (whole 64-byte blocks of keys and points).
+__global__ __launch_bounds__(kThreads) void k_bin_compact(GridView g, BinLayout L, const float* __restrict__ xyz,
+                                                          uint32_t chunks, int world,
+                                                          const uint8_t* __restrict__ flagsIn,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          uint32_t* __restrict__ keys, Pt* __restrict__ pts)
+{
+    constexpr int kWaves = kThreads / 64;
+    __shared__ uint32_t runBase[kClassifyItems][kWaves];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t flags = flagsIn[uint64_t(c) * kThreads + threadIdx.x];
+        uint64_t ball[kClassifyItems];
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            ball[j] = __ballot((flags >> j) & 1u);
+            if (lane == 0) runBase[j][wave] = uint32_t(__popcll(ball[j]));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = offsets[c];
+            for (int j = 0; j < kClassifyItems; ++j)
+                for (int w = 0; w < kWaves; ++w) {
+                    const uint32_t v = runBase[j][w];
+                    runBase[j][w] = acc;
+                    acc += v;
+                }
+        }
+        __syncthreads();
+        const uint64_t below = (uint64_t(1) << lane) - 1u;
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            if (!((flags >> j) & 1u)) continue;
+            const uint32_t pos = runBase[j][wave] + uint32_t(__popcll(ball[j] & below));
+            const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
+            float p[3];
+            loadPoint(g, xyz, i, world, p);
+            keys[pos] = binOf(L, p);
+            Pt t;
+            t.x = p[0];
+            t.y = p[1];
+            t.z = p[2];
+            t.id = uint32_t(i);
+            pts[pos] = t;
+        }
+        __syncthreads(); // runBase is reused by the next chunk
     }
 }
 
@@ -165,58 +472,61 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
         brick[v] = leaf ? Codec<GT>::get(leaf, Accessor<GT>::leafOffset(gx, gy, gz), g.channel) : slotValue[slot];
     }
     __syncthreads();
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBinThreads) {
-        const Pt t = sorted[q];
+    // wave-uniform loop (the record store is a wave-wide operation): 64 consecutive queries per wave
+    for (uint32_t base = q0 + (threadIdx.x & ~63u); base < q1; base += kBinThreads) {
+        const uint32_t q = base + (threadIdx.x & 63u);
+        bool valid = q < q1;
+        const Pt t = sorted[valid ? q : q0];
         const uint32_t id = t.id;
         const float p[3] = {t.x, t.y, t.z};
         const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
         const int32_t ci = int32_t(fx) - o[0], cj = int32_t(fy) - o[1], ck = int32_t(fz) - o[2];
         // every FD6 sample cell lies in [cell-1, cell+2] (p +- 1 rounds at most one voxel further up),
         // so the whole stencil is inside the brick iff the cell is in [1, kBrick-4]
-        if (uint32_t(ci - 1) > uint32_t(kBrick - 5) || uint32_t(cj - 1) > uint32_t(kBrick - 5) ||
-            uint32_t(ck - 1) > uint32_t(kBrick - 5)) {
+        if (valid && (uint32_t(ci - 1) > uint32_t(kBrick - 5) || uint32_t(cj - 1) > uint32_t(kBrick - 5) ||
+                      uint32_t(ck - 1) > uint32_t(kBrick - 5))) {
             spill[atomicAdd(spillCount, 1u)] = id;
-            continue;
+            valid = false;
         }
-        auto cell = [&](float qx, float qy, float qz) -> float {
-            const float gx = floorf(qx), gy = floorf(qy), gz = floorf(qz);
-            const int32_t base = ((int32_t(gx) - o[0]) * kBrick + (int32_t(gy) - o[1])) * kBrick + (int32_t(gz) - o[2]);
-            Cell c;
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                    for (int e = 0; e < 2; ++e) c.v[a][bb][e] = brick[base + (a * kBrick + bb) * kBrick + e];
-            return trilerp(c, qx - gx, qy - gy, qz - gz);
-        };
-        const int32_t c0 = (ci * kBrick + cj) * kBrick + ck;
         float nnv = 0.0f, vv = 0.0f;
-        if (wantNN) {
-            const int32_t ri = int32_t(rintf(p[0])) - o[0], rj = int32_t(rintf(p[1])) - o[1], rk = int32_t(rintf(p[2])) - o[2];
-            nnv = brick[(ri * kBrick + rj) * kBrick + rk];
-        }
-        if (wantVal) vv = cell(p[0], p[1], p[2]);
         float g3[3] = {0.0f, 0.0f, 0.0f};
-        if (mode == 1) {
-            g3[0] = (cell(p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f) - cell(p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f)) * k.fdx;
-            g3[1] = (cell(p[0] + 0.0f, p[1] + 1.0f, p[2] + 0.0f) - cell(p[0] - 0.0f, p[1] - 1.0f, p[2] - 0.0f)) * k.fdy;
-            g3[2] = (cell(p[0] + 0.0f, p[1] + 0.0f, p[2] + 1.0f) - cell(p[0] - 0.0f, p[1] - 0.0f, p[2] - 1.0f)) * k.fdz;
-        } else if (mode == 2) {
-            g3[0] = k.gsInv2dx * (brick[c0 + kBrick * kBrick] - brick[c0 - kBrick * kBrick]);
-            g3[1] = k.gsInv2dx * (brick[c0 + kBrick] - brick[c0 - kBrick]);
-            g3[2] = k.gsInv2dx * (brick[c0 + 1] - brick[c0 - 1]);
-        } else if (mode == 3) {
-            Cell C;
+        if (valid) {
+            auto load = [&](int32_t cb, Cell& c) {
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int bb = 0; bb < 2; ++bb)
+                    for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                    for (int e = 0; e < 2; ++e) C.v[a][bb][e] = brick[c0 + (a * kBrick + bb) * kBrick + e];
-            trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, g3);
+                        for (int e = 0; e < 2; ++e) c.v[a][bb][e] = brick[cb + (a * kBrick + bb) * kBrick + e];
+            };
+            auto cell = [&](float qx, float qy, float qz) -> float {
+                const float gx = floorf(qx), gy = floorf(qy), gz = floorf(qz);
+                const int32_t cb = ((int32_t(gx) - o[0]) * kBrick + (int32_t(gy) - o[1])) * kBrick + (int32_t(gz) - o[2]);
+                Cell c;
+                load(cb, c);
+                return trilerp(c, qx - gx, qy - gy, qz - gz);
+            };
+            const int32_t c0 = (ci * kBrick + cj) * kBrick + ck;
+            if (wantNN) {
+                const int32_t ri = int32_t(rintf(p[0])) - o[0], rj = int32_t(rintf(p[1])) - o[1], rk = int32_t(rintf(p[2])) - o[2];
+                nnv = brick[(ri * kBrick + rj) * kBrick + rk];
+            }
+            if (wantVal) vv = cell(p[0], p[1], p[2]);
+            if (mode == 1) {
+                g3[0] = (cell(p[0] + 1.0f, p[1] + 0.0f, p[2] + 0.0f) - cell(p[0] - 1.0f, p[1] - 0.0f, p[2] - 0.0f)) * k.fdx;
+                g3[1] = (cell(p[0] + 0.0f, p[1] + 1.0f, p[2] + 0.0f) - cell(p[0] - 0.0f, p[1] - 1.0f, p[2] - 0.0f)) * k.fdy;
+                g3[2] = (cell(p[0] + 0.0f, p[1] + 0.0f, p[2] + 1.0f) - cell(p[0] - 0.0f, p[1] - 0.0f, p[2] - 1.0f)) * k.fdz;
+            } else if (mode == 2) {
+                g3[0] = k.gsInv2dx * (brick[c0 + kBrick * kBrick] - brick[c0 - kBrick * kBrick]);
+                g3[1] = k.gsInv2dx * (brick[c0 + kBrick] - brick[c0 - kBrick]);
+                g3[2] = k.gsInv2dx * (brick[c0 + 1] - brick[c0 - 1]);
+            } else if (mode == 3) {
+                Cell C;
+                load(c0, C);
+                trilinearGradient(C, p[0] - fx, p[1] - fy, p[2] - fz, g3);
+            }
         }
-        storeRec(rec, id, nnv, vv, g3);
+        storeRecQuad(rec, valid, id, nnv, vv, g3);
     }
 }
 
@@ -228,12 +538,18 @@ __global__ __launch_bounds__(kThreads) void k_sample_list(GridView g, const floa
 {
     Accessor<GT> acc(g);
     const uint32_t m = *count;
-    for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < m; j += gridDim.x * kThreads) {
-        const uint32_t id = list[j];
-        float p[3];
-        loadPoint(g, xyz, id, world, p);
-        const SampleOut r = sampleEval(acc, p, wantNN, wantVal, mode, k);
-        storeRec(rec, id, r.nn, r.val, r.g);
+    for (uint32_t base = blockIdx.x * kThreads + (threadIdx.x & ~63u); base < m; base += gridDim.x * kThreads) {
+        const uint32_t j = base + (threadIdx.x & 63u);
+        const bool valid = j < m;
+        SampleOut r{0.0f, 0.0f, {0.0f, 0.0f, 0.0f}};
+        uint32_t id = 0;
+        if (valid) {
+            id = list[j];
+            float p[3];
+            loadPoint(g, xyz, id, world, p);
+            r = sampleEval(acc, p, wantNN, wantVal, mode, k);
+        }
+        storeRecQuad(rec, valid, id, r.nn, r.val, r.g);
     }
 }
 
@@ -250,6 +566,66 @@ __global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec
             grad[3 * uint64_t(i)] = a.z;
             grad[3 * uint64_t(i) + 1] = a.w;
             grad[3 * uint64_t(i) + 2] = gz;
+        }
+    }
+}
+
+// Outputs of a classified call, in the classify pass's chunk order: a marked query copies its record,
+// the others are answered from their slot values (sampleSlots, bit-identical to the lane accessor).
+__global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T, const float* __restrict__ xyz,
+                                                         uint32_t n, uint32_t chunks, int world, bool wantNN,
+                                                         bool wantVal, int mode, SampleConsts k,
+                                                         const uint8_t* __restrict__ flagsIn,
+                                                         const Rec* __restrict__ rec, float* __restrict__ nn,
+                                                         float* __restrict__ val, float* __restrict__ grad)
+{
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t flags = flagsIn[uint64_t(c) * kThreads + threadIdx.x];
+        // the marked queries' records, all loads issued before the first use
+        float4 ra[kClassifyItems];
+        float rb[kClassifyItems];
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
+            const bool marked = ((flags >> j) & 1u) != 0; // only queries below n are marked
+            ra[j] = marked ? rec[i].a : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            rb[j] = marked && grad ? rec[i].b.x : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
+            if (!((flags >> j) & 1u)) continue;
+            if (nn) nn[i] = ra[j].x;
+            if (val) val[i] = ra[j].y;
+            if (grad) {
+                grad[3 * i] = ra[j].z;
+                grad[3 * i + 1] = ra[j].w;
+                grad[3 * i + 2] = rb[j];
+            }
+        }
+        for (int j = 0; j < kClassifyItems; ++j) {
+            const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
+            if (i >= n || ((flags >> j) & 1u)) continue;
+            float o[5];
+            {
+                float p[3];
+                loadPoint(g, xyz, i, world, p);
+                SlotBlock B;
+                loadSlots(T, p, B);
+                const SampleOut r = sampleSlots(B, p, wantNN, wantVal, mode, k);
+                o[0] = r.nn;
+                o[1] = r.val;
+                o[2] = r.g[0];
+                o[3] = r.g[1];
+                o[4] = r.g[2];
+            }
+            if (nn) nn[i] = o[0];
+            if (val) val[i] = o[1];
+            if (grad) {
+                grad[3 * i] = o[2];
+                grad[3 * i + 1] = o[3];
+                grad[3 * i + 2] = o[4];
+            }
         }
     }
 }
@@ -434,20 +810,66 @@ bool makeLayout(const GridView& g, BinLayout& L)
 }
 
 // Scratch of one binned call: keys[2n] | pts[2n] | begin[m] | end[m] | spillCount | flag | spill[n]
-// | rec[n] (forward only) | sort temporaries, one stream-ordered allocation.
+// | forward only: rec[n], classify marks, counts and offsets, scan temporaries | sort temporaries,
+// one stream-ordered allocation.
 struct Bins {
     BinLayout L;
     uint32_t n = 0;
+    uint32_t m = 0; // queries in the sort (all n, or those the forward's classify pass kept)
     char* base = nullptr;
     uint32_t *keys = nullptr, *keys2 = nullptr, *begin = nullptr, *end = nullptr, *spillCount = nullptr,
              *flag = nullptr, *spill = nullptr;
+    uint32_t chunks = 0;                                 // classify chunks (forward only; 0: not classified)
+    SlotTable T{};                                       // the classify pass's slot table
+    uint8_t* cflags = nullptr;                           // [chunks * kThreads] per-thread marks
+    uint32_t *counts = nullptr, *offsets = nullptr;      // [chunks + 1] marked per chunk, their scan
+    void* scanTemp = nullptr;
+    size_t scanBytes = 0;
     Pt *idx = nullptr, *idx2 = nullptr;
     Rec* rec = nullptr;
 };
 
-// Allocate, key, sort and bound the bins of n points.  0, or -1 (no layout), -2 (allocation),
-// -4 (sort: nothing was launched that reads the sorted arrays; the scratch is already freed).
-int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool withRecords, Bins& B, hipStream_t s)
+// What the forward's classify pass evaluates for the queries it answers itself.
+struct Classify {
+    bool wantNN, wantVal;
+    int mode;
+};
+
+constexpr uint64_t kMaxTableSlots = uint64_t(1) << 24; // larger regions sort every query
+
+// The slot table's box: the bins' region (2 voxels around the grid's bbox, rounded up to bins)
+// plus one slot on each side.
+bool slotBox(const BinLayout& L, SlotTable& T)
+{
+    uint64_t count = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t lo = (int64_t(L.lo[a]) >> 3) - 1, hi = ((int64_t(L.lo[a]) + (int64_t(L.nb[a]) << kBinLog2) - 1) >> 3) + 1;
+        T.lo[a] = int32_t(lo);
+        T.n[a] = uint32_t(hi - lo + 1);
+        count *= T.n[a];
+    }
+    return count <= kMaxTableSlots;
+}
+
+template<uint32_t GT>
+void launchClassify(const GridView& g, const Bins& B, const SlotTable& T, float* tv, uint32_t* tl, const float* xyz,
+                    int world, const Classify& c, uint32_t blocks, hipStream_t s)
+{
+    const uint64_t count = uint64_t(T.n[0]) * T.n[1] * T.n[2];
+    hipLaunchKernelGGL(k_slot_table<GT>, dim3(uint32_t(std::min<uint64_t>((count + kThreads - 1) / kThreads, 1u << 16))),
+                       dim3(kThreads), 0, s, g, T, tv, tl);
+    ProfScope prof("k_bin_classify", s);
+    hipLaunchKernelGGL(k_bin_classify, dim3(blocks), dim3(kThreads), 0, s, g, T, xyz, B.n, B.chunks, world, c.mode,
+                       B.cflags, B.counts);
+}
+
+// Allocate, key, sort and bound the bins of n points.  With `cls` (the forward; requires
+// withRecords) the queries that touch no leaf are answered by the classify pass and only the
+// others are sorted: the call then waits for the stream once, to size the sort.  0, or -1 (no
+// layout), -2 (allocation), -3 (grid type), -4 (sort or HIP error: nothing was launched that reads
+// the sorted arrays; the scratch is already freed).
+int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool withRecords, Bins& B, hipStream_t s,
+              const Classify* cls = nullptr)
 {
     if (!makeLayout(g, B.L)) return -1;
     const uint32_t m = B.L.nbins;
@@ -457,11 +879,22 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
                                            static_cast<uint32_t*>(nullptr), static_cast<Pt*>(nullptr),
                                            static_cast<Pt*>(nullptr), n, 0, int(B.L.bits), s) != hipSuccess)
         return -4;
+    SlotTable T{};
+    if (cls && !slotBox(B.L, T)) cls = nullptr; // too large a region for the slot table: sort every query
+    const uint64_t slots = cls ? uint64_t(T.n[0]) * T.n[1] * T.n[2] : 0;
+    const uint32_t chunks = cls ? uint32_t((uint64_t(n) + kChunk - 1) / kChunk) : 0u;
+    size_t scanBytes = 0;
+    if (cls && hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, static_cast<uint32_t*>(nullptr),
+                                                static_cast<uint32_t*>(nullptr), chunks + 1, s) != hipSuccess)
+        return -4;
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t oKeys = 0, oKeys2 = up(oKeys + 4ull * n), oIdx = up(oKeys2 + 4ull * n),
                  oIdx2 = up(oIdx + sizeof(Pt) * size_t(n)), oBegin = up(oIdx2 + sizeof(Pt) * size_t(n)),
                  oEnd = up(oBegin + 4ull * m), oCount = up(oEnd + 4ull * m), oSpill = up(oCount + 8),
-                 oRec = up(oSpill + 4ull * n), oTemp = up(oRec + (withRecords ? sizeof(Rec) * size_t(n) : 0)),
+                 oRec = up(oSpill + 4ull * n), oFlags = up(oRec + (withRecords ? sizeof(Rec) * size_t(n) : 0)),
+                 oCounts = up(oFlags + size_t(chunks) * kThreads), oOffsets = up(oCounts + 4ull * (chunks + 1)),
+                 oScan = up(oOffsets + 4ull * (chunks + 1)), oTabV = up(oScan + scanBytes),
+                 oTabL = up(oTabV + 4 * slots), oTemp = up(oTabL + 4 * ((slots + 63) / 64) * 2),
                  total = oTemp + up(sortBytes);
     keepPoolMemory();
     if (hipMallocAsync(reinterpret_cast<void**>(&B.base), total, s) != hipSuccess) return -2;
@@ -474,6 +907,12 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
     B.end = reinterpret_cast<uint32_t*>(base + oEnd);
     B.spillCount = reinterpret_cast<uint32_t*>(base + oCount);
     B.flag = B.spillCount + 1;
+    B.chunks = chunks;
+    B.cflags = reinterpret_cast<uint8_t*>(base + oFlags);
+    B.counts = reinterpret_cast<uint32_t*>(base + oCounts);
+    B.offsets = reinterpret_cast<uint32_t*>(base + oOffsets);
+    B.scanTemp = base + oScan;
+    B.scanBytes = scanBytes;
     B.spill = reinterpret_cast<uint32_t*>(base + oSpill);
     B.rec = withRecords ? reinterpret_cast<Rec*>(base + oRec) : nullptr;
     void* temp = base + oTemp;
@@ -481,14 +920,56 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
     const uint32_t zeroWords = uint32_t((oCount + 8) / 4 - oBegin / 4); // begin[], end[], spillCount and flag
     hipLaunchKernelGGL(k_fill_u32, dim3((zeroWords + kThreads - 1) / kThreads), dim3(kThreads), 0, s, B.begin,
                        zeroWords, 0u);
-    hipLaunchKernelGGL(k_bin_keys, dim3(gridN), dim3(kThreads), 0, s, g, B.L, xyz, n, world, B.keys, B.idx);
-    if (hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, B.keys, B.keys2, B.idx, B.idx2, n, 0, int(B.L.bits), s) !=
-        hipSuccess) {
+    B.m = n;
+    if (cls) {
+        // classify: enough blocks to fill the chip, each staging the leaf bitmap once
+        const uint32_t blocks = std::min<uint32_t>(chunks, 4096u);
+        // counts[chunks] stays 0, so the exclusive scan's last entry is the number of marked queries
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kThreads), 0, s, B.counts + chunks, 1u, 0u);
+        float* tv = reinterpret_cast<float*>(base + oTabV);
+        uint32_t* tl = reinterpret_cast<uint32_t*>(base + oTabL);
+        T.value = tv;
+        T.leaf = tl;
+        B.T = T;
+        switch (g.type) {
+        case GT_FLOAT: launchClassify<GT_FLOAT>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
+        case GT_FP4: launchClassify<GT_FP4>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
+        case GT_FP8: launchClassify<GT_FP8>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
+        case GT_FP16: launchClassify<GT_FP16>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
+        case GT_FPN: launchClassify<GT_FPN>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
+        case GT_ONINDEX: launchClassify<GT_ONINDEX>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
+        default: (void)hipFreeAsync(B.base, s); return -3;
+        }
+        uint32_t m = 0;
+        if (hipGetLastError() != hipSuccess ||
+            hipcub::DeviceScan::ExclusiveSum(B.scanTemp, B.scanBytes, B.counts, B.offsets, chunks + 1, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(&m, B.offsets + chunks, sizeof(m), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess || m > n) {
+            (void)hipGetLastError();
+            return hipFreeAsync(B.base, s) == hipSuccess ? -4 : -5;
+        }
+        B.m = m;
+        if (m)
+            hipLaunchKernelGGL(k_bin_compact, dim3(blocks), dim3(kThreads), 0, s, g, B.L, xyz, chunks, world, B.cflags,
+                               B.offsets, B.keys, B.idx);
+    } else {
+        hipLaunchKernelGGL(k_bin_keys, dim3(gridN), dim3(kThreads), 0, s, g, B.L, xyz, n, world, B.keys, B.idx);
+    }
+    if (B.m == 0) return 0;
+    hipError_t sorted;
+    {
+        ProfScope prof("bin_sort", s);
+        sorted = hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, B.keys, B.keys2, B.idx, B.idx2, B.m, 0,
+                                                    int(B.L.bits), s);
+    }
+    if (sorted != hipSuccess) {
         // the sorted arrays were never written: launch nothing that reads them
         (void)hipGetLastError();
         return hipFreeAsync(B.base, s) == hipSuccess ? -4 : -5;
     }
-    hipLaunchKernelGGL(k_bin_bounds, dim3(gridN), dim3(kThreads), 0, s, B.keys2, n, B.begin, B.end);
+    const uint32_t gridM = std::min<uint32_t>((B.m + kThreads - 1) / kThreads, 1u << 16);
+    hipLaunchKernelGGL(k_bin_bounds, dim3(gridM), dim3(kThreads), 0, s, B.keys2, B.m, B.begin, B.end);
     return 0;
 }
 
@@ -520,13 +1001,14 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
 {
     if (n64 >= (uint64_t(1) << 31)) return -1;
     Bins B;
-    const int bs = binPoints(g, xyz, uint32_t(n64), world, true, B, s);
+    const bool wantNN = nn != nullptr, wantVal = val != nullptr;
+    const int gm = grad ? mode : 0;
+    const Classify cls{wantNN, wantVal, gm};
+    const int bs = binPoints(g, xyz, uint32_t(n64), world, true, B, s, &cls);
     if (bs) return bs;
     const uint32_t n = B.n;
     int st = 0;
-    const bool wantNN = nn != nullptr, wantVal = val != nullptr;
-    const int gm = grad ? mode : 0;
-    switch (g.type) {
+    if (B.m) switch (g.type) {
     case GT_FLOAT: launchSample<GT_FLOAT>(g, B.L, xyz, world, B.idx2, B.begin, B.end, B.rec, wantNN, wantVal, gm, B.spill, B.spillCount, s); break;
     case GT_FP4: launchSample<GT_FP4>(g, B.L, xyz, world, B.idx2, B.begin, B.end, B.rec, wantNN, wantVal, gm, B.spill, B.spillCount, s); break;
     case GT_FP8: launchSample<GT_FP8>(g, B.L, xyz, world, B.idx2, B.begin, B.end, B.rec, wantNN, wantVal, gm, B.spill, B.spillCount, s); break;
@@ -535,9 +1017,15 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
     case GT_ONINDEX: launchSample<GT_ONINDEX>(g, B.L, xyz, world, B.idx2, B.begin, B.end, B.rec, wantNN, wantVal, gm, B.spill, B.spillCount, s); break;
     default: st = -3;
     }
-    if (st == 0)
+    if (st == 0 && B.chunks) {
+        ProfScope prof("k_bin_finish", s);
+        hipLaunchKernelGGL(k_bin_finish, dim3(std::min<uint32_t>(B.chunks, 4096u)), dim3(kThreads), 0, s, g, B.T, xyz, n,
+                           B.chunks, world, wantNN, wantVal, gm, sampleConsts(g), B.cflags, B.rec, nn, val, grad);
+    } else if (st == 0) {
+        ProfScope prof("k_unpack", s);
         hipLaunchKernelGGL(k_unpack, dim3(std::min<uint32_t>((n + kThreads - 1) / kThreads, 1u << 14)), dim3(kThreads), 0,
                            s, B.rec, n, nn, val, grad);
+    }
     if (hipGetLastError() != hipSuccess) st = -4;
     if (hipFreeAsync(B.base, s) != hipSuccess) st = -5;
     return st;
