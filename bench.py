@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--no-diff", action="store_true", help="skip the differentiable-march workload")
     p.add_argument("--no-multi", action="store_true", help="skip the quantised multi-grid workload")
     p.add_argument("--no-build", action="store_true", help="skip the device-side grid construction workload")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the point-query adjoint timing")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="also run one elementwise copy of MIB MiB (reads and writes MIB MiB) to scale HBM counters")
     return p.parse_args()
@@ -218,6 +219,35 @@ def diff_march(torch, nvq, steps, cpu=True):
                                           "touch (2144 B each)")}
     if cpu:
         out["cpu_baseline"], out["grad_error"] = c4_cpu_leg(torch, nvq, fp_np, fp, E, D, target)
+    return out
+
+
+def c2_adjoint(torch, nvq, grids, steps):
+    """Point-query adjoint on the C2 batches: forward (nearest + trilinear + FD6) vs forward +
+    nvq_sample_bwd with seeded upstream gradients for every output, per grid."""
+    out = {}
+    for name, grid, pts in grids:
+        n = pts.shape[0]
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(99)
+        gn = torch.randn(n, generator=gen, device="cuda")
+        gv = torch.randn(n, generator=gen, device="cuda")
+        gg = torch.randn((n, 3), generator=gen, device="cuda")
+        accum = torch.zeros(grid.grad_size, dtype=torch.int64, device="cuda")
+
+        def fwd():
+            nvq.ops.sample(grid, pts, grad_mode=1)
+
+        def bwd():
+            accum.zero_()
+            nvq.ops.sample_backward(grid, pts, gn, gv, gg, grad_mode=1, accum=accum)
+
+        wf, _ = timed_launches(torch, [fwd], steps, 1)
+        wb, _ = timed_launches(torch, [fwd, bwd], steps, 1)
+        kb, kn = kernel_ms("k_bin_sample_bwd")
+        out[name] = {"fwd_ms": 1e3 * wf / steps, "fwd_bwd_ms": 1e3 * wb / steps, "ratio": wb / wf,
+                     "bin_bwd_kernel_ms": kb / max(kn, 1), "values_touched": int((accum != 0).sum().item())}
+        del accum
     return out
 
 
@@ -734,6 +764,12 @@ def main():
                                            "of the whole nvq_sample call (all its kernels)"},
              "c2_samples_per_s": {"torus": n / (1e-3 * call_ms[0]), "fog": n / (1e-3 * call_ms[1])},
              "fog_grid": {"bytes": int(fog_np.nbytes), "leaves": fog.leaf_count}}
+
+    if not args.no_adjoint:
+        extra["c2_adjoint"] = c2_adjoint(torch, nvq, [("torus", torus, p_torus), ("fog", fog, p_fog)], 3)
+        log("C2 adjoint: " + ", ".join(f"{k} fwd {v['fwd_ms']:.2f} ms, fwd+bwd {v['fwd_bwd_ms']:.2f} ms "
+                                        f"(ratio {v['ratio']:.2f}, k_bin_sample_bwd {v['bin_bwd_kernel_ms']:.2f} ms)"
+                                        for k, v in extra["c2_adjoint"].items()))
 
     if not args.no_extra:
         # C1: level-set ray hit, 512x512 camera at (0,0,400) FOV 45, sphere R=100

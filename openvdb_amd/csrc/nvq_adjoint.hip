@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const floa
         W.moveTo(acc, g, int32_t(fx), int32_t(fy), int32_t(fz), sink);
         const float u = x0 - fx, v = x1 - fy, w = x2 - fz;
         const float rho = trilerp(W.c, u, v, w);
-        const float a = expf(-sigma * rho * dt);
+        const float a = stepAbsorption(sigma, rho, dt);
         L += T * (1.0f - a) * rho;        // same operations as the forward march
         const float R = Ltot - L;         // radiance of the later steps
         const float gk = gL * (T * ((1.0f - a) + rho * sdt * a) - sdt * R) - tTerm;

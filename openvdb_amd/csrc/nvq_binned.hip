@@ -863,11 +863,12 @@ void launchClassify(const GridView& g, const Bins& B, const SlotTable& T, float*
                        B.cflags, B.counts);
 }
 
-// Allocate, key, sort and bound the bins of n points.  With `cls` (the forward; requires
-// withRecords) the queries that touch no leaf are answered by the classify pass and only the
-// others are sorted: the call then waits for the stream once, to size the sort.  0, or -1 (no
-// layout), -2 (allocation), -3 (grid type), -4 (sort or HIP error: nothing was launched that reads
-// the sorted arrays; the scratch is already freed).
+// Allocate, key, sort and bound the bins of n points.  With `cls`, only the queries whose block
+// touches a leaf (marked by the classify pass, whose slot table and marks stay in B) are sorted;
+// the forward answers the others in k_bin_finish, the adjoint of a Float grid drops them.  The call
+// then waits for the stream once, to size the sort.  0, or -1 (no layout), -2 (allocation), -3
+// (grid type), -4 (sort or HIP error: nothing was launched that reads the sorted arrays; the
+// scratch is already freed).
 int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool withRecords, Bins& B, hipStream_t s,
               const Classify* cls = nullptr)
 {
@@ -1041,7 +1042,10 @@ int sampleBackward(const GridView& g, const float* xyz, uint64_t n64, int world,
     uint32_t over = 0;
     if (sampleBinnedSupported(g, n64) && !forceDirectSampling()) {
         Bins B;
-        const int bs = binPoints(g, xyz, uint32_t(n64), world, false, B, s);
+        // Float grids: a query whose block touches no leaf reads only tile and background values,
+        // which receive no gradient, so only the queries the classify pass marks are sorted
+        const Classify cls{false, false, gm};
+        const int bs = binPoints(g, xyz, uint32_t(n64), world, false, B, s, g.type == GT_FLOAT ? &cls : nullptr);
         if (bs) return bs;
         if (g.type == GT_FLOAT) launchSampleBwd<GT_FLOAT>(g, B, xyz, world, gnn, gval, ggrad, gm, scale, acc, s);
         else launchSampleBwd<GT_ONINDEX>(g, B, xyz, world, gnn, gval, ggrad, gm, scale, acc, s);

@@ -73,7 +73,7 @@ __device__ __forceinline__ void fogMarchRay(const GridView& g, const float* __re
         const float fx = floorf(x0), fy = floorf(x1), fz = floorf(x2);
         W.moveTo(acc, g, int32_t(fx), int32_t(fy), int32_t(fz), none);
         const float rho = trilerp(W.c, x0 - fx, x1 - fy, x2 - fz);
-        const float a = expf(-sigma * rho * dt);
+        const float a = stepAbsorption(sigma, rho, dt);
         L += T * (1.0f - a) * rho;
         T *= a;
     });
