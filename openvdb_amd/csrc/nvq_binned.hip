@@ -13,7 +13,7 @@
 //                    size the sort
 //   radix sort     : (key, point) pairs of the marked queries (rocPRIM/hipCUB DeviceRadixSort)
 //   k_bin_bounds   : [begin, end) of every non-empty bin in the sorted order
-//   k_bin_sample   : one workgroup per bin stages the 20^3 brick of voxel values around the bin in
+//   k_bin_sample   : one 512-thread workgroup per bin stages the 20^3 brick of voxel values around the bin in
 //                    LDS (ReadAccessor::getValue semantics, so results are bit-identical) and
 //                    evaluates the bin's queries from it; each query's results go to its original
 //                    index as one 64-byte record, written by the four lanes of a quad
@@ -38,7 +38,7 @@ namespace nvq {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kBinThreads = 1024;       // one bin per workgroup: 16 waves share the 32 KB brick
+constexpr int kBinThreads = 512;        // one bin per workgroup: 8 waves share the 32 KB brick
 constexpr int kBinLog2 = 4;
 constexpr int kBin = 1 << kBinLog2;     // bin edge in voxels
 constexpr int kBrick = kBin + 4;        // brick covers [bin-1, bin+kBin+2]
@@ -570,6 +570,17 @@ __global__ __launch_bounds__(kThreads) void k_unpack(const Rec* __restrict__ rec
     }
 }
 
+// One 12-byte store per gradient, so that a wavefront's store covers one contiguous run
+__device__ __forceinline__ void storeGrad(float* __restrict__ grad, uint64_t i, float x, float y, float z)
+{
+    typedef float v3f __attribute__((ext_vector_type(3)));
+    v3f g3;
+    g3.x = x;
+    g3.y = y;
+    g3.z = z;
+    __builtin_memcpy(grad + 3 * i, &g3, 12);
+}
+
 // Outputs of a classified call, in the classify pass's chunk order: a marked query copies its record,
 // the others are answered from their slot values (sampleSlots, bit-identical to the lane accessor).
 __global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T, const float* __restrict__ xyz,
@@ -597,11 +608,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T
             if (!((flags >> j) & 1u)) continue;
             if (nn) nn[i] = ra[j].x;
             if (val) val[i] = ra[j].y;
-            if (grad) {
-                grad[3 * i] = ra[j].z;
-                grad[3 * i + 1] = ra[j].w;
-                grad[3 * i + 2] = rb[j];
-            }
+            if (grad) storeGrad(grad, i, ra[j].z, ra[j].w, rb[j]);
         }
         for (int j = 0; j < kClassifyItems; ++j) {
             const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
@@ -621,11 +628,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T
             }
             if (nn) nn[i] = o[0];
             if (val) val[i] = o[1];
-            if (grad) {
-                grad[3 * i] = o[2];
-                grad[3 * i + 1] = o[3];
-                grad[3 * i + 2] = o[4];
-            }
+            if (grad) storeGrad(grad, i, o[2], o[3], o[4]);
         }
     }
 }
