@@ -677,25 +677,45 @@ __device__ __forceinline__ bool isoCrossingCore(Accessor<GT>& acc, const GridVie
     HDDA h;
     h.init(ray, ray.t0, ray.t1, int32_t(p0.dim));
     const float v0 = p0.value - iso;
+    // The reference nests a voxel-by-voxel loop through the narrow band inside the node-level
+    // loop.  Here both are one loop with a mode flag: every lane performs the same sequence of
+    // HDDA steps and probes as the nested form (a step that fails ends the march in both, since
+    // HDDA time only grows), but a wavefront no longer runs one lane's band walk while the others
+    // wait at their own outer step, which made rays at the silhouette cost the sum of their
+    // lanes' walks.
+    bool band = false;
     while (h.stepOnce()) {
-        rayAt(ray, h.t0 + Delta, p);
-        c[0] = ifloor(p[0]);
-        c[1] = ifloor(p[1]);
-        c[2] = ifloor(p[2]);
-        const Probe q = acc.probe(c[0], c[1], c[2]);
-        h.update(ray, int32_t(q.dim));
-        if (h.dim > 1 || !q.active) continue;
-        while (h.stepOnce()) { // in the narrow band
-            const Probe r = acc.probe(h.vx, h.vy, h.vz);
-            if (!r.active) break;
-            v = r.value - iso;
+        int32_t x, y, z;
+        if (band) {
+            x = h.vx;
+            y = h.vy;
+            z = h.vz;
+        } else {
+            rayAt(ray, h.t0 + Delta, p);
+            x = ifloor(p[0]);
+            y = ifloor(p[1]);
+            z = ifloor(p[2]);
+        }
+        const Probe q = acc.probe(x, y, z);
+        if (band) { // in the narrow band
+            if (!q.active) {
+                band = false;
+                continue;
+            }
+            v = q.value - iso;
             if (v * v0 < 0) {
-                c[0] = h.vx;
-                c[1] = h.vy;
-                c[2] = h.vz;
+                c[0] = x;
+                c[1] = y;
+                c[2] = z;
                 t = h.t0;
                 return true;
             }
+        } else {
+            c[0] = x;
+            c[1] = y;
+            c[2] = z;
+            h.update(ray, int32_t(q.dim));
+            band = !(h.dim > 1) && q.active;
         }
     }
     return false;
