@@ -43,14 +43,15 @@ __global__ void k_clear_flag(uint32_t* flag) { *flag = 0u; }
 
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_fog_march_bwd(GridView g, const float* __restrict__ eye,
-                                                          const float* __restrict__ dir, Sched s, float dt, float sigma,
+                                                          const float* __restrict__ dir, Sched s,
+                                                          const uint32_t* __restrict__ order, float dt, float sigma,
                                                           const float* __restrict__ Lfwd, const float* __restrict__ Tfwd,
                                                           const float* __restrict__ gLin, const float* __restrict__ gTin,
                                                           unsigned long long* __restrict__ accum, float scale,
                                                           uint32_t* __restrict__ overflow)
 {
     uint64_t i;
-    if (!schedule(s, i)) return;
+    if (!scheduleOrdered(s, order, i)) return;
     const float gL = gLin ? gLin[i] : 0.0f, gT = gTin ? gTin[i] : 0.0f;
     if (gL == 0.0f && gT == 0.0f) return;
     Accessor<GT> acc(g);
@@ -108,19 +109,25 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
 {
     const Sched sc = makeSched(n, width);
     const float scale = ldexpf(1.0f, shift);
+    TileOrder to;
+    if (tileOrder(g, eye, dir, sc, to, s) != 0) return -1;
     uint32_t* flag = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) return -1;
+    if (hipMallocAsync(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) {
+        releaseTileOrder(to, s);
+        return -1;
+    }
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flag);
     auto kernel = g.type == GT_ONINDEX ? k_fog_march_bwd<GT_ONINDEX> : k_fog_march_bwd<GT_FLOAT>;
     {
         ProfScope prof("k_fog_march_bwd", s);
-        hipLaunchKernelGGL(kernel, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T, gL,
-                           gT, reinterpret_cast<unsigned long long*>(accum), scale, flag);
+        hipLaunchKernelGGL(kernel, dim3(orderedBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, to.order, dt, sigma,
+                           L, T, gL, gT, reinterpret_cast<unsigned long long*>(accum), scale, flag);
     }
     bool ok = hipGetLastError() == hipSuccess;
     uint32_t over = 0;
     ok = hipMemcpyAsync(&over, flag, sizeof(over), hipMemcpyDeviceToHost, s) == hipSuccess && ok;
     ok = hipFreeAsync(flag, s) == hipSuccess && ok;
+    ok = releaseTileOrder(to, s) == 0 && ok;
     ok = hipStreamSynchronize(s) == hipSuccess && ok;
     if (!ok) return -1;
     return over ? -2 : 0;

@@ -9,9 +9,13 @@
 namespace nvq {
 
 // ------------------------------------------------------------------ ray-tile scheduling
-// Rays of a row-major image are processed in 8x8 pixel tiles (one tile per 64-wide wavefront);
-// workgroups are remapped so each XCD (hardware dispatch is round-robin over the 8 XCDs) walks a
-// contiguous band of tiles and keeps its own L2 warm with the nodes that band touches.
+// Rays of a row-major image are processed in 8x8 pixel tiles (one tile per 64-wide wavefront).
+// Hardware dispatch is round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each with its
+// own L2.  Workgroups are remapped so that the image is cut into bands of whole tile rows, 16
+// bands per XCD (at least one row each), and band k runs on XCD k % 8: the tiles of a band share
+// an L2, and every XCD gets a share of each part of the image.  With one contiguous band per
+// XCD, the XCDs holding the rows through the middle of a volume worked on long after the others
+// had run out of tiles.  Linear batches keep one contiguous range per XCD.
 constexpr int kBlock = 256;
 
 struct Sched {
@@ -19,7 +23,8 @@ struct Sched {
     uint32_t width, height, tilesX;
     uint64_t tiles;
     uint32_t blocks;  // logical blocks
-    uint32_t perXcd;  // logical blocks per XCD
+    uint32_t perXcd;  // workgroups launched per XCD
+    uint32_t group;   // consecutive logical blocks (a band) given to one XCD
 };
 
 inline Sched makeSched(uint64_t n, uint32_t width)
@@ -32,17 +37,23 @@ inline Sched makeSched(uint64_t n, uint32_t width)
         s.tilesX = (width + 7) / 8;
         s.tiles = uint64_t(s.tilesX) * ((s.height + 7) / 8);
         s.blocks = uint32_t((s.tiles + (kBlock / 64) - 1) / (kBlock / 64));
+        const uint32_t row = (s.tilesX + (kBlock / 64) - 1) / (kBlock / 64); // workgroups per tile row
+        const uint32_t rows = s.blocks / (8u * 16u) / row;
+        s.group = (rows > 1 ? rows : 1) * row;
     } else {
         s.blocks = uint32_t((n + kBlock - 1) / kBlock);
+        s.group = (s.blocks + 7) / 8;
+        s.tiles = (n + 63) / 64;
     }
-    s.perXcd = (s.blocks + 7) / 8;
+    if (s.group == 0) s.group = 1;
+    s.perXcd = (s.blocks + 8 * s.group - 1) / (8 * s.group) * s.group;
     return s;
 }
 
 __device__ __forceinline__ bool schedule(const Sched& s, uint64_t& idx)
 {
-    const uint32_t b = blockIdx.x;
-    const uint32_t lb = (b & 7u) * s.perXcd + (b >> 3);
+    const uint32_t b = blockIdx.x, x = b & 7u, q = b >> 3;
+    const uint32_t lb = ((q / s.group) * 8u + x) * s.group + q % s.group;
     if (lb >= s.blocks) return false;
     if (s.width == 0) {
         idx = uint64_t(lb) * kBlock + threadIdx.x;
@@ -59,6 +70,45 @@ __device__ __forceinline__ bool schedule(const Sched& s, uint64_t& idx)
 }
 
 inline uint32_t launchBlocks(const Sched& s) { return s.perXcd * 8u; }
+
+// Ray `lane` of tile `tile`: 8x8 pixels of a row-major image, or 64 consecutive rays of a linear
+// batch.
+__device__ __forceinline__ bool tileRay(const Sched& s, uint64_t tile, uint32_t lane, uint64_t& idx)
+{
+    if (s.width == 0) {
+        idx = tile * 64u + lane;
+        return idx < s.n;
+    }
+    const uint32_t tx = uint32_t(tile % s.tilesX), ty = uint32_t(tile / s.tilesX);
+    const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
+    if (px >= s.width || py >= s.height) return false;
+    idx = uint64_t(py) * s.width + px;
+    return true;
+}
+
+// Longest-first tile schedule of the fog march and its adjoint.  A march costs about its ray's
+// length inside the grid's bbox, and tiles of a camera image differ several-fold in cost (rays
+// through the middle of a volume against rays that graze it).  Dispatched in image order, the
+// costly tiles of the last rows started late and ran alone at the end of the launch.  Taking the
+// tiles in descending order of their longest clipped ray instead, wavefront k of the launch runs
+// tile order[k]: the launch ends with its cheapest tiles, and consecutive workgroups (one per
+// XCD in turn) get tiles of equal cost.
+__device__ __forceinline__ bool scheduleOrdered(const Sched& s, const uint32_t* __restrict__ order, uint64_t& idx)
+{
+    const uint64_t k = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+    if (k >= s.tiles) return false;
+    return tileRay(s, order[k], threadIdx.x & 63u, idx);
+}
+inline uint32_t orderedBlocks(const Sched& s) { return uint32_t((s.tiles + kBlock / 64 - 1) / (kBlock / 64)); }
+
+// Stream-ordered scratch holding the order; releaseTileOrder frees it after the launches queued
+// on the stream so far.
+struct TileOrder {
+    void* base = nullptr;
+    const uint32_t* order = nullptr;
+};
+int tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st);
+int releaseTileOrder(TileOrder& o, hipStream_t st);
 
 
 // Large-batch point sampling through spatial bins and LDS bricks (nvq_binned.hip).

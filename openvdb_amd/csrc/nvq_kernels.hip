@@ -179,12 +179,72 @@ __global__ __launch_bounds__(kBlock) void k_zero_crossing(GridView g, const floa
 
 template<uint32_t GT>
 __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* __restrict__ eye,
-                                                      const float* __restrict__ dir, Sched s, float dt, float sigma,
+                                                      const float* __restrict__ dir, Sched s,
+                                                      const uint32_t* __restrict__ order, float dt, float sigma,
                                                       float* __restrict__ Lout, float* __restrict__ Tout)
 {
     uint64_t i;
-    if (!schedule(s, i)) return;
+    if (!scheduleOrdered(s, order, i)) return;
     fogMarchRay<GT>(g, eye, dir, i, dt, sigma, Lout, Tout);
+}
+
+// ------------------------------------------------------------------ longest-first tile order
+// Cost key of a tile: its longest ray inside the grid's bbox (index-space length, the march's
+// own clip), as the top 12 bits of the float (exponent and 4 mantissa bits: 6 % classes),
+// inverted so that key 0 is the costliest; rays that miss the bbox count as length 0.
+constexpr uint32_t kOrderKeys = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_tile_cost(GridView g, const float* __restrict__ eye,
+                                                      const float* __restrict__ dir, Sched s,
+                                                      uint32_t* __restrict__ key)
+{
+    const uint64_t tile = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+    if (tile >= s.tiles) return; // whole wavefronts leave together
+    uint64_t i;
+    float len = 0.0f;
+    if (tileRay(s, tile, threadIdx.x & 63u, i)) {
+        const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
+        const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+        Ray R = worldToIndexRay(g, e, d);
+        if (rayClip(R, g.bbox) && double(R.t1) <= 1e20 && R.t1 > R.t0) len = R.t1 - R.t0;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) len = fmaxf(len, __shfl_xor(len, m));
+    if ((threadIdx.x & 63u) == 0) key[tile] = kOrderKeys - 1u - (__float_as_uint(len) >> 19);
+}
+
+// One workgroup: counting sort of the tile keys (histogram and scan in LDS), tiles of one key in
+// the order the workgroup's atomics hand out their slots.
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ key, uint64_t tiles,
+                                                              uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t count[kOrderKeys];
+    __shared__ uint32_t part[kOrderThreads];
+    constexpr int per = kOrderKeys / kOrderThreads;
+    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) count[k] = 0;
+    __syncthreads();
+    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) atomicAdd(&count[key[t]], 1u);
+    __syncthreads();
+    uint32_t local[per], sum = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        local[j] = sum;
+        sum += count[threadIdx.x * per + j];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kOrderThreads; off <<= 1) { // inclusive scan of the per-thread sums
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t base = part[threadIdx.x] - sum;
+#pragma unroll
+    for (int j = 0; j < per; ++j) count[threadIdx.x * per + j] = base + local[j];
+    __syncthreads();
+    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) order[atomicAdd(&count[key[t]], 1u)] = uint32_t(t);
 }
 
 // ------------------------------------------------------------------ host helpers
@@ -245,11 +305,12 @@ template<uint32_t GT> struct RunZeroCrossing {
     }
 };
 template<uint32_t GT> struct RunFogMarch {
-    static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, float dt, float sigma,
-                    float* L, float* T, hipStream_t s)
+    static void run(const GridView& g, const float* eye, const float* dir, const Sched& sc, const uint32_t* order,
+                    float dt, float sigma, float* L, float* T, hipStream_t s)
     {
         ProfScope prof("k_fog_march", s);
-        hipLaunchKernelGGL(k_fog_march<GT>, dim3(launchBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, dt, sigma, L, T);
+        hipLaunchKernelGGL(k_fog_march<GT>, dim3(orderedBlocks(sc)), dim3(kBlock), 0, s, g, eye, dir, sc, order, dt,
+                           sigma, L, T);
     }
 };
 
@@ -323,6 +384,37 @@ int validate(const uint8_t* b, uint64_t bytes)
 }
 
 } // namespace
+
+int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st)
+{
+    out = TileOrder{};
+    if (s.tiles == 0 || s.tiles > (uint64_t(1) << 32)) return -1;
+    const size_t keyBytes = (size_t(s.tiles) * 4 + 255) & ~size_t(255);
+    if (hipMallocAsync(&out.base, keyBytes + size_t(s.tiles) * 4, st) != hipSuccess) {
+        out.base = nullptr;
+        return -1;
+    }
+    uint32_t* key = static_cast<uint32_t*>(out.base);
+    uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(out.base) + keyBytes);
+    {
+        ProfScope prof("k_tile_order", st);
+        hipLaunchKernelGGL(k_tile_cost, dim3(orderedBlocks(s)), dim3(kBlock), 0, st, g, eye, dir, s, key);
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, key, s.tiles, order);
+    }
+    out.order = order;
+    if (hipGetLastError() != hipSuccess) {
+        releaseTileOrder(out, st);
+        return -1;
+    }
+    return 0;
+}
+
+int nvq::releaseTileOrder(TileOrder& o, hipStream_t st)
+{
+    const hipError_t e = o.base ? hipFreeAsync(o.base, st) : hipSuccess;
+    o = TileOrder{};
+    return e == hipSuccess ? 0 : -1;
+}
 
 // ======================================================================= C ABI
 bool nvq::forceDirectSampling()
@@ -572,8 +664,12 @@ int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint
     if (n == 0) return NVQ_OK;
     if (n > (uint64_t(1) << 38)) return NVQ_ERR_INVALID_ARG;
     const Sched sc = makeSched(n, image_width);
-    return launchTyped<RunFogMarch>(grid->view.type, grid->view, eye, dir, sc, dt, sigma_t, radiance, transmittance,
-                                    S(stream));
+    TileOrder to;
+    if (tileOrder(grid->view, eye, dir, sc, to, S(stream)) != 0) return NVQ_ERR_HIP;
+    const int st = launchTyped<RunFogMarch>(grid->view.type, grid->view, eye, dir, sc, to.order, dt, sigma_t, radiance,
+                                            transmittance, S(stream));
+    const int fr = releaseTileOrder(to, S(stream));
+    return st != NVQ_OK ? st : (fr != 0 ? NVQ_ERR_HIP : NVQ_OK);
 }
 
 int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,
