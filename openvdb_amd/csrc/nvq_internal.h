@@ -101,6 +101,64 @@ __device__ __forceinline__ bool scheduleOrdered(const Sched& s, const uint32_t* 
 }
 inline uint32_t orderedBlocks(const Sched& s) { return uint32_t((s.tiles + kBlock / 64 - 1) / (kBlock / 64)); }
 
+// Cost keys of tiles: the top 12 bits of a non-negative float cost (exponent and 4 mantissa
+// bits: 6 % classes), inverted so that key 0 is the costliest; cost 0 gets the last key.
+constexpr uint32_t kOrderKeys = 4096;
+__device__ __forceinline__ uint32_t costKey(float cost) { return kOrderKeys - 1u - (__float_as_uint(cost) >> 19); }
+
+// Index-space length of a world ray inside the grid's bbox (the march's own map and clip); 0 when
+// it misses.
+__device__ __forceinline__ float clippedLength(const GridView& g, const float* e, const float* d)
+{
+    const float we[3] = {e[0], e[1], e[2]}, wd[3] = {d[0], d[1], d[2]};
+    Ray R = worldToIndexRay(g, we, wd);
+    return rayClip(R, g.bbox) && double(R.t1) <= 1e20 && R.t1 > R.t0 ? R.t1 - R.t0 : 0.0f;
+}
+
+__device__ __forceinline__ float waveMax(float x)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = fmaxf(x, __shfl_xor(x, m));
+    return x;
+}
+
+constexpr int kOrderThreads = 1024;
+namespace {
+// One workgroup: counting sort of the tile keys (histogram and scan in LDS), tiles of one key in
+// the order the workgroup's atomics hand out their slots.
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ key, uint64_t tiles,
+                                                              uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t count[kOrderKeys];
+    __shared__ uint32_t part[kOrderThreads];
+    constexpr int per = kOrderKeys / kOrderThreads;
+    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) count[k] = 0;
+    __syncthreads();
+    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) atomicAdd(&count[key[t]], 1u);
+    __syncthreads();
+    uint32_t local[per], sum = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        local[j] = sum;
+        sum += count[threadIdx.x * per + j];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kOrderThreads; off <<= 1) { // inclusive scan of the per-thread sums
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t base = part[threadIdx.x] - sum;
+#pragma unroll
+    for (int j = 0; j < per; ++j) count[threadIdx.x * per + j] = base + local[j];
+    __syncthreads();
+    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) order[atomicAdd(&count[key[t]], 1u)] = uint32_t(t);
+}
+
+} // namespace
+
 // Stream-ordered scratch holding the order; releaseTileOrder frees it after the launches queued
 // on the stream so far.
 struct TileOrder {

@@ -189,11 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* _
 }
 
 // ------------------------------------------------------------------ longest-first tile order
-// Cost key of a tile: its longest ray inside the grid's bbox (index-space length, the march's
-// own clip), as the top 12 bits of the float (exponent and 4 mantissa bits: 6 % classes),
-// inverted so that key 0 is the costliest; rays that miss the bbox count as length 0.
-constexpr uint32_t kOrderKeys = 4096;
-
+// Cost of a tile: its longest ray inside the grid's bbox.
 __global__ __launch_bounds__(kBlock) void k_tile_cost(GridView g, const float* __restrict__ eye,
                                                       const float* __restrict__ dir, Sched s,
                                                       uint32_t* __restrict__ key)
@@ -202,49 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_cost(GridView g, const float* _
     if (tile >= s.tiles) return; // whole wavefronts leave together
     uint64_t i;
     float len = 0.0f;
-    if (tileRay(s, tile, threadIdx.x & 63u, i)) {
-        const float e[3] = {eye[3 * i], eye[3 * i + 1], eye[3 * i + 2]};
-        const float d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
-        Ray R = worldToIndexRay(g, e, d);
-        if (rayClip(R, g.bbox) && double(R.t1) <= 1e20 && R.t1 > R.t0) len = R.t1 - R.t0;
-    }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) len = fmaxf(len, __shfl_xor(len, m));
-    if ((threadIdx.x & 63u) == 0) key[tile] = kOrderKeys - 1u - (__float_as_uint(len) >> 19);
-}
-
-// One workgroup: counting sort of the tile keys (histogram and scan in LDS), tiles of one key in
-// the order the workgroup's atomics hand out their slots.
-constexpr int kOrderThreads = 1024;
-__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ key, uint64_t tiles,
-                                                              uint32_t* __restrict__ order)
-{
-    __shared__ uint32_t count[kOrderKeys];
-    __shared__ uint32_t part[kOrderThreads];
-    constexpr int per = kOrderKeys / kOrderThreads;
-    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) count[k] = 0;
-    __syncthreads();
-    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) atomicAdd(&count[key[t]], 1u);
-    __syncthreads();
-    uint32_t local[per], sum = 0;
-#pragma unroll
-    for (int j = 0; j < per; ++j) {
-        local[j] = sum;
-        sum += count[threadIdx.x * per + j];
-    }
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < kOrderThreads; off <<= 1) { // inclusive scan of the per-thread sums
-        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    const uint32_t base = part[threadIdx.x] - sum;
-#pragma unroll
-    for (int j = 0; j < per; ++j) count[threadIdx.x * per + j] = base + local[j];
-    __syncthreads();
-    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) order[atomicAdd(&count[key[t]], 1u)] = uint32_t(t);
+    if (tileRay(s, tile, threadIdx.x & 63u, i)) len = clippedLength(g, eye + 3 * i, dir + 3 * i);
+    len = waveMax(len);
+    if ((threadIdx.x & 63u) == 0) key[tile] = costKey(len);
 }
 
 // ------------------------------------------------------------------ host helpers

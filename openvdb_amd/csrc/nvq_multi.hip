@@ -1,7 +1,8 @@
 // Persistent multi-grid ray query: one launch answers the rays of up to kMaxGrids grids of
 // mixed leaf codecs and classes.  Wavefronts pull 8x8 ray tiles from a device work queue
-// (forEachQueuedTile: chunks interleaved over one counter per XCD) until every grid's tiles are
-// taken (the persistent-warp pattern of the reference's iso-surface renderer,
+// (forEachQueuedTile: chunks interleaved over one counter per XCD), costliest tiles first (a
+// per-tile estimate, counting-sorted before the launch; the no-allocation variant keeps grid
+// and image order), until every grid's tiles are taken (the persistent-warp pattern of the reference's iso-surface renderer,
 // examples/ex_raytrace_iso_surface/common.h:79-110, 166-189), so small and large grids share
 // the machine without per-grid launches or tail effects.  Level-set
 // grids run zeroCrossing, fog volumes run the emission-absorption march.
@@ -33,12 +34,14 @@ __device__ __forceinline__ void runRay(const GridView& g, uint32_t op, const flo
 
 __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float* __restrict__ eye,
                                                         const float* __restrict__ dir, float dt, float sigma,
-                                                        uint32_t* __restrict__ counters, uint8_t* __restrict__ hit,
+                                                        uint32_t* __restrict__ counters,
+                                                        const uint32_t* __restrict__ order, uint8_t* __restrict__ hit,
                                                         int32_t* __restrict__ ijk, float* __restrict__ a,
                                                         float* __restrict__ b, float* __restrict__ normal)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    forEachQueuedTile(counters, A.totalTiles, [&](uint32_t t) {
+    forEachQueuedTile(counters, A.totalTiles, [&](uint32_t q) {
+        const uint32_t t = order ? order[q] : q;
         uint32_t j = 0;
         while (j + 1 < A.k && A.slot[j + 1].tileBase <= t) ++j;
         const MultiSlot& S = A.slot[j];
@@ -59,6 +62,31 @@ __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float
             }
         }
     });
+}
+
+// Cost of a tile of the multi-grid launch for the longest-first queue order: its longest ray
+// inside its grid's bbox, a sixteenth of that for zeroCrossing.  A march steps every half voxel
+// with a trilinear sample per step; zeroCrossing moves by nodes and leaves to the surface and
+// stops at the first crossing, at far fewer probes per unit length.
+__global__ __launch_bounds__(kBlock) void k_multi_tile_cost(MultiArgs A, const float* __restrict__ eye,
+                                                            const float* __restrict__ dir, uint32_t* __restrict__ key)
+{
+    const uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (t >= A.totalTiles) return; // whole wavefronts leave together
+    uint32_t j = 0;
+    while (j + 1 < A.k && A.slot[j + 1].tileBase <= t) ++j;
+    const MultiSlot& S = A.slot[j];
+    const uint32_t local = t - S.tileBase;
+    const uint32_t tx = local % S.tilesX, ty = local / S.tilesX;
+    const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
+    const uint64_t i = S.rayBase + uint64_t(py) * S.width + px;
+    float c = 0.0f;
+    if (px < S.width && py < S.height && i < S.rayEnd) {
+        c = clippedLength(*S.view, eye + 3 * i, dir + 3 * i);
+        if (S.op == kOpZeroCrossing) c *= 0.0625f;
+    }
+    c = waveMax(c);
+    if (lane == 0) key[t] = costKey(c);
 }
 
 constexpr int kMaxDevices = 64;
@@ -100,7 +128,7 @@ int multiQueryNoAlloc(const MultiArgs& A, const float* eye, const float* dir, fl
     // the counters keep the previous call's values: clear them with a kernel (no memset node)
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
     hipLaunchKernelGGL(k_multi_query, dim3(queryBlocks(A, cus)), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter,
-                       hit, ijk, a, b, normal);
+                       nullptr, hit, ijk, a, b, normal);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
@@ -110,14 +138,29 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return -1;
-    uint32_t* counter = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
+    // queue counters, then the tile cost keys and the longest-first order (4 bytes per tile each)
+    const size_t keyBytes = (size_t(A.totalTiles) * 4 + 255) & ~size_t(255);
+    uint8_t* base = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&base), kQueueBytes + keyBytes + size_t(A.totalTiles) * 4, s) !=
+        hipSuccess)
+        return -2;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(base);
+    uint32_t* key = reinterpret_cast<uint32_t*>(base + kQueueBytes);
+    uint32_t* order = reinterpret_cast<uint32_t*>(base + kQueueBytes + keyBytes);
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
-    ProfScope prof("k_multi_query", s);
-    hipLaunchKernelGGL(k_multi_query, dim3(queryBlocks(A, cus)), dim3(kBlock), 0, s, A, eye, dir, dt, sigma, counter,
-                       hit, ijk, a, b, normal);
+    {
+        ProfScope prof("k_tile_order", s);
+        hipLaunchKernelGGL(k_multi_tile_cost, dim3((A.totalTiles + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
+                           s, A, eye, dir, key);
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, key, uint64_t(A.totalTiles), order);
+    }
+    {
+        ProfScope prof("k_multi_query", s);
+        hipLaunchKernelGGL(k_multi_query, dim3(queryBlocks(A, cus)), dim3(kBlock), 0, s, A, eye, dir, dt, sigma,
+                           counter, order, hit, ijk, a, b, normal);
+    }
     const bool ok = hipGetLastError() == hipSuccess;
-    if (hipFreeAsync(counter, s) != hipSuccess) return -3;
+    if (hipFreeAsync(base, s) != hipSuccess) return -3;
     return ok ? 0 : -4;
 }
 
