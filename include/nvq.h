@@ -114,8 +114,10 @@ int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* v
  *          grad_mode 2 = GradStencil::gradient at RoundDown(xyz)  math/Stencils.h:502-507
  *          grad_mode 3 = TrilinearSampler::gradient (index space) math/SampleFromVoxels.h:232-271
  * nn / val / grad may be NULL.  Replaces the nanobind op sampleFromVoxels(points, d_grid, values[, gradients]).
- * Batches of 2^20 points or more are binned through stream-ordered scratch and wait for `stream`
- * once (to size the sort of the points that touch a leaf); smaller batches do not wait. */
+ * Batches of 2^20 points or more are binned through a scratch block the library keeps per
+ * (device, stream), about 120 bytes per point, grown when a larger batch comes and held until
+ * the process ends; they wait for `stream` once (to size the sort of the points that touch a
+ * leaf).  Smaller batches do not wait and take no scratch. */
 int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                int grad_mode, void* stream);
 

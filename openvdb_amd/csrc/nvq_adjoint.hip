@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 namespace nvq {
 namespace {
 
@@ -112,7 +114,7 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
     TileOrder to;
     if (tileOrder(g, eye, dir, sc, to, s) != 0) return -1;
     uint32_t* flag = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) {
+    if (poolMalloc(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) {
         releaseTileOrder(to, s);
         return -1;
     }
@@ -124,12 +126,14 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
                            L, T, gL, gT, reinterpret_cast<unsigned long long*>(accum), scale, flag);
     }
     bool ok = hipGetLastError() == hipSuccess;
-    uint32_t over = 0;
-    ok = hipMemcpyAsync(&over, flag, sizeof(over), hipMemcpyDeviceToHost, s) == hipSuccess && ok;
+    const void* st = stageCopy(flag, sizeof(uint32_t), s);
+    ok = st && ok;
     ok = hipFreeAsync(flag, s) == hipSuccess && ok;
     ok = releaseTileOrder(to, s) == 0 && ok;
     ok = hipStreamSynchronize(s) == hipSuccess && ok;
     if (!ok) return -1;
+    uint32_t over = 0;
+    std::memcpy(&over, st, sizeof(over));
     return over ? -2 : 0;
 }
 

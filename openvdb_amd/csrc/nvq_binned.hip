@@ -29,10 +29,16 @@
 #include "nvq_sample.h"
 
 #include <hip/hip_runtime.h>
+
+#include <cstring>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <utility>
 
 namespace nvq {
 namespace {
@@ -812,13 +818,70 @@ bool makeLayout(const GridView& g, BinLayout& L)
     return true;
 }
 
+// ------------------------------------------------------------------ per-stream workspace
+// The scratch of a binned call (several GiB for 2^26 queries) is leased from one grow-only block
+// per (device, stream) that stays allocated between calls.  Allocating and freeing it per call
+// (stream-ordered) made hipFreeAsync block the host for about 230 ms per call once some tens of
+// such calls had run.  Calls on one stream execute in stream order, so they can share the block;
+// the lease's lock serialises host threads that issue calls on the same stream.  When a call
+// needs more, the old block is freed in stream order and a larger one taken; the last block of
+// each stream is kept until the process ends.
+struct StreamWorkspace {
+    std::mutex use;
+    char* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex gWorkspaceLock;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<StreamWorkspace>> gWorkspaces;
+
+class WorkspaceLease {
+public:
+    WorkspaceLease() = default;
+    WorkspaceLease(const WorkspaceLease&) = delete;
+    WorkspaceLease& operator=(const WorkspaceLease&) = delete;
+    ~WorkspaceLease()
+    {
+        if (w_) w_->use.unlock();
+    }
+    // The stream's block, at least `bytes` long, held until the lease ends; nullptr on failure.
+    char* acquire(size_t bytes, hipStream_t s)
+    {
+        int dev = 0;
+        if (w_ || hipGetDevice(&dev) != hipSuccess) return nullptr;
+        StreamWorkspace* w;
+        {
+            std::lock_guard<std::mutex> lock(gWorkspaceLock);
+            std::unique_ptr<StreamWorkspace>& e = gWorkspaces[std::make_pair(dev, s)];
+            if (!e) e = std::make_unique<StreamWorkspace>();
+            w = e.get();
+        }
+        w->use.lock();
+        w_ = w;
+        if (w->bytes < bytes) {
+            if (w->ptr) (void)hipFreeAsync(w->ptr, s); // after the earlier calls' kernels on s
+            w->ptr = nullptr;
+            w->bytes = 0;
+            const size_t want = bytes + bytes / 8;
+            void* p = nullptr;
+            if (poolMalloc(&p, want, s) != hipSuccess) return nullptr;
+            w->ptr = static_cast<char*>(p);
+            w->bytes = want;
+        }
+        return w->ptr;
+    }
+
+private:
+    StreamWorkspace* w_ = nullptr;
+};
+
 // Scratch of one binned call: keys[2n] | pts[2n] | begin[m] | end[m] | spillCount | flag | spill[n]
 // | forward only: rec[n], classify marks, counts and offsets, scan temporaries | sort temporaries,
-// one stream-ordered allocation.
+// in the stream's workspace.
 struct Bins {
     BinLayout L;
     uint32_t n = 0;
     uint32_t m = 0; // queries in the sort (all n, or those the forward's classify pass kept)
+    WorkspaceLease lease;
     char* base = nullptr;
     uint32_t *keys = nullptr, *keys2 = nullptr, *begin = nullptr, *end = nullptr, *spillCount = nullptr,
              *flag = nullptr, *spill = nullptr;
@@ -900,8 +963,8 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
                  oScan = up(oOffsets + 4ull * (chunks + 1)), oTabV = up(oScan + scanBytes),
                  oTabL = up(oTabV + 4 * slots), oTemp = up(oTabL + 4 * ((slots + 63) / 64) * 2),
                  total = oTemp + up(sortBytes);
-    keepPoolMemory();
-    if (hipMallocAsync(reinterpret_cast<void**>(&B.base), total, s) != hipSuccess) return -2;
+    B.base = B.lease.acquire(total, s);
+    if (!B.base) return -2;
     char* base = B.base;
     B.keys = reinterpret_cast<uint32_t*>(base + oKeys);
     B.keys2 = reinterpret_cast<uint32_t*>(base + oKeys2);
@@ -942,16 +1005,15 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
         case GT_FP16: launchClassify<GT_FP16>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
         case GT_FPN: launchClassify<GT_FPN>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
         case GT_ONINDEX: launchClassify<GT_ONINDEX>(g, B, T, tv, tl, xyz, world, *cls, blocks, s); break;
-        default: (void)hipFreeAsync(B.base, s); return -3;
+        default: return -3;
         }
         uint32_t m = 0;
         if (hipGetLastError() != hipSuccess ||
             hipcub::DeviceScan::ExclusiveSum(B.scanTemp, B.scanBytes, B.counts, B.offsets, chunks + 1, s) !=
                 hipSuccess ||
-            hipMemcpyAsync(&m, B.offsets + chunks, sizeof(m), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess || m > n) {
+            readBack(&m, B.offsets + chunks, sizeof(m), s) != hipSuccess || m > n) {
             (void)hipGetLastError();
-            return hipFreeAsync(B.base, s) == hipSuccess ? -4 : -5;
+            return -4;
         }
         B.m = m;
         if (m)
@@ -970,7 +1032,7 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
     if (sorted != hipSuccess) {
         // the sorted arrays were never written: launch nothing that reads them
         (void)hipGetLastError();
-        return hipFreeAsync(B.base, s) == hipSuccess ? -4 : -5;
+        return -4;
     }
     const uint32_t gridM = std::min<uint32_t>((B.m + kThreads - 1) / kThreads, 1u << 16);
     hipLaunchKernelGGL(k_bin_bounds, dim3(gridM), dim3(kThreads), 0, s, B.keys2, B.m, B.begin, B.end);
@@ -1031,7 +1093,6 @@ int sampleBinned(const GridView& g, const float* xyz, uint64_t n64, int world, f
                            s, B.rec, n, nn, val, grad);
     }
     if (hipGetLastError() != hipSuccess) st = -4;
-    if (hipFreeAsync(B.base, s) != hipSuccess) st = -5;
     return st;
 }
 
@@ -1053,14 +1114,15 @@ int sampleBackward(const GridView& g, const float* xyz, uint64_t n64, int world,
         if (g.type == GT_FLOAT) launchSampleBwd<GT_FLOAT>(g, B, xyz, world, gnn, gval, ggrad, gm, scale, acc, s);
         else launchSampleBwd<GT_ONINDEX>(g, B, xyz, world, gnn, gval, ggrad, gm, scale, acc, s);
         bool ok = hipGetLastError() == hipSuccess;
-        ok = hipMemcpyAsync(&over, B.flag, sizeof(over), hipMemcpyDeviceToHost, s) == hipSuccess && ok;
-        ok = hipFreeAsync(B.base, s) == hipSuccess && ok;
+        const void* st = stageCopy(B.flag, sizeof(over), s);
+        ok = st && ok;
         ok = hipStreamSynchronize(s) == hipSuccess && ok;
         if (!ok) return -4;
+        std::memcpy(&over, st, sizeof(over));
         return over ? -6 : 0;
     }
     uint32_t* flag = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) return -2;
+    if (poolMalloc(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) return -2;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kThreads), 0, s, flag, 1u, 0u);
     const SampleConsts k = sampleConsts(g);
     const uint32_t blocks = uint32_t(std::min<uint64_t>((n64 + kThreads - 1) / kThreads, 1u << 20));
@@ -1074,10 +1136,12 @@ int sampleBackward(const GridView& g, const float* xyz, uint64_t n64, int world,
                                gval, ggrad, gm, k, scale, acc, flag);
     }
     bool ok = hipGetLastError() == hipSuccess;
-    ok = hipMemcpyAsync(&over, flag, sizeof(over), hipMemcpyDeviceToHost, s) == hipSuccess && ok;
+    const void* st = stageCopy(flag, sizeof(over), s);
+    ok = st && ok;
     ok = hipFreeAsync(flag, s) == hipSuccess && ok;
     ok = hipStreamSynchronize(s) == hipSuccess && ok;
     if (!ok) return -4;
+    std::memcpy(&over, st, sizeof(over));
     return over ? -6 : 0;
 }
 

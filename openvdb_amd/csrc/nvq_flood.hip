@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 using namespace nvq;
 
 namespace {
@@ -191,8 +193,7 @@ int nvq_signed_flood_fill(nvq_grid* grid, void* stream)
     // RootData::mBackground, read by the kernels on the device
     const float* bg = reinterpret_cast<const float*>(root + kRootBackground);
     int* missing = nullptr;
-    keepPoolMemory(); // stream-ordered pool: no device-wide synchronisation per call
-    if (hipMallocAsync(reinterpret_cast<void**>(&missing), sizeof(int), s) != hipSuccess) return NVQ_ERR_HIP;
+    if (poolMalloc(reinterpret_cast<void**>(&missing), sizeof(int), s) != hipSuccess) return NVQ_ERR_HIP;
     k_sff_flag_clear<<<1, 64, 0, s>>>(missing);
     if (L.count[0]) k_sff_leaf<<<(L.count[0] + 3u) / 4u, 256, 0, s>>>(tree + L.nodeOff[0], L.count[0], bg);
     if (L.count[1]) k_sff_node<4><<<uint32_t((uint64_t(L.count[1]) * 4096u + 255u) / 256u), 256, 0, s>>>(tree + L.nodeOff[1], L.count[1], bg);
@@ -200,10 +201,12 @@ int nvq_signed_flood_fill(nvq_grid* grid, void* stream)
     if (L.tableSize) k_sff_root<<<(L.tableSize + 255u) / 256u, 256, 0, s>>>(root, L.tableSize, bg, missing);
     int flag = 0;
     int rc = NVQ_ERR_HIP;
-    const bool launched = hipGetLastError() == hipSuccess &&
-                          hipMemcpyAsync(&flag, missing, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess;
+    const void* st = hipGetLastError() == hipSuccess ? stageCopy(missing, sizeof(int), s) : nullptr;
     (void)hipFreeAsync(missing, s);
-    if (launched && hipStreamSynchronize(s) == hipSuccess) rc = flag ? NVQ_ERR_INVALID_ARG : NVQ_OK;
+    if (st && hipStreamSynchronize(s) == hipSuccess) {
+        std::memcpy(&flag, st, sizeof(int));
+        rc = flag ? NVQ_ERR_INVALID_ARG : NVQ_OK;
+    }
     return rc;
 }
 

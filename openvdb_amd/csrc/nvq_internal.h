@@ -8,6 +8,27 @@
 
 namespace nvq {
 
+// ------------------------------------------------------------------ stream-ordered scratch
+// Every stream-ordered allocation of the library comes from one memory pool per device that
+// keeps up to kPoolKeepBytes of freed memory mapped for the next call, instead of returning it to
+// the driver at each synchronisation; the process's default pool is left as the caller set it.
+// Freed with hipFreeAsync.
+constexpr uint64_t kPoolKeepBytes = uint64_t(32) << 30;
+hipError_t poolMalloc(void** p, size_t bytes, hipStream_t s);
+
+// Small device-to-host reads (counts, flags, a bbox) go through a pinned host buffer of
+// kStageBytes per host thread, allocated on its first use and kept until the process ends.  A
+// copy into pageable host memory is not queued: the runtime waits for it in the call, and a
+// hipFreeAsync issued right after such a copy stalled the host for about 230 ms per call once
+// some tens of multi-GiB point-query calls had run.
+constexpr size_t kStageBytes = 256;
+// Queues a copy of `bytes` <= kStageBytes (whole 32-bit words, 4-byte aligned) at device address
+// src into the calling thread's pinned buffer and returns it (valid once the stream has drained),
+// or nullptr.
+void* stageCopy(const void* src, size_t bytes, hipStream_t s);
+// stageCopy, wait for the stream, then copy into dst.
+hipError_t readBack(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // ------------------------------------------------------------------ ray-tile scheduling
 // Rays of a row-major image are processed in 8x8 pixel tiles (one tile per 64-wide wavefront).
 // Hardware dispatch is round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each with its
@@ -254,18 +275,6 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
                      float sigma, const float* L, const float* T, const float* gL, const float* gT, int64_t* accum,
                      int shift, hipStream_t s);
 int fixedToFloat(const int64_t* accum, uint64_t count, int shift, float* out, hipStream_t s);
-
-// Keep freed stream-ordered allocations in the device's default pool instead of returning them
-// to the driver at every synchronisation, so repeated large calls do not re-map memory.
-inline void keepPoolMemory()
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
-    uint64_t thr = ~uint64_t(0);
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-}
 
 // Handle for a grid buffer of `bytes` at device address d, its headers read from the host copy b
 // (nvq_kernels.hip).  `owner` says how nvq_grid_free releases d.

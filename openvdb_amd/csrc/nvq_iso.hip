@@ -93,7 +93,7 @@ int launchIso(const GridView& g, const IsoCamera& c, int persistent, float* imag
         return -1;
     const uint32_t tilesX = (uint32_t(c.width) + 7) / 8, tiles = tilesX * ((uint32_t(c.height) + 7) / 8);
     uint32_t* counter = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
+    if (poolMalloc(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
     uint32_t blocks = uint32_t(cus) * 8u;
     const uint32_t wavesNeeded = (tiles + (kBlock / 64) - 1) / (kBlock / 64);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 using namespace nvq;
@@ -51,9 +52,8 @@ nvq::GridLayout nvq::gridLayout(const nvq_grid* g)
 int nvq::gridRefreshBBox(nvq_grid* g, hipStream_t s)
 {
     int32_t box[6];
-    if (hipMemcpyAsync(box, static_cast<const uint8_t*>(g->dbuf) + kGridSize + g->nodeOff[3] + kRootBBox, sizeof(box),
-                       hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
+    if (readBack(box, static_cast<const uint8_t*>(g->dbuf) + kGridSize + g->nodeOff[3] + kRootBBox, sizeof(box), s) !=
+        hipSuccess)
         return NVQ_ERR_HIP;
     for (int i = 0; i < 6; ++i) g->view.bbox[i] = g->info.index_bbox[i] = box[i];
     if (hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -341,12 +341,77 @@ int validate(const uint8_t* b, uint64_t bytes)
 
 } // namespace
 
+hipError_t nvq::poolMalloc(void** p, size_t bytes, hipStream_t s)
+{
+    constexpr int kDevices = 64;
+    static std::mutex mu;
+    static hipMemPool_t pools[kDevices];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kDevices) return hipMallocAsync(p, bytes, s);
+    hipMemPool_t pool = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!pools[dev]) {
+            hipMemPoolProps props = {};
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            hipMemPool_t made = nullptr;
+            e = hipMemPoolCreate(&made, &props);
+            if (e != hipSuccess) return e;
+            uint64_t keep = kPoolKeepBytes;
+            e = hipMemPoolSetAttribute(made, hipMemPoolAttrReleaseThreshold, &keep);
+            if (e != hipSuccess) {
+                (void)hipMemPoolDestroy(made);
+                return e;
+            }
+            pools[dev] = made;
+        }
+        pool = pools[dev];
+    }
+    return hipMallocFromPoolAsync(p, bytes, pool, s);
+}
+
+namespace {
+// copies `words` 32-bit words from device memory into mapped pinned host memory
+__global__ void k_stage_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t words)
+{
+    if (threadIdx.x < words) dst[threadIdx.x] = src[threadIdx.x];
+}
+} // namespace
+
+void* nvq::stageCopy(const void* src, size_t bytes, hipStream_t s)
+{
+    thread_local void* stage = nullptr;
+    if (bytes > kStageBytes || bytes % 4 || reinterpret_cast<uintptr_t>(src) % 4) return nullptr;
+    if (!stage && hipHostMalloc(&stage, kStageBytes, hipHostMallocDefault) != hipSuccess) {
+        stage = nullptr;
+        return nullptr;
+    }
+    // a kernel store rather than a copy command: no copy is queued ahead of the caller's next
+    // stream-ordered free
+    hipLaunchKernelGGL(k_stage_words, dim3(1), dim3(kStageBytes / 4), 0, s, static_cast<uint32_t*>(stage),
+                       static_cast<const uint32_t*>(src), uint32_t(bytes / 4));
+    return hipGetLastError() == hipSuccess ? stage : nullptr;
+}
+
+hipError_t nvq::readBack(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    const void* st = stageCopy(src, bytes, s);
+    if (!st) return hipErrorUnknown;
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) std::memcpy(dst, st, bytes);
+    return e;
+}
+
 int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st)
 {
     out = TileOrder{};
     if (s.tiles == 0 || s.tiles > (uint64_t(1) << 32)) return -1;
     const size_t keyBytes = (size_t(s.tiles) * 4 + 255) & ~size_t(255);
-    if (hipMallocAsync(&out.base, keyBytes + size_t(s.tiles) * 4, st) != hipSuccess) {
+    if (poolMalloc(&out.base, keyBytes + size_t(s.tiles) * 4, st) != hipSuccess) {
         out.base = nullptr;
         return -1;
     }
@@ -456,8 +521,7 @@ int nvq::makeGridHandle(const uint8_t* b, uint64_t bytes, void* d, int owner, hi
     for (int i = 0; i < 3; ++i) in.voxel_size[i] = v.voxelSize[i];
     in.background = v.background;
     in.device_buffer = d;
-    keepPoolMemory();
-    if (hipMallocAsync(reinterpret_cast<void**>(&g->dview), sizeof(GridView), stream) != hipSuccess ||
+    if (poolMalloc(reinterpret_cast<void**>(&g->dview), sizeof(GridView), stream) != hipSuccess ||
         hipMemcpyAsync(g->dview, &g->view, sizeof(GridView), hipMemcpyHostToDevice, stream) != hipSuccess) {
         if (g->dview) (void)hipFreeAsync(g->dview, stream);
         delete g;

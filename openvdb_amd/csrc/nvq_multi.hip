@@ -141,7 +141,7 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     // queue counters, then the tile cost keys and the longest-first order (4 bytes per tile each)
     const size_t keyBytes = (size_t(A.totalTiles) * 4 + 255) & ~size_t(255);
     uint8_t* base = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&base), kQueueBytes + keyBytes + size_t(A.totalTiles) * 4, s) !=
+    if (poolMalloc(reinterpret_cast<void**>(&base), kQueueBytes + keyBytes + size_t(A.totalTiles) * 4, s) !=
         hipSuccess)
         return -2;
     uint32_t* counter = reinterpret_cast<uint32_t*>(base);

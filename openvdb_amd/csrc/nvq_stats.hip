@@ -370,8 +370,7 @@ int nvq_update_grid_stats_mode(nvq_grid* grid, int mode, void* stream)
     const hipStream_t s = static_cast<hipStream_t>(stream);
     const uint64_t slots = uint64_t(L.count[0]) + L.count[1] + L.count[2] + 1;
     Acc* acc = nullptr;
-    keepPoolMemory(); // stream-ordered pool: no device-wide synchronisation per call
-    if (hipMallocAsync(reinterpret_cast<void**>(&acc), slots * sizeof(Acc), s) != hipSuccess) return NVQ_ERR_HIP;
+    if (poolMalloc(reinterpret_cast<void**>(&acc), slots * sizeof(Acc), s) != hipSuccess) return NVQ_ERR_HIP;
     Acc* leafAcc = acc;
     Acc* lowerAcc = leafAcc + L.count[0];
     Acc* upperAcc = lowerAcc + L.count[1];

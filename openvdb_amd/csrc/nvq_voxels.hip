@@ -923,9 +923,7 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
             return NVQ_ERR_HIP;
     }
     Cnt tot;
-    if (hipMemcpyAsync(&tot, chunkIncl + (chunks - 1), sizeof(Cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return NVQ_ERR_HIP;
+    if (readBack(&tot, chunkIncl + (chunks - 1), sizeof(Cnt), s) != hipSuccess) return NVQ_ERR_HIP;
 
     // 3. layout (GridT::memUsage sums of PointsToGrid::getBuffer :848-856), zeroed buffer
     L.rootHdr = (L.isIndex || L.isPoint) ? 96u : 64u;
@@ -936,8 +934,7 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     L.offMeta = L.offLeaf + uint64_t(L.leafSize) * tot.l;
     L.total = P ? L.offMeta + 288u + uint64_t(P->size) * n32 : L.offMeta;
     void* dbuf = nullptr;
-    keepPoolMemory();
-    if (hipMallocAsync(&dbuf, L.total, s) != hipSuccess) return NVQ_ERR_HIP;
+    if (poolMalloc(&dbuf, L.total, s) != hipSuccess) return NVQ_ERR_HIP;
     uint8_t* buf = static_cast<uint8_t*>(dbuf);
     Header hdr{};
     hdr.bytes = uint32_t(kRootOff + L.rootHdr);
@@ -951,7 +948,7 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     void* extra = nullptr;
     Carve st{ws + deadStart};
     if (starts > dead) {
-        if (hipMallocAsync(&extra, starts, s) != hipSuccess) {
+        if (poolMalloc(&extra, starts, s) != hipSuccess) {
             (void)hipFreeAsync(dbuf, s);
             return NVQ_ERR_HIP;
         }
@@ -1013,8 +1010,7 @@ int buildSorted(Workspace& W, const int32_t* ijk, uint32_t n32, Layout L, int en
     }
     uint32_t most = 0;
     if (rc == NVQ_OK && P &&
-        (hipMemcpyAsync(&most, maxCount, sizeof(most), hipMemcpyDeviceToHost, s) != hipSuccess ||
-         hipStreamSynchronize(s) != hipSuccess || most > 65535u)) {
+        (readBack(&most, maxCount, sizeof(most), s) != hipSuccess || most > 65535u)) {
         // a leaf with more points than its uint16 values can count (the reference throws)
         (void)nvq_grid_free(*out);
         *out = nullptr;
@@ -1071,7 +1067,7 @@ int maxPerVoxelSorted(const int32_t* ijk, uint32_t n, const Layout& L, int endBi
     const size_t bytes = 3 * carveBytes<K>(n) + 2 * carveBytes<uint32_t>(n) + carveBytes<uint32_t>(2) +
                          carveBytes<uint8_t>(std::max({sortBytes, rleBytes, maxBytes}));
     void* mem = nullptr;
-    if (hipMallocAsync(&mem, bytes, s) != hipSuccess) return NVQ_ERR_HIP;
+    if (poolMalloc(&mem, bytes, s) != hipSuccess) return NVQ_ERR_HIP;
     Carve c{static_cast<uint8_t*>(mem)};
     K* keys = c.take<K>(n);
     K* sorted = c.take<K>(n);
@@ -1086,11 +1082,9 @@ int maxPerVoxelSorted(const int32_t* ijk, uint32_t n, const Layout& L, int endBi
     if (hipGetLastError() == hipSuccess &&
         hipcub::DeviceRadixSort::SortKeys(temp, sortBytes, keys, sorted, int(n), 0, endBit, s) == hipSuccess &&
         hipcub::DeviceRunLengthEncode::Encode(temp, rleBytes, sorted, unique, counts, small, int(n), s) == hipSuccess &&
-        hipMemcpyAsync(&runs, small, sizeof(runs), hipMemcpyDeviceToHost, s) == hipSuccess &&
-        hipStreamSynchronize(s) == hipSuccess &&
+        readBack(&runs, small, sizeof(runs), s) == hipSuccess &&
         hipcub::DeviceReduce::Max(temp, maxBytes, counts, small + 1, int(runs), s) == hipSuccess &&
-        hipMemcpyAsync(&most, small + 1, sizeof(most), hipMemcpyDeviceToHost, s) == hipSuccess &&
-        hipStreamSynchronize(s) == hipSuccess)
+        readBack(&most, small + 1, sizeof(most), s) == hipSuccess)
         rc = NVQ_OK;
     (void)hipFreeAsync(mem, s);
     return rc;
@@ -1107,8 +1101,7 @@ int maxPerVoxel(Workspace& W, const int32_t* ijk, uint32_t n, hipStream_t s, uin
     k_v2g_bounds<<<nb, kB, 0, s>>>(ijk, n, part);
     k_v2g_bounds_final<<<1, kB, 0, s>>>(part, nb, bnd);
     int32_t box[6];
-    if (hipMemcpyAsync(box, bnd, sizeof(box), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-        return NVQ_ERR_HIP;
+    if (readBack(box, bnd, sizeof(box), s) != hipSuccess) return NVQ_ERR_HIP;
     Layout L{};
     uint64_t tiles = 1;
     int tileBits = 0, lowerBits = 0;
@@ -1133,8 +1126,7 @@ int buildFromVoxels(Workspace& W, const int32_t* ijk, uint64_t n, uint32_t grid_
     k_v2g_bounds<<<nb, kB, 0, s>>>(ijk, n, part);
     k_v2g_bounds_final<<<1, kB, 0, s>>>(part, nb, bnd);
     int32_t box[6];
-    if (hipMemcpyAsync(box, bnd, sizeof(box), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-        return NVQ_ERR_HIP;
+    if (readBack(box, bnd, sizeof(box), s) != hipSuccess) return NVQ_ERR_HIP;
     Layout L{};
     uint64_t tiles = 1;
     int tileBits = 0, lowerBits = 0;
@@ -1187,9 +1179,8 @@ int pointsToGrid(const T* xyz, uint64_t n, double voxel_size, uint32_t point_typ
     if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return NVQ_ERR_NO_DEVICE;
     Workspace& W = workspaceOf(device);
     std::lock_guard<std::mutex> hold(W.lock);
-    keepPoolMemory();
     int32_t* ijk = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+    if (poolMalloc(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
     k_p2g_voxels<T><<<blocksFor(n, kB), kB, 0, s>>>(xyz, uint32_t(n), P.invF, P.invD, ijk);
     const int rc = buildFromVoxels(W, ijk, n, 23u, voxel_size, name, nullptr, &P, s, out);
     (void)hipFreeAsync(ijk, s);
@@ -1225,9 +1216,8 @@ int pointsToGridDensity(const T* xyz, uint64_t n, uint32_t target, int tolerance
     {
         Workspace& W = workspaceOf(device);
         std::lock_guard<std::mutex> hold(W.lock);
-        keepPoolMemory();
         int32_t* ijk = nullptr;
-        if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+        if (poolMalloc(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
         int rc = NVQ_OK;
         for (int iter = 0;;) {
             if (!(dx > 0.0) || !std::isfinite(dx)) {
@@ -1293,9 +1283,8 @@ int nvq_voxels_to_grid_fp(const void* xyz, int xyz_f64, uint64_t n, uint32_t gri
     const hipStream_t s = static_cast<hipStream_t>(stream);
     Workspace& W = workspaceOf(device);
     std::lock_guard<std::mutex> hold(W.lock);
-    keepPoolMemory();
     int32_t* ijk = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
+    if (poolMalloc(reinterpret_cast<void**>(&ijk), 12ull * n, s) != hipSuccess) return NVQ_ERR_HIP;
     if (xyz_f64)
         k_round_voxels<double><<<blocksFor(n, kB), kB, 0, s>>>(static_cast<const double*>(xyz), uint32_t(n), ijk);
     else
