@@ -122,10 +122,18 @@ __device__ __forceinline__ bool scheduleOrdered(const Sched& s, const uint32_t* 
 }
 inline uint32_t orderedBlocks(const Sched& s) { return uint32_t((s.tiles + kBlock / 64 - 1) / (kBlock / 64)); }
 
-// Cost keys of tiles: the top 12 bits of a non-negative float cost (exponent and 4 mantissa
-// bits: 6 % classes), inverted so that key 0 is the costliest; cost 0 gets the last key.
-constexpr uint32_t kOrderKeys = 4096;
-__device__ __forceinline__ uint32_t costKey(float cost) { return kOrderKeys - 1u - (__float_as_uint(cost) >> 19); }
+// Cost keys of tiles: the top 12 bits of a positive float cost (exponent and 4 mantissa bits:
+// 6 % classes), inverted so that key 0 is the costliest.  Tiles of cost 0 (every ray misses the
+// bbox), often most of an image, are spread over the last 64 keys by tile index: the counting
+// sort's atomics on one key serialise.  No positive cost reaches those keys (it would need to be
+// below 2^-123).
+constexpr uint32_t kOrderKeys = 4096, kZeroKeys = 64;
+__device__ __forceinline__ uint32_t costKey(float cost, uint64_t tile)
+{
+    if (!(cost > 0.0f)) return kOrderKeys - kZeroKeys + uint32_t(tile % kZeroKeys);
+    const uint32_t k = kOrderKeys - 1u - (__float_as_uint(cost) >> 19);
+    return k < kOrderKeys - kZeroKeys ? k : kOrderKeys - kZeroKeys - 1u;
+}
 
 // Index-space length of a world ray inside the grid's bbox (the march's own map and clip); 0 when
 // it misses.
@@ -136,6 +144,9 @@ __device__ __forceinline__ float clippedLength(const GridView& g, const float* e
     return rayClip(R, g.bbox) && double(R.t1) <= 1e20 && R.t1 > R.t0 ? R.t1 - R.t0 : 0.0f;
 }
 
+// lanes 0, 7, 56 and 63: the corners of an 8x8 ray tile
+__device__ __forceinline__ bool cornerLane(uint32_t lane) { return ((lane & 7u) == 0u || (lane & 7u) == 7u) && (lane < 8u || lane >= 56u); }
+
 __device__ __forceinline__ float waveMax(float x)
 {
 #pragma unroll
@@ -143,7 +154,7 @@ __device__ __forceinline__ float waveMax(float x)
     return x;
 }
 
-constexpr int kOrderThreads = 1024;
+constexpr int kOrderThreads = 1024, kOrderBatch = 16;
 namespace {
 // One workgroup: counting sort of the tile keys (histogram and scan in LDS), tiles of one key in
 // the order the workgroup's atomics hand out their slots.
@@ -155,7 +166,19 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
     constexpr int per = kOrderKeys / kOrderThreads;
     for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) count[k] = 0;
     __syncthreads();
-    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) atomicAdd(&count[key[t]], 1u);
+    // kOrderBatch keys per thread are loaded before their atomics, so each thread keeps that many
+    // loads in flight
+    for (uint64_t t0 = 0; t0 < tiles; t0 += uint64_t(kOrderThreads) * kOrderBatch) {
+        uint32_t k[kOrderBatch];
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const uint64_t t = t0 + uint64_t(j) * kOrderThreads + threadIdx.x;
+            k[j] = t < tiles ? key[t] : kOrderKeys;
+        }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j)
+            if (k[j] < kOrderKeys) atomicAdd(&count[k[j]], 1u);
+    }
     __syncthreads();
     uint32_t local[per], sum = 0;
 #pragma unroll
@@ -175,7 +198,17 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
 #pragma unroll
     for (int j = 0; j < per; ++j) count[threadIdx.x * per + j] = base + local[j];
     __syncthreads();
-    for (uint64_t t = threadIdx.x; t < tiles; t += kOrderThreads) order[atomicAdd(&count[key[t]], 1u)] = uint32_t(t);
+    for (uint64_t t0 = 0; t0 < tiles; t0 += uint64_t(kOrderThreads) * kOrderBatch) {
+        uint32_t k[kOrderBatch];
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const uint64_t t = t0 + uint64_t(j) * kOrderThreads + threadIdx.x;
+            k[j] = t < tiles ? key[t] : kOrderKeys;
+        }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j)
+            if (k[j] < kOrderKeys) order[atomicAdd(&count[k[j]], 1u)] = uint32_t(t0 + uint64_t(j) * kOrderThreads + threadIdx.x);
+    }
 }
 
 } // namespace

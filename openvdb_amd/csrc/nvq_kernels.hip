@@ -189,18 +189,19 @@ __global__ __launch_bounds__(kBlock) void k_fog_march(GridView g, const float* _
 }
 
 // ------------------------------------------------------------------ longest-first tile order
-// Cost of a tile: its longest ray inside the grid's bbox.
+// Cost of a tile: the longest of its four corner rays inside the grid's bbox.
 __global__ __launch_bounds__(kBlock) void k_tile_cost(GridView g, const float* __restrict__ eye,
                                                       const float* __restrict__ dir, Sched s,
                                                       uint32_t* __restrict__ key)
 {
     const uint64_t tile = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
     if (tile >= s.tiles) return; // whole wavefronts leave together
+    const uint32_t lane = threadIdx.x & 63u;
     uint64_t i;
     float len = 0.0f;
-    if (tileRay(s, tile, threadIdx.x & 63u, i)) len = clippedLength(g, eye + 3 * i, dir + 3 * i);
+    if (cornerLane(lane) && tileRay(s, tile, lane, i)) len = clippedLength(g, eye + 3 * i, dir + 3 * i);
     len = waveMax(len);
-    if ((threadIdx.x & 63u) == 0) key[tile] = costKey(len);
+    if (lane == 0) key[tile] = costKey(len, tile);
 }
 
 // ------------------------------------------------------------------ host helpers

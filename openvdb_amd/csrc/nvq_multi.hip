@@ -64,14 +64,16 @@ __global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float
     });
 }
 
-// Cost of a tile of the multi-grid launch for the longest-first queue order: its longest ray
-// inside its grid's bbox, a sixteenth of that for zeroCrossing.  A march steps every half voxel
+// Cost of a tile of the multi-grid launch for the longest-first queue order: the longest of its
+// four corner rays inside its grid's bbox, a sixteenth of that for zeroCrossing.  A march steps every half voxel
 // with a trilinear sample per step; zeroCrossing moves by nodes and leaves to the surface and
 // stops at the first crossing, at far fewer probes per unit length.
 __global__ __launch_bounds__(kBlock) void k_multi_tile_cost(MultiArgs A, const float* __restrict__ eye,
                                                             const float* __restrict__ dir, uint32_t* __restrict__ key)
 {
-    const uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    // the tile, its slot and the slot's grid view are the same for the whole wavefront: scalar
+    const uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & 63u;
     if (t >= A.totalTiles) return; // whole wavefronts leave together
     uint32_t j = 0;
     while (j + 1 < A.k && A.slot[j + 1].tileBase <= t) ++j;
@@ -81,12 +83,12 @@ __global__ __launch_bounds__(kBlock) void k_multi_tile_cost(MultiArgs A, const f
     const uint32_t px = tx * 8 + (lane & 7u), py = ty * 8 + (lane >> 3);
     const uint64_t i = S.rayBase + uint64_t(py) * S.width + px;
     float c = 0.0f;
-    if (px < S.width && py < S.height && i < S.rayEnd) {
+    if (cornerLane(lane) && px < S.width && py < S.height && i < S.rayEnd) {
         c = clippedLength(*S.view, eye + 3 * i, dir + 3 * i);
         if (S.op == kOpZeroCrossing) c *= 0.0625f;
     }
     c = waveMax(c);
-    if (lane == 0) key[t] = costKey(c);
+    if (lane == 0) key[t] = costKey(c, t);
 }
 
 constexpr int kMaxDevices = 64;
