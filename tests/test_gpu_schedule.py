@@ -1,0 +1,125 @@
+"""Launch schedules and scratch reuse leave results unchanged.
+
+- The fog march and its adjoint take ray tiles in descending order of an estimated cost; a ray's
+  radiance, transmittance and the adjoint's integer sums must not depend on that order or on the
+  tile shape.  The same rays marched as a row-major image (8x8 tiles, ragged right column and
+  bottom row) and as a linear batch (64 consecutive rays per tile, ragged last tile) must agree bit
+  for bit, and both with the FP64 oracle within 1e-5.
+- The multi-grid launch hands out its queue costliest first, the no-allocation variant in queue
+  order: both must give identical outputs.
+- Binned point queries lease a grow-only scratch block per (device, stream).  Batches that grow and
+  shrink on one stream, and batches on two streams at once, must stay bit-exact against the oracle
+  and the lane-per-query path.
+"""
+import numpy as np
+import pytest
+
+from conftest import points_near, rng
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _nvq():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import openvdb_amd
+    return openvdb_amd
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _camera(nvq, w, h, box, seed):
+    """A pinhole camera of w x h rays looking at the middle of an index box from outside it."""
+    r = rng(seed)
+    lo, hi = np.array(box[:3], np.float64), np.array(box[3:], np.float64) + 1.0
+    c = 0.5 * (lo + hi)
+    v = r.normal(size=3)
+    v /= np.linalg.norm(v)
+    eye = tuple(float(x) for x in c + 1.2 * float(np.linalg.norm(hi - lo)) * v)
+    up = (0.0, 1.0, 0.0) if abs(v[1]) < 0.9 else (1.0, 0.0, 0.0)
+    # camera_rays makes square-pixel images of width x height
+    e, d = nvq.tools.camera_rays(w, h, eye, tuple(float(x) for x in c), up, 45.0)
+    return e, d
+
+
+@pytest.fixture(scope="module")
+def fog64():
+    nvq = _nvq()
+    return nvq.tools.create_fog_fbm([0, 0, 0, 63, 63, 63], seed=5, amplitude=1.0)
+
+
+def test_fog_march_image_and_linear_orders_agree(oracle, fog64):
+    nvq = _nvq()
+    d = nvq.DeviceGrid(fog64)
+    w, h = 61, 45  # neither a multiple of 8: ragged tiles on the right and at the bottom
+    e, dr = _camera(nvq, w, h, [0, 0, 0, 63, 63, 63], 3)
+    n = w * h
+    assert n % 64 != 0  # ragged last tile of the linear batch
+    print(f"[schedule] fog march image {w}x{h} and linear n={n}", flush=True)
+    E, D = _t(e), _t(dr)
+    Li, Ti = nvq.ops.fog_march(d, E, D, 0.5, 0.05, image_width=w)
+    Ll, Tl = nvq.ops.fog_march(d, E, D, 0.5, 0.05)
+    assert torch.equal(Li, Ll) and torch.equal(Ti, Tl)
+    oL, oT, steps = oracle.fog_march(fog64, e, dr, 0.5, 0.05)
+    assert steps.max() > 50
+    np.testing.assert_allclose(Li.cpu().numpy().astype(np.float64), oL, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(Ti.cpu().numpy().astype(np.float64), oT, rtol=1e-5, atol=1e-6)
+    # the adjoint's integer sums do not depend on the tile order either
+    gL = _t(rng(4).normal(0, 1, n).astype(np.float32))
+    ai = nvq.ops.fog_march_backward(d, E, D, Li, gL, None, None, 0.5, 0.05, image_width=w)
+    al = nvq.ops.fog_march_backward(d, E, D, Ll, gL, None, None, 0.5, 0.05)
+    assert torch.equal(ai, al)
+    assert int((ai != 0).sum()) > 1000
+
+
+def test_multi_query_ordered_and_queue_order_agree(fog64):
+    nvq = _nvq()
+    sphere = nvq.tools.create_level_set_sphere(20.0, (0.0, 0.0, 0.0), 1.0, 3.0)
+    grids = [nvq.DeviceGrid(sphere), nvq.DeviceGrid(fog64), nvq.DeviceGrid(sphere)]
+    sides = [40, 56, 24]
+    boxes = [g.index_bbox for g in grids]
+    parts = [_camera(nvq, s, s, b, 10 + k) for k, (s, b) in enumerate(zip(sides, boxes))]
+    e = np.concatenate([p[0] for p in parts])
+    dr = np.concatenate([p[1] for p in parts])
+    offs = [0]
+    for s in sides:
+        offs.append(offs[-1] + s * s)
+    print(f"[schedule] multi-grid query n={len(e)}", flush=True)
+    E, D = _t(e), _t(dr)
+    a = nvq.ops.multi_query(grids, E, D, offs, sides, 0.5, 0.05)
+    b = nvq.ops.multi_query(grids, E, D, offs, sides, 0.5, 0.05, noalloc=True)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+    assert int(a[0].sum()) > 100  # zeroCrossing hits on the spheres
+
+
+def test_binned_scratch_grows_shrinks_and_serves_two_streams(oracle, torus):
+    nvq = _nvq()
+    d = nvq.DeviceGrid(torus)
+    pts = points_near(oracle, torus, (1 << 21) + 77, seed=31)
+    P = _t(pts)
+    # on one stream: 2^20, then larger (the block grows), then 2^20 again (reused)
+    for n in ((1 << 20) + 5, (1 << 21) + 77, (1 << 20) + 11):
+        print(f"[schedule] binned sample n={n}", flush=True)
+        nn, val, g = nvq.ops.sample(d, P[:n].contiguous(), grad_mode=1)
+        onn, oval, og = oracle.sample(torus, pts[:n], grad_mode=1)
+        assert np.array_equal(val.cpu().numpy().view(np.uint32), oval.view(np.uint32))
+        assert np.array_equal(g.cpu().numpy().view(np.uint32), og.view(np.uint32))
+        assert np.array_equal(nn.cpu().numpy().view(np.uint32), onn.view(np.uint32))
+    # two streams at once, each with its own block
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    A, B = P[: (1 << 20) + 3].contiguous(), P[(1 << 20):].contiguous()
+    torch.cuda.synchronize()
+    print("[schedule] binned sample on two streams", flush=True)
+    with torch.cuda.stream(s1):
+        ra = nvq.ops.sample(d, A, grad_mode=1)
+    with torch.cuda.stream(s2):
+        rb = nvq.ops.sample(d, B, grad_mode=1)
+    torch.cuda.synchronize()
+    for r, x in ((ra, pts[: (1 << 20) + 3]), (rb, pts[(1 << 20):])):
+        onn, oval, og = oracle.sample(torus, x, grad_mode=1)
+        assert np.array_equal(r[1].cpu().numpy().view(np.uint32), oval.view(np.uint32))
+        assert np.array_equal(r[2].cpu().numpy().view(np.uint32), og.view(np.uint32))
