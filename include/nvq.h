@@ -4,12 +4,14 @@
  * C ABI of libnvq.so.  Plain pointers and sizes only.  Unless a name says
  * "host_", every pointer argument is a DEVICE pointer.  A `stream` is a
  * hipStream_t passed as void* (NULL = the default stream).  Every call is
- * stream-ordered and asynchronous.  Every call returns NVQ_OK (0) or a negative
- * NVQ_ERR_* status.  Nothing throws across the ABI.  Apart from the opt-in
- * kernel profiler (nvq_profile_*) and the work-queue array of nvq_multi_query_noalloc,
- * the library holds no global mutable state: calls on distinct streams are thread-safe.  The caller
- * owns every buffer and keeps the grid alive until work queued on it has
- * finished.
+ * stream-ordered and asynchronous unless its comment says it waits.  Every call returns NVQ_OK (0)
+ * or a negative NVQ_ERR_* status.  Nothing throws across the ABI.  Query scratch (ray-tile orders,
+ * queue counters, the sort buffers of binned point queries) lives in one grow-only block per
+ * (device, stream) that the library keeps until the process ends; besides it, the library holds
+ * the opt-in kernel profiler (nvq_profile_*), the work-queue array of nvq_multi_query_noalloc and
+ * a small pinned host buffer per calling thread for values it reads back.  Calls on distinct
+ * streams are thread-safe; calls on one stream from several threads are serialised.  The caller
+ * owns every buffer and keeps the grid alive until work queued on it has finished.
  *
  * The grid is an unmodified NanoVDB buffer (magic "NanoVDB1", ABI major 32),
  * as written by NanoVDB's own tools (createLevelSetSphere, createNanoGrid, …)
@@ -114,10 +116,9 @@ int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* v
  *          grad_mode 2 = GradStencil::gradient at RoundDown(xyz)  math/Stencils.h:502-507
  *          grad_mode 3 = TrilinearSampler::gradient (index space) math/SampleFromVoxels.h:232-271
  * nn / val / grad may be NULL.  Replaces the nanobind op sampleFromVoxels(points, d_grid, values[, gradients]).
- * Batches of 2^20 points or more are binned through a scratch block the library keeps per
- * (device, stream), about 120 bytes per point, grown when a larger batch comes and held until
- * the process ends; they wait for `stream` once (to size the sort of the points that touch a
- * leaf).  Smaller batches do not wait and take no scratch. */
+ * Batches of 2^20 points or more are binned through the stream's scratch block (about 120 bytes
+ * per point) and wait for `stream` once (to size the sort of the points that touch a leaf).
+ * Smaller batches do not wait and take no scratch. */
 int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,
                int grad_mode, void* stream);
 

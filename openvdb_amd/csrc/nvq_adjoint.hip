@@ -112,12 +112,8 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
     const Sched sc = makeSched(n, width);
     const float scale = ldexpf(1.0f, shift);
     TileOrder to;
-    if (tileOrder(g, eye, dir, sc, to, s) != 0) return -1;
-    uint32_t* flag = nullptr;
-    if (poolMalloc(reinterpret_cast<void**>(&flag), sizeof(uint32_t), s) != hipSuccess) {
-        releaseTileOrder(to, s);
-        return -1;
-    }
+    if (tileOrder(g, eye, dir, sc, to, s, sizeof(uint32_t)) != 0) return -1;
+    uint32_t* flag = reinterpret_cast<uint32_t*>(to.extra);
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flag);
     auto kernel = g.type == GT_ONINDEX ? k_fog_march_bwd<GT_ONINDEX> : k_fog_march_bwd<GT_FLOAT>;
     {
@@ -128,8 +124,6 @@ int fogMarchBackward(const GridView& g, const float* eye, const float* dir, uint
     bool ok = hipGetLastError() == hipSuccess;
     const void* st = stageCopy(flag, sizeof(uint32_t), s);
     ok = st && ok;
-    ok = hipFreeAsync(flag, s) == hipSuccess && ok;
-    ok = releaseTileOrder(to, s) == 0 && ok;
     ok = hipStreamSynchronize(s) == hipSuccess && ok;
     if (!ok) return -1;
     uint32_t over = 0;

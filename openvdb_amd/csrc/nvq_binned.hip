@@ -35,10 +35,6 @@
 
 #include <algorithm>
 #include <cstdint>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <utility>
 
 namespace nvq {
 namespace {
@@ -817,62 +813,6 @@ bool makeLayout(const GridView& g, BinLayout& L)
     while ((uint64_t(1) << L.bits) < total) ++L.bits;
     return true;
 }
-
-// ------------------------------------------------------------------ per-stream workspace
-// The scratch of a binned call (several GiB for 2^26 queries) is leased from one grow-only block
-// per (device, stream) that stays allocated between calls.  Allocating and freeing it per call
-// (stream-ordered) made hipFreeAsync block the host for about 230 ms per call once some tens of
-// such calls had run.  Calls on one stream execute in stream order, so they can share the block;
-// the lease's lock serialises host threads that issue calls on the same stream.  When a call
-// needs more, the old block is freed in stream order and a larger one taken; the last block of
-// each stream is kept until the process ends.
-struct StreamWorkspace {
-    std::mutex use;
-    char* ptr = nullptr;
-    size_t bytes = 0;
-};
-std::mutex gWorkspaceLock;
-std::map<std::pair<int, hipStream_t>, std::unique_ptr<StreamWorkspace>> gWorkspaces;
-
-class WorkspaceLease {
-public:
-    WorkspaceLease() = default;
-    WorkspaceLease(const WorkspaceLease&) = delete;
-    WorkspaceLease& operator=(const WorkspaceLease&) = delete;
-    ~WorkspaceLease()
-    {
-        if (w_) w_->use.unlock();
-    }
-    // The stream's block, at least `bytes` long, held until the lease ends; nullptr on failure.
-    char* acquire(size_t bytes, hipStream_t s)
-    {
-        int dev = 0;
-        if (w_ || hipGetDevice(&dev) != hipSuccess) return nullptr;
-        StreamWorkspace* w;
-        {
-            std::lock_guard<std::mutex> lock(gWorkspaceLock);
-            std::unique_ptr<StreamWorkspace>& e = gWorkspaces[std::make_pair(dev, s)];
-            if (!e) e = std::make_unique<StreamWorkspace>();
-            w = e.get();
-        }
-        w->use.lock();
-        w_ = w;
-        if (w->bytes < bytes) {
-            if (w->ptr) (void)hipFreeAsync(w->ptr, s); // after the earlier calls' kernels on s
-            w->ptr = nullptr;
-            w->bytes = 0;
-            const size_t want = bytes + bytes / 8;
-            void* p = nullptr;
-            if (poolMalloc(&p, want, s) != hipSuccess) return nullptr;
-            w->ptr = static_cast<char*>(p);
-            w->bytes = want;
-        }
-        return w->ptr;
-    }
-
-private:
-    StreamWorkspace* w_ = nullptr;
-};
 
 // Scratch of one binned call: keys[2n] | pts[2n] | begin[m] | end[m] | spillCount | flag | spill[n]
 // | forward only: rec[n], classify marks, counts and offsets, scan temporaries | sort temporaries,

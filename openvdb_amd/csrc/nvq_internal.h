@@ -114,11 +114,14 @@ __device__ __forceinline__ bool tileRay(const Sched& s, uint64_t tile, uint32_t 
 // tiles in descending order of their longest clipped ray instead, wavefront k of the launch runs
 // tile order[k]: the launch ends with its cheapest tiles, and consecutive workgroups (one per
 // XCD in turn) get tiles of equal cost.
+// A launch of at most kOrderMinTiles tiles has every tile resident at once: it runs in image
+// order (order == nullptr) without the two ordering kernels.
+constexpr uint64_t kOrderMinTiles = 4096;
 __device__ __forceinline__ bool scheduleOrdered(const Sched& s, const uint32_t* __restrict__ order, uint64_t& idx)
 {
     const uint64_t k = uint64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
     if (k >= s.tiles) return false;
-    return tileRay(s, order[k], threadIdx.x & 63u, idx);
+    return tileRay(s, order ? order[k] : k, threadIdx.x & 63u, idx);
 }
 inline uint32_t orderedBlocks(const Sched& s) { return uint32_t((s.tiles + kBlock / 64 - 1) / (kBlock / 64)); }
 
@@ -213,14 +216,38 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
 
 } // namespace
 
-// Stream-ordered scratch holding the order; releaseTileOrder frees it after the launches queued
-// on the stream so far.
-struct TileOrder {
-    void* base = nullptr;
-    const uint32_t* order = nullptr;
+// Per-call scratch of the query entry points is leased from one grow-only block per (device,
+// stream) that stays allocated between calls: on this runtime hipFreeAsync waits for the stream's
+// queued work, so a stream-ordered allocation and free per call made every call synchronous (and,
+// for the multi-GiB scratch of binned point queries, stalled the host for about 230 ms per call
+// once some tens of calls had run).  Calls on one stream execute in stream order, so they can
+// share the block; the lease's lock serialises host threads that issue calls on the same stream.
+// When a call needs more, the old block is freed in stream order and a larger one taken; the last
+// block of each stream is kept until the process ends.
+struct StreamWorkspace;
+class WorkspaceLease {
+public:
+    WorkspaceLease() = default;
+    WorkspaceLease(const WorkspaceLease&) = delete;
+    WorkspaceLease& operator=(const WorkspaceLease&) = delete;
+    ~WorkspaceLease();
+    // The stream's block, at least `bytes` long, held until the lease ends; nullptr on failure.
+    // One lease per call: a second acquire on the same lease fails.
+    char* acquire(size_t bytes, hipStream_t s);
+
+private:
+    StreamWorkspace* w_ = nullptr;
 };
-int tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st);
-int releaseTileOrder(TileOrder& o, hipStream_t st);
+
+// The longest-first order of a ray launch (nullptr up to kOrderMinTiles tiles), in the stream's
+// workspace, with `extra` more bytes, 256-byte aligned, for the caller.
+struct TileOrder {
+    WorkspaceLease lease;
+    const uint32_t* order = nullptr;
+    char* extra = nullptr;
+};
+int tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st,
+              size_t extraBytes = 0);
 
 
 // Large-batch point sampling through spatial bins and LDS bricks (nvq_binned.hip).

@@ -92,8 +92,9 @@ int launchIso(const GridView& g, const IsoCamera& c, int persistent, float* imag
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return -1;
     const uint32_t tilesX = (uint32_t(c.width) + 7) / 8, tiles = tilesX * ((uint32_t(c.height) + 7) / 8);
-    uint32_t* counter = nullptr;
-    if (poolMalloc(reinterpret_cast<void**>(&counter), kQueueBytes, s) != hipSuccess) return -2;
+    WorkspaceLease lease;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(lease.acquire(kQueueBytes, s));
+    if (!counter) return -2;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
     uint32_t blocks = uint32_t(cus) * 8u;
     const uint32_t wavesNeeded = (tiles + (kBlock / 64) - 1) / (kBlock / 64);
@@ -104,9 +105,7 @@ int launchIso(const GridView& g, const IsoCamera& c, int persistent, float* imag
         hipLaunchKernelGGL(k_iso_render_persistent<GT>, dim3(blocks), dim3(kBlock), 0, s, g, c, tilesX, tiles, counter,
                            image, hit, t);
     }
-    const bool ok = hipGetLastError() == hipSuccess;
-    if (hipFreeAsync(counter, s) != hipSuccess) return -3;
-    return ok ? 0 : -4;
+    return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
 } // namespace

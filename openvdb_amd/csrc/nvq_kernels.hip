@@ -17,7 +17,11 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <map>
+#include <memory>
 #include <mutex>
+#include <utility>
 #include <new>
 
 using namespace nvq;
@@ -407,35 +411,74 @@ hipError_t nvq::readBack(void* dst, const void* src, size_t bytes, hipStream_t s
     return e;
 }
 
-int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st)
+struct nvq::StreamWorkspace {
+    std::mutex use;
+    char* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+namespace {
+std::mutex gWorkspaceLock;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<nvq::StreamWorkspace>> gWorkspaces;
+} // namespace
+
+nvq::WorkspaceLease::~WorkspaceLease()
 {
-    out = TileOrder{};
-    if (s.tiles == 0 || s.tiles > (uint64_t(1) << 32)) return -1;
-    const size_t keyBytes = (size_t(s.tiles) * 4 + 255) & ~size_t(255);
-    if (poolMalloc(&out.base, keyBytes + size_t(s.tiles) * 4, st) != hipSuccess) {
-        out.base = nullptr;
-        return -1;
+    if (w_) w_->use.unlock();
+}
+
+char* nvq::WorkspaceLease::acquire(size_t bytes, hipStream_t s)
+{
+    int dev = 0;
+    if (w_ || hipGetDevice(&dev) != hipSuccess) return nullptr;
+    StreamWorkspace* w;
+    {
+        std::lock_guard<std::mutex> lock(gWorkspaceLock);
+        std::unique_ptr<StreamWorkspace>& e = gWorkspaces[std::make_pair(dev, s)];
+        if (!e) e = std::make_unique<StreamWorkspace>();
+        w = e.get();
     }
-    uint32_t* key = static_cast<uint32_t*>(out.base);
-    uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(out.base) + keyBytes);
+    w->use.lock();
+    w_ = w;
+    if (w->bytes < bytes || !w->ptr) {
+        if (w->ptr) (void)hipFreeAsync(w->ptr, s); // after the earlier calls' kernels on s
+        w->ptr = nullptr;
+        w->bytes = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 8, size_t(1) << 20);
+        void* p = nullptr;
+        if (poolMalloc(&p, want, s) != hipSuccess) return nullptr;
+        w->ptr = static_cast<char*>(p);
+        w->bytes = want;
+    }
+    return w->ptr;
+}
+
+int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const Sched& s, TileOrder& out, hipStream_t st,
+                   size_t extraBytes)
+{
+    if (s.tiles == 0 || s.tiles > (uint64_t(1) << 32)) return -1;
+    const bool ordered = s.tiles > kOrderMinTiles;
+    const size_t keyBytes = ordered ? (size_t(s.tiles) * 4 + 255) & ~size_t(255) : 0;
+    const size_t orderBytes = keyBytes;
+    char* base = nullptr;
+    if (keyBytes + orderBytes + extraBytes) {
+        base = out.lease.acquire(keyBytes + orderBytes + extraBytes, st);
+        if (!base) return -1;
+    }
+    out.extra = extraBytes ? base + keyBytes + orderBytes : nullptr;
+    if (!ordered) {
+        out.order = nullptr;
+        return 0;
+    }
+    uint32_t* key = reinterpret_cast<uint32_t*>(base);
+    uint32_t* order = reinterpret_cast<uint32_t*>(base + keyBytes);
     {
         ProfScope prof("k_tile_order", st);
         hipLaunchKernelGGL(k_tile_cost, dim3(orderedBlocks(s)), dim3(kBlock), 0, st, g, eye, dir, s, key);
         hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, key, s.tiles, order);
     }
     out.order = order;
-    if (hipGetLastError() != hipSuccess) {
-        releaseTileOrder(out, st);
-        return -1;
-    }
-    return 0;
-}
-
-int nvq::releaseTileOrder(TileOrder& o, hipStream_t st)
-{
-    const hipError_t e = o.base ? hipFreeAsync(o.base, st) : hipSuccess;
-    o = TileOrder{};
-    return e == hipSuccess ? 0 : -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // ======================================================================= C ABI
@@ -687,10 +730,8 @@ int nvq_fog_march(const nvq_grid* grid, const float* eye, const float* dir, uint
     const Sched sc = makeSched(n, image_width);
     TileOrder to;
     if (tileOrder(grid->view, eye, dir, sc, to, S(stream)) != 0) return NVQ_ERR_HIP;
-    const int st = launchTyped<RunFogMarch>(grid->view.type, grid->view, eye, dir, sc, to.order, dt, sigma_t, radiance,
-                                            transmittance, S(stream));
-    const int fr = releaseTileOrder(to, S(stream));
-    return st != NVQ_OK ? st : (fr != 0 ? NVQ_ERR_HIP : NVQ_OK);
+    return launchTyped<RunFogMarch>(grid->view.type, grid->view, eye, dir, sc, to.order, dt, sigma_t, radiance,
+                                    transmittance, S(stream));
 }
 
 int nvq_fog_march_bwd(const nvq_grid* grid, const float* eye, const float* dir, uint64_t n, uint32_t image_width,

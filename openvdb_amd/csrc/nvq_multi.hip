@@ -140,17 +140,18 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return -1;
-    // queue counters, then the tile cost keys and the longest-first order (4 bytes per tile each)
-    const size_t keyBytes = (size_t(A.totalTiles) * 4 + 255) & ~size_t(255);
-    uint8_t* base = nullptr;
-    if (poolMalloc(reinterpret_cast<void**>(&base), kQueueBytes + keyBytes + size_t(A.totalTiles) * 4, s) !=
-        hipSuccess)
-        return -2;
+    // queue counters, then the tile cost keys and the longest-first order (4 bytes per tile each;
+    // none up to kOrderMinTiles tiles, which are all resident at once)
+    const bool ordered = A.totalTiles > kOrderMinTiles;
+    const size_t keyBytes = ordered ? (size_t(A.totalTiles) * 4 + 255) & ~size_t(255) : 0;
+    WorkspaceLease lease;
+    char* base = lease.acquire(kQueueBytes + 2 * keyBytes, s);
+    if (!base) return -2;
     uint32_t* counter = reinterpret_cast<uint32_t*>(base);
     uint32_t* key = reinterpret_cast<uint32_t*>(base + kQueueBytes);
-    uint32_t* order = reinterpret_cast<uint32_t*>(base + kQueueBytes + keyBytes);
+    uint32_t* order = ordered ? reinterpret_cast<uint32_t*>(base + kQueueBytes + keyBytes) : nullptr;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
-    {
+    if (ordered) {
         ProfScope prof("k_tile_order", s);
         hipLaunchKernelGGL(k_multi_tile_cost, dim3((A.totalTiles + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
                            s, A, eye, dir, key);
@@ -161,9 +162,7 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
         hipLaunchKernelGGL(k_multi_query, dim3(queryBlocks(A, cus)), dim3(kBlock), 0, s, A, eye, dir, dt, sigma,
                            counter, order, hit, ijk, a, b, normal);
     }
-    const bool ok = hipGetLastError() == hipSuccess;
-    if (hipFreeAsync(base, s) != hipSuccess) return -3;
-    return ok ? 0 : -4;
+    return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
 } // namespace nvq
