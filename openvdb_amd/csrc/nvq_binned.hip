@@ -604,19 +604,15 @@ __global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T
             ra[j] = marked ? rec[i].a : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             rb[j] = marked && grad ? rec[i].b.x : 0.0f;
         }
+        // one store per output for every query of the chunk, whether its result comes from a
+        // record or from the slot table: stores that covered the marked and the other queries of a
+        // line separately wrote each output line twice
 #pragma unroll
         for (int j = 0; j < kClassifyItems; ++j) {
             const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
-            if (!((flags >> j) & 1u)) continue;
-            if (nn) nn[i] = ra[j].x;
-            if (val) val[i] = ra[j].y;
-            if (grad) storeGrad(grad, i, ra[j].z, ra[j].w, rb[j]);
-        }
-        for (int j = 0; j < kClassifyItems; ++j) {
-            const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
-            if (i >= n || ((flags >> j) & 1u)) continue;
-            float o[5];
-            {
+            if (i >= n) continue;
+            float o[5] = {ra[j].x, ra[j].y, ra[j].z, ra[j].w, rb[j]};
+            if (!((flags >> j) & 1u)) {
                 float p[3];
                 loadPoint(g, xyz, i, world, p);
                 SlotBlock B;
