@@ -32,10 +32,7 @@ __device__ __forceinline__ void runRay(const GridView& g, uint32_t op, const flo
     else fogMarchRay<GT>(g, eye, dir, i, dt, sigma, a, b);
 }
 
-// Every codec's zeroCrossing and march bodies in one kernel take 137 VGPRs, 3 wavefronts per SIMD;
-// asking for 4 caps them at 128 with a 28-byte spill, and the 16-grid launch runs 2.5 % faster
-// (5 and 6 wavefronts spill 180 and 244 bytes and run slower).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_multi_query(MultiArgs A, const float* __restrict__ eye,
+__global__ __launch_bounds__(kBlock) void k_multi_query(MultiArgs A, const float* __restrict__ eye,
                                                         const float* __restrict__ dir, float dt, float sigma,
                                                         uint32_t* __restrict__ counters,
                                                         const uint32_t* __restrict__ order, uint8_t* __restrict__ hit,
