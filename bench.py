@@ -611,8 +611,8 @@ def cpu_baseline(grids_np, pts_np, log2n):
 
 
 def host_threads():
-    """Host threads for the reference CPU legs: every core of the box, capped at 16 (the GPU box's
-    CPU share for one GPU)."""
+    """Host threads for the reference CPU legs: every core of the host, capped at 16 (the host CPU
+    share that goes with one GPU)."""
     return max(1, min(16, os.cpu_count() or 1))
 
 
