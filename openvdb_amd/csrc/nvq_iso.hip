@@ -70,10 +70,12 @@ __global__ __launch_bounds__(kBlock) void k_iso_render_persistent(GridView g, Is
                                                                   float* __restrict__ tout)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    Accessor<GT> acc(g); // kept across the pixels this lane renders
     forEachQueuedTile(counters, tiles, [&](uint32_t t) {
         const uint32_t x = (t % tilesX) * 8 + (lane & 7u), y = (t / tilesX) * 8 + (lane >> 3);
-        if (x < uint32_t(c.width) && y < uint32_t(c.height)) renderPixel<GT>(acc, g, c, x, y, image, hit, tout);
+        if (x < uint32_t(c.width) && y < uint32_t(c.height)) {
+            Accessor<GT> acc(g);
+            renderPixel<GT>(acc, g, c, x, y, image, hit, tout);
+        }
     });
 }
 
