@@ -353,15 +353,23 @@ __global__ __launch_bounds__(kThreads) void k_bin_classify(GridView g, SlotTable
     for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
         const uint64_t c0 = uint64_t(c) * kChunk;
         uint32_t flags = 0;
+        // the chunk's points of this thread are loaded before the first test, so their loads
+        // overlap instead of each waiting for the previous test
+        float P[kClassifyItems][3];
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            const uint64_t i = c0 + uint64_t(j) * kThreads + threadIdx.x;
+            P[j][0] = P[j][1] = P[j][2] = 0.0f;
+            if (i < n) loadPoint(g, xyz, i, world, P[j]);
+        }
+#pragma unroll
         for (int j = 0; j < kClassifyItems; ++j) {
             const uint64_t i = c0 + uint64_t(j) * kThreads + threadIdx.x;
             if (i >= n) continue;
-            float p[3];
-            loadPoint(g, xyz, i, world, p);
             SlotBlock B;
             // FD6 samples that round into a farther cell read outside the block's slots: such
             // queries go through the sort with the others
-            if (touchesLeaf(T, bits, p, B) || !adjointInBlock(p, mode)) flags |= 1u << j;
+            if (touchesLeaf(T, bits, P[j], B) || !adjointInBlock(P[j], mode)) flags |= 1u << j;
         }
         flagsOut[uint64_t(c) * kThreads + threadIdx.x] = uint8_t(flags);
         uint32_t total;
@@ -402,13 +410,19 @@ __global__ __launch_bounds__(kThreads) void k_bin_compact(GridView g, BinLayout 
         }
         __syncthreads();
         const uint64_t below = (uint64_t(1) << lane) - 1u;
+        // the marked points are loaded before the first store (the loads overlap)
+        float P[kClassifyItems][3];
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            P[j][0] = P[j][1] = P[j][2] = 0.0f;
+            if ((flags >> j) & 1u) loadPoint(g, xyz, uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x, world, P[j]);
+        }
 #pragma unroll
         for (int j = 0; j < kClassifyItems; ++j) {
             if (!((flags >> j) & 1u)) continue;
             const uint32_t pos = runBase[j][wave] + uint32_t(__popcll(ball[j] & below));
             const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
-            float p[3];
-            loadPoint(g, xyz, i, world, p);
+            const float* p = P[j];
             keys[pos] = binOf(L, p);
             Pt t;
             t.x = p[0];
