@@ -488,11 +488,15 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
         brick[v] = leaf ? Codec<GT>::get(leaf, Accessor<GT>::leafOffset(gx, gy, gz), g.channel) : slotValue[slot];
     }
     __syncthreads();
-    // wave-uniform loop (the record store is a wave-wide operation): 64 consecutive queries per wave
-    for (uint32_t base = q0 + (threadIdx.x & ~63u); base < q1; base += kBinThreads) {
+    // wave-uniform loop (the record store is a wave-wide operation): 64 consecutive queries per wave;
+    // the next round's point is loaded before this round's evaluation
+    uint32_t base = q0 + (threadIdx.x & ~63u);
+    Pt next = sorted[base + (threadIdx.x & 63u) < q1 ? base + (threadIdx.x & 63u) : q0];
+    for (; base < q1; base += kBinThreads) {
         const uint32_t q = base + (threadIdx.x & 63u);
         bool valid = q < q1;
-        const Pt t = sorted[valid ? q : q0];
+        const Pt t = next;
+        if (base + kBinThreads < q1) next = sorted[q + kBinThreads < q1 ? q + kBinThreads : q0];
         const uint32_t id = t.id;
         const float p[3] = {t.x, t.y, t.z};
         const float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
