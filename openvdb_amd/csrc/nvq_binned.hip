@@ -622,6 +622,14 @@ __global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T
             ra[j] = marked ? rec[i].a : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             rb[j] = marked && grad ? rec[i].b.x : 0.0f;
         }
+        // and the points of the other queries, answered from the slot table
+        float P[kClassifyItems][3];
+#pragma unroll
+        for (int j = 0; j < kClassifyItems; ++j) {
+            const uint64_t i = uint64_t(c) * kChunk + uint64_t(j) * kThreads + threadIdx.x;
+            P[j][0] = P[j][1] = P[j][2] = 0.0f;
+            if (i < n && !((flags >> j) & 1u)) loadPoint(g, xyz, i, world, P[j]);
+        }
         // one store per output for every query of the chunk, whether its result comes from a
         // record or from the slot table: stores that covered the marked and the other queries of a
         // line separately wrote each output line twice
@@ -631,8 +639,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T
             if (i >= n) continue;
             float o[5] = {ra[j].x, ra[j].y, ra[j].z, ra[j].w, rb[j]};
             if (!((flags >> j) & 1u)) {
-                float p[3];
-                loadPoint(g, xyz, i, world, p);
+                const float* p = P[j];
                 SlotBlock B;
                 loadSlots(T, p, B);
                 const SampleOut r = sampleSlots(B, p, wantNN, wantVal, mode, k);
