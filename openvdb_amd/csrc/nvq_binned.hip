@@ -489,7 +489,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_sample(GridView g, BinLayou
     }
     __syncthreads();
     // wave-uniform loop (the record store is a wave-wide operation): 64 consecutive queries per wave;
-    // the next round's point is loaded before this round's evaluation
+    // the next iteration's point is loaded before this iteration's evaluation
     uint32_t base = q0 + (threadIdx.x & ~63u);
     Pt next = sorted[base + (threadIdx.x & 63u) < q1 ? base + (threadIdx.x & 63u) : q0];
     for (; base < q1; base += kBinThreads) {
