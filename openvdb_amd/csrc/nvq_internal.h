@@ -158,37 +158,70 @@ __device__ __forceinline__ float waveMax(float x)
 }
 
 constexpr int kOrderThreads = 1024, kOrderBatch = 16;
-namespace {
-// One workgroup: counting sort of the tile keys (histogram and scan in LDS), tiles of one key in
-// the order the workgroup's atomics hand out their slots.
-__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __restrict__ key, uint64_t tiles,
-                                                              uint32_t* __restrict__ order)
+// The counting sort runs over up to kOrderParts contiguous parts of the tile range (at least
+// kOrderPartMin tiles each), one workgroup each: part histograms in LDS, one workgroup turning
+// them into per-part offsets, then each part placing its tiles with LDS atomics.  For 262K tiles
+// it takes 30 us; one workgroup for the whole range took 161 us, its atomics on one key serialised.
+constexpr uint32_t kOrderParts = 128, kOrderPartMin = 8192;
+inline uint32_t orderParts(uint64_t tiles)
 {
-    __shared__ uint32_t count[kOrderKeys];
-    __shared__ uint32_t part[kOrderThreads];
-    constexpr int per = kOrderKeys / kOrderThreads;
-    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) count[k] = 0;
+    const uint64_t p = (tiles + kOrderPartMin - 1) / kOrderPartMin;
+    return uint32_t(p < 1 ? 1 : (p > kOrderParts ? kOrderParts : p));
+}
+// bytes of per-part counts the sort needs besides the keys and the order
+inline size_t orderScratchBytes(uint64_t tiles) { return size_t(orderParts(tiles)) * kOrderKeys * 4; }
+
+namespace {
+// Part w = blockIdx.x covers tiles [w * per, min((w + 1) * per, tiles)); its key counts go to
+// count[w * kOrderKeys + k].
+__global__ __launch_bounds__(kOrderThreads) void k_order_hist(const uint32_t* __restrict__ key, uint64_t tiles,
+                                                              uint64_t per, uint32_t* __restrict__ count)
+{
+    __shared__ uint32_t c[kOrderKeys];
+    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) c[k] = 0;
     __syncthreads();
+    const uint64_t lo = uint64_t(blockIdx.x) * per, hi = lo + per < tiles ? lo + per : tiles;
     // kOrderBatch keys per thread are loaded before their atomics, so each thread keeps that many
     // loads in flight
-    for (uint64_t t0 = 0; t0 < tiles; t0 += uint64_t(kOrderThreads) * kOrderBatch) {
+    for (uint64_t t0 = lo; t0 < hi; t0 += uint64_t(kOrderThreads) * kOrderBatch) {
         uint32_t k[kOrderBatch];
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j) {
             const uint64_t t = t0 + uint64_t(j) * kOrderThreads + threadIdx.x;
-            k[j] = t < tiles ? key[t] : kOrderKeys;
+            k[j] = t < hi ? key[t] : kOrderKeys;
         }
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j)
-            if (k[j] < kOrderKeys) atomicAdd(&count[k[j]], 1u);
+            if (k[j] < kOrderKeys) atomicAdd(&c[k[j]], 1u);
     }
     __syncthreads();
-    uint32_t local[per], sum = 0;
+    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) count[uint64_t(blockIdx.x) * kOrderKeys + k] = c[k];
+}
+
+// One workgroup: the counts of `parts` parts become each part's first slot per key, in place (key
+// order first, then part order within a key).  Thread x owns keys 4x..4x+3 and reads them as one
+// 16-byte word per part, 16 parts in flight at a time.
+__global__ __launch_bounds__(kOrderThreads) void k_order_offsets(uint32_t* __restrict__ count, uint32_t parts)
+{
+    static_assert(kOrderKeys == 4 * kOrderThreads, "four keys per thread");
+    constexpr uint32_t kBatch = 16;
+    __shared__ uint32_t part[kOrderThreads];
+    uint4* rows = reinterpret_cast<uint4*>(count); // row w: kOrderThreads words of four keys
+    uint32_t tot[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t w0 = 0; w0 < parts; w0 += kBatch) {
+        uint4 c[kBatch];
 #pragma unroll
-    for (int j = 0; j < per; ++j) {
-        local[j] = sum;
-        sum += count[threadIdx.x * per + j];
+        for (uint32_t b = 0; b < kBatch; ++b)
+            c[b] = w0 + b < parts ? rows[uint64_t(w0 + b) * kOrderThreads + threadIdx.x] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            tot[0] += c[b].x;
+            tot[1] += c[b].y;
+            tot[2] += c[b].z;
+            tot[3] += c[b].w;
+        }
     }
+    const uint32_t sum = tot[0] + tot[1] + tot[2] + tot[3];
     part[threadIdx.x] = sum;
     __syncthreads();
     for (int off = 1; off < kOrderThreads; off <<= 1) { // inclusive scan of the per-thread sums
@@ -197,24 +230,61 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint32_t* __
         part[threadIdx.x] += v;
         __syncthreads();
     }
-    const uint32_t base = part[threadIdx.x] - sum;
+    uint32_t run[4];
+    run[0] = part[threadIdx.x] - sum;
+    run[1] = run[0] + tot[0];
+    run[2] = run[1] + tot[1];
+    run[3] = run[2] + tot[2];
+    for (uint32_t w0 = 0; w0 < parts; w0 += kBatch) {
+        uint4 c[kBatch];
 #pragma unroll
-    for (int j = 0; j < per; ++j) count[threadIdx.x * per + j] = base + local[j];
+        for (uint32_t b = 0; b < kBatch; ++b)
+            c[b] = w0 + b < parts ? rows[uint64_t(w0 + b) * kOrderThreads + threadIdx.x] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            if (w0 + b >= parts) continue;
+            rows[uint64_t(w0 + b) * kOrderThreads + threadIdx.x] = make_uint4(run[0], run[1], run[2], run[3]);
+            run[0] += c[b].x;
+            run[1] += c[b].y;
+            run[2] += c[b].z;
+            run[3] += c[b].w;
+        }
+    }
+}
+
+// Part w places its tiles from its first slots, in the order its atomics hand them out.
+__global__ __launch_bounds__(kOrderThreads) void k_order_place(const uint32_t* __restrict__ key, uint64_t tiles,
+                                                               uint64_t per, const uint32_t* __restrict__ first,
+                                                               uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t slot[kOrderKeys];
+    for (int k = threadIdx.x; k < int(kOrderKeys); k += kOrderThreads) slot[k] = first[uint64_t(blockIdx.x) * kOrderKeys + k];
     __syncthreads();
-    for (uint64_t t0 = 0; t0 < tiles; t0 += uint64_t(kOrderThreads) * kOrderBatch) {
+    const uint64_t lo = uint64_t(blockIdx.x) * per, hi = lo + per < tiles ? lo + per : tiles;
+    for (uint64_t t0 = lo; t0 < hi; t0 += uint64_t(kOrderThreads) * kOrderBatch) {
         uint32_t k[kOrderBatch];
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j) {
             const uint64_t t = t0 + uint64_t(j) * kOrderThreads + threadIdx.x;
-            k[j] = t < tiles ? key[t] : kOrderKeys;
+            k[j] = t < hi ? key[t] : kOrderKeys;
         }
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j)
-            if (k[j] < kOrderKeys) order[atomicAdd(&count[k[j]], 1u)] = uint32_t(t0 + uint64_t(j) * kOrderThreads + threadIdx.x);
+            if (k[j] < kOrderKeys) order[atomicAdd(&slot[k[j]], 1u)] = uint32_t(t0 + uint64_t(j) * kOrderThreads + threadIdx.x);
     }
 }
-
 } // namespace
+
+// Queues the counting sort of `tiles` keys (< kOrderKeys each) into `order`, with `count` of
+// orderScratchBytes(tiles) bytes.
+inline void launchTileOrder(const uint32_t* key, uint64_t tiles, uint32_t* order, uint32_t* count, hipStream_t s)
+{
+    const uint32_t parts = orderParts(tiles);
+    const uint64_t per = (tiles + parts - 1) / parts;
+    hipLaunchKernelGGL(k_order_hist, dim3(parts), dim3(kOrderThreads), 0, s, key, tiles, per, count);
+    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(kOrderThreads), 0, s, count, parts);
+    hipLaunchKernelGGL(k_order_place, dim3(parts), dim3(kOrderThreads), 0, s, key, tiles, per, count, order);
+}
 
 // Per-call scratch of the query entry points is leased from one grow-only block per (device,
 // stream) that stays allocated between calls: on this runtime hipFreeAsync waits for the stream's

@@ -460,12 +460,13 @@ int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const 
     const bool ordered = s.tiles > kOrderMinTiles;
     const size_t keyBytes = ordered ? (size_t(s.tiles) * 4 + 255) & ~size_t(255) : 0;
     const size_t orderBytes = keyBytes;
+    const size_t countBytes = ordered ? orderScratchBytes(s.tiles) : 0;
     char* base = nullptr;
-    if (keyBytes + orderBytes + extraBytes) {
-        base = out.lease.acquire(keyBytes + orderBytes + extraBytes, st);
+    if (keyBytes + orderBytes + countBytes + extraBytes) {
+        base = out.lease.acquire(keyBytes + orderBytes + countBytes + extraBytes, st);
         if (!base) return -1;
     }
-    out.extra = extraBytes ? base + keyBytes + orderBytes : nullptr;
+    out.extra = extraBytes ? base + keyBytes + orderBytes + countBytes : nullptr;
     if (!ordered) {
         out.order = nullptr;
         return 0;
@@ -475,7 +476,7 @@ int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const 
     {
         ProfScope prof("k_tile_order", st);
         hipLaunchKernelGGL(k_tile_cost, dim3(orderedBlocks(s)), dim3(kBlock), 0, st, g, eye, dir, s, key);
-        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, st, key, s.tiles, order);
+        launchTileOrder(key, s.tiles, order, reinterpret_cast<uint32_t*>(base + keyBytes + orderBytes), st);
     }
     out.order = order;
     return hipGetLastError() == hipSuccess ? 0 : -1;

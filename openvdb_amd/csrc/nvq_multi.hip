@@ -144,8 +144,9 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     // none up to kOrderMinTiles tiles, which are all resident at once)
     const bool ordered = A.totalTiles > kOrderMinTiles;
     const size_t keyBytes = ordered ? (size_t(A.totalTiles) * 4 + 255) & ~size_t(255) : 0;
+    const size_t countBytes = ordered ? orderScratchBytes(A.totalTiles) : 0;
     WorkspaceLease lease;
-    char* base = lease.acquire(kQueueBytes + 2 * keyBytes, s);
+    char* base = lease.acquire(kQueueBytes + 2 * keyBytes + countBytes, s);
     if (!base) return -2;
     uint32_t* counter = reinterpret_cast<uint32_t*>(base);
     uint32_t* key = reinterpret_cast<uint32_t*>(base + kQueueBytes);
@@ -155,7 +156,8 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
         ProfScope prof("k_tile_order", s);
         hipLaunchKernelGGL(k_multi_tile_cost, dim3((A.totalTiles + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
                            s, A, eye, dir, key);
-        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, key, uint64_t(A.totalTiles), order);
+        launchTileOrder(key, uint64_t(A.totalTiles), order,
+                        reinterpret_cast<uint32_t*>(base + kQueueBytes + 2 * keyBytes), s);
     }
     {
         ProfScope prof("k_multi_query", s);
