@@ -474,7 +474,7 @@ int nvq::tileOrder(const GridView& g, const float* eye, const float* dir, const 
     uint32_t* key = reinterpret_cast<uint32_t*>(base);
     uint32_t* order = reinterpret_cast<uint32_t*>(base + keyBytes);
     {
-        ProfScope prof("k_tile_order", st);
+        ProfScope prof("tile_order", st);
         hipLaunchKernelGGL(k_tile_cost, dim3(orderedBlocks(s)), dim3(kBlock), 0, st, g, eye, dir, s, key);
         launchTileOrder(key, s.tiles, order, reinterpret_cast<uint32_t*>(base + keyBytes + orderBytes), st);
     }

@@ -153,7 +153,7 @@ int multiQuery(const MultiArgs& A, const float* eye, const float* dir, float dt,
     uint32_t* order = ordered ? reinterpret_cast<uint32_t*>(base + kQueueBytes + keyBytes) : nullptr;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(kQueueBands), 0, s, counter);
     if (ordered) {
-        ProfScope prof("k_tile_order", s);
+        ProfScope prof("tile_order", s);
         hipLaunchKernelGGL(k_multi_tile_cost, dim3((A.totalTiles + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0,
                            s, A, eye, dir, key);
         launchTileOrder(key, uint64_t(A.totalTiles), order,
