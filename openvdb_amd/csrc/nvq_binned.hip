@@ -603,7 +603,9 @@ __device__ __forceinline__ void storeGrad(float* __restrict__ grad, uint64_t i, 
 
 // Outputs of a classified call, in the classify pass's chunk order: a marked query copies its record,
 // the others are answered from their slot values (sampleSlots, bit-identical to the lane accessor).
-__global__ __launch_bounds__(kThreads) void k_bin_finish(GridView g, SlotTable T, const float* __restrict__ xyz,
+// 146 VGPRs uncapped (3 wavefronts per SIMD); capped at 4 wavefronts (128, a 60-byte spill) it runs
+// 1.30 -> 1.21 ms per 2^26-query call; at 5 (96, 208 bytes) 1.81 ms.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bin_finish(GridView g, SlotTable T, const float* __restrict__ xyz,
                                                          uint32_t n, uint32_t chunks, int world, bool wantNN,
                                                          bool wantVal, int mode, SampleConsts k,
                                                          const uint8_t* __restrict__ flagsIn,
