@@ -116,7 +116,7 @@ int nvq_get_value(const nvq_grid* grid, const int32_t* ijk, uint64_t n, float* v
  *          grad_mode 2 = GradStencil::gradient at RoundDown(xyz)  math/Stencils.h:502-507
  *          grad_mode 3 = TrilinearSampler::gradient (index space) math/SampleFromVoxels.h:232-271
  * nn / val / grad may be NULL.  Replaces the nanobind op sampleFromVoxels(points, d_grid, values[, gradients]).
- * Batches of 2^20 points or more are binned through the stream's scratch block (about 120 bytes
+ * Batches of 2^20 points or more are binned through the stream's scratch block (about 110 bytes
  * per point) and wait for `stream` once (to size the sort of the points that touch a leaf).
  * Smaller batches do not wait and take no scratch. */
 int nvq_sample(const nvq_grid* grid, const float* xyz, uint64_t n, int world, float* nn, float* val, float* grad,

@@ -846,6 +846,7 @@ struct Bins {
     uint32_t m = 0; // queries in the sort (all n, or those the forward's classify pass kept)
     WorkspaceLease lease;
     char* base = nullptr;
+    // keys/idx: the unsorted (key, point) pairs; after the sort, keys2/idx2 name the sorted ones
     uint32_t *keys = nullptr, *keys2 = nullptr, *begin = nullptr, *end = nullptr, *spillCount = nullptr,
              *flag = nullptr, *spill = nullptr;
     uint32_t chunks = 0;                                 // classify chunks (forward only; 0: not classified)
@@ -904,11 +905,15 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
     if (!makeLayout(g, B.L)) return -1;
     const uint32_t m = B.L.nbins;
     B.n = n;
+    // The sort ping-pongs between keys/keys2 and idx/idx2, so its own temporaries are only its
+    // histograms and look-back state (the form with separate outputs adds a 20-byte-per-point buffer).
     size_t sortBytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, static_cast<uint32_t*>(nullptr),
-                                           static_cast<uint32_t*>(nullptr), static_cast<Pt*>(nullptr),
-                                           static_cast<Pt*>(nullptr), n, 0, int(B.L.bits), s) != hipSuccess)
-        return -4;
+    {
+        hipcub::DoubleBuffer<uint32_t> dk(nullptr, nullptr);
+        hipcub::DoubleBuffer<Pt> dv(nullptr, nullptr);
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, dk, dv, n, 0, int(B.L.bits), s) != hipSuccess)
+            return -4;
+    }
     SlotTable T{};
     if (cls && !slotBox(B.L, T)) cls = nullptr; // too large a region for the slot table: sort every query
     const uint64_t slots = cls ? uint64_t(T.n[0]) * T.n[1] * T.n[2] : 0;
@@ -989,8 +994,12 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
     hipError_t sorted;
     {
         ProfScope prof("bin_sort", s);
-        sorted = hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, B.keys, B.keys2, B.idx, B.idx2, B.m, 0,
-                                                    int(B.L.bits), s);
+        hipcub::DoubleBuffer<uint32_t> dk(B.keys, B.keys2);
+        hipcub::DoubleBuffer<Pt> dv(B.idx, B.idx2);
+        sorted = hipcub::DeviceRadixSort::SortPairs(temp, sortBytes, dk, dv, B.m, 0, int(B.L.bits), s);
+        // the sorted pairs end in either buffer, depending on the number of digit passes
+        B.keys2 = dk.Current();
+        B.idx2 = dv.Current();
     }
     if (sorted != hipSuccess) {
         // the sorted arrays were never written: launch nothing that reads them
