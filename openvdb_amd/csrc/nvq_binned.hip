@@ -435,14 +435,33 @@ __global__ __launch_bounds__(kThreads) void k_bin_compact(GridView g, BinLayout 
     }
 }
 
+// Four consecutive sorted keys per thread: one 16-byte load (`keys` is 256-byte aligned) and the
+// keys on either side, so all three loads are in flight at once.  Bin keys are below the bin
+// count, so ~0u stands for "no key" before the first and after the last.
 __global__ __launch_bounds__(kThreads) void k_bin_bounds(const uint32_t* __restrict__ keys, uint32_t n,
                                                          uint32_t* __restrict__ begin, uint32_t* __restrict__ end)
 {
-    const uint32_t stride = gridDim.x * kThreads;
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-        const uint32_t k = keys[i];
-        if (i == 0 || keys[i - 1] != k) begin[k] = i;
-        if (i == n - 1 || keys[i + 1] != k) end[k] = i + 1;
+    const uint64_t i0 = (uint64_t(blockIdx.x) * kThreads + threadIdx.x) * 4u;
+    if (i0 >= n) return;
+    uint32_t k[6]; // keys[i0 - 1 .. i0 + 4]
+    if (i0 + 4u <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+        k[1] = v.x;
+        k[2] = v.y;
+        k[3] = v.z;
+        k[4] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : ~0u;
+    }
+    k[0] = i0 ? keys[i0 - 1] : ~0u;
+    k[5] = i0 + 4u < n ? keys[i0 + 4] : ~0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = uint32_t(i0) + uint32_t(j);
+        if (i >= n) break;
+        if (k[j] != k[j + 1]) begin[k[j + 1]] = i;
+        if (k[j + 2] != k[j + 1]) end[k[j + 1]] = i + 1;
     }
 }
 
@@ -1006,7 +1025,8 @@ int binPoints(const GridView& g, const float* xyz, uint32_t n, int world, bool w
         (void)hipGetLastError();
         return -4;
     }
-    const uint32_t gridM = std::min<uint32_t>((B.m + kThreads - 1) / kThreads, 1u << 16);
+    // four keys per thread, one pass over the grid
+    const uint32_t gridM = uint32_t((uint64_t(B.m) + 4u * kThreads - 1) / (4u * kThreads));
     hipLaunchKernelGGL(k_bin_bounds, dim3(gridM), dim3(kThreads), 0, s, B.keys2, B.m, B.begin, B.end);
     return 0;
 }
