@@ -155,9 +155,10 @@ struct SlotTable {
     const uint32_t* leaf; // one bit per slot (bit i%32 of word i/32): a leaf holds the slot
 };
 
-// Bitmaps of at most this many 32-bit words are staged in LDS by the classify pass (256K slots, the
-// C2 grids need 6-8K words); larger ones are read from global memory.
-constexpr uint32_t kLdsLeafWords = 8192;
+// Bitmaps of at most this many 32-bit words (48 KB, 384K slots) are staged in LDS by the classify
+// pass, which takes exactly the bitmap's size of dynamic LDS; larger ones are read from global
+// memory.  The torus (300, 80) needs 6.5K words, the 512^3 fog 10.3K.
+constexpr uint32_t kLdsLeafWords = 12288;
 
 // One thread per slot; the bitmap words are assembled with a wave ballot (64 slots per wave, so
 // every word is written whole by one lane).  The launch covers whole waves: count rounded up to 64.
@@ -339,14 +340,14 @@ __device__ __forceinline__ uint32_t blockPrefix(uint32_t cnt, uint32_t* waveSum,
 // wave and lane order.  The others are answered from their slot values by k_bin_finish.
 __global__ __launch_bounds__(kThreads) void k_bin_classify(GridView g, SlotTable T, const float* __restrict__ xyz,
                                                            uint32_t n, uint32_t chunks, int world, int mode,
-                                                           uint8_t* __restrict__ flagsOut, uint32_t* __restrict__ counts)
+                                                           uint32_t ldsWords, uint8_t* __restrict__ flagsOut,
+                                                           uint32_t* __restrict__ counts)
 {
     __shared__ uint32_t waveSum[kThreads / 64];
-    __shared__ uint32_t ldsLeaf[kLdsLeafWords];
-    const uint64_t words = (uint64_t(T.n[0]) * T.n[1] * T.n[2] + 31) / 32;
+    extern __shared__ uint32_t ldsLeaf[]; // ldsWords words: the whole bitmap, or none
     const uint32_t* bits = T.leaf;
-    if (words <= kLdsLeafWords) {
-        for (uint32_t w = threadIdx.x; w < uint32_t(words); w += kThreads) ldsLeaf[w] = T.leaf[w];
+    if (ldsWords) {
+        for (uint32_t w = threadIdx.x; w < ldsWords; w += kThreads) ldsLeaf[w] = T.leaf[w];
         __syncthreads();
         bits = ldsLeaf;
     }
@@ -908,8 +909,10 @@ void launchClassify(const GridView& g, const Bins& B, const SlotTable& T, float*
     hipLaunchKernelGGL(k_slot_table<GT>, dim3(uint32_t(std::min<uint64_t>((count + kThreads - 1) / kThreads, 1u << 16))),
                        dim3(kThreads), 0, s, g, T, tv, tl);
     ProfScope prof("k_bin_classify", s);
-    hipLaunchKernelGGL(k_bin_classify, dim3(blocks), dim3(kThreads), 0, s, g, T, xyz, B.n, B.chunks, world, c.mode,
-                       B.cflags, B.counts);
+    const uint64_t words = (count + 31) / 32;
+    const uint32_t ldsWords = words <= kLdsLeafWords ? uint32_t(words) : 0u;
+    hipLaunchKernelGGL(k_bin_classify, dim3(blocks), dim3(kThreads), size_t(ldsWords) * 4u, s, g, T, xyz, B.n, B.chunks,
+                       world, c.mode, ldsWords, B.cflags, B.counts);
 }
 
 // Allocate, key, sort and bound the bins of n points.  With `cls`, only the queries whose block
